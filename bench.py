@@ -1,0 +1,196 @@
+"""Benchmark: PHJ-PURE hash join, 2^27 build rows x 2^30 probe rows (uint64 keys),
+build payloads u32+u64, probe payloads u64+u128, 50% of probe rows match,
+radix 2 passes x 256 partitions, load factor 0.5, on one MI355X.
+
+One step = one full runPHJ (build shuffle, probe shuffle, per-leaf build and
+probe, output materialised in HBM) over inputs already resident in HBM.
+Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--scale-log2 0] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "phj-bench_amd"))
+
+import phj_gpu as G  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "join input tuples/s & % HBM peak, 2^27⋈2^30 u64; batched vs plain speedup"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def widths(types):
+    return sum(G.TYPE_SIZE[t] for t in types)
+
+
+def cpu_baseline(scale_log2, threads):
+    """The compiled reference's runPHJ (oracle/_ref) on a bounded sample of the
+    same workload, on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    if not O.ref_available():
+        return None
+    rb, rp = 22 - scale_log2, 25 - scale_log2
+    spec = O.make_spec(seed=42, build_rows=1 << rb, probe_rows=1 << rp, distribution=0, match_rate=0.5,
+                       build_types=(G.U32, G.U64), probe_types=(G.U64, G.U128))
+    b, p = O.gen(spec, threads=threads)
+    O.ref_run(3, b, p, (8, 8), threads=threads, want_checksum=False)  # warm-up
+    runs = []
+    for _ in range(2):
+        r = O.ref_run(3, b, p, (8, 8), threads=threads, want_checksum=False)
+        runs.append(r.e2e_wall_ms)
+    ms = min(runs)
+    return {"value": (b.rows + p.rows) / (ms / 1e3), "unit": "tuples/s", "cores": threads, "kind": "reference",
+            "sample": f"reference runPHJ (oracle/_ref, its own e2e_wall_ms), 2^{rb} x 2^{rp} rows of the same "
+                      f"distribution and schema, pass_bits {{8,8}}, best of 2 after 1 warm-up"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale-log2", type=int, default=0, help="shrink both sides by 2^k (testing only)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the join on a single MI355X; "
+              "multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    rb, rp = 27 - args.scale_log2, 30 - args.scale_log2
+    R, S = 1 << rb, 1 << rp
+    btypes, ptypes = (G.U32, G.U64), (G.U64, G.U128)
+    pass_bits = [8, 8]
+    spec_kw = dict(seed=42, build_rows=R, probe_rows=S, distribution=G.DIST_UNIFORM, match_rate=0.5,
+                   build_types=btypes, probe_types=ptypes)
+    spec = G.gen_spec(**spec_kw)
+    ctx = G.Context(0)
+    log(f"generating build 2^{rb} and probe 2^{rp} rows on the device")
+    build, probe = ctx.generate(spec)
+    cfg = G.RadixConfig(pass_bits)
+
+    for w in range(args.warmup):
+        r = ctx.runPHJ(build, probe, cfg)
+        log(f"warmup {w}: e2e {r.e2e_wall_ms:.2f} ms out_rows {r.out_rows}")
+        r.release()
+
+    ctx.set_kernel_timing(True)
+    ctx.reset_kernel_times()
+    ctx.synchronize()
+    e2e, phases = [], []
+    last = None
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        if last is not None:
+            last.release()
+        last = ctx.runPHJ(build, probe, cfg)
+        e2e.append(last.e2e_wall_ms)
+        phases.append((last.build_shuffle.wall_ms, last.probe_shuffle.wall_ms, last.build.wall_ms, last.probe.wall_ms))
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ktimes = ctx.kernel_times()
+    ctx.set_kernel_timing(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step (wall), e2e events {['%.2f' % x for x in e2e]}")
+
+    out_rows = last.out_rows
+    got = last.checksum()
+    want = ctx.expected_checksum(spec)
+    parity = got == want
+    log(f"checksum gpu={got} closed-form={want} parity={parity}")
+    last.release()
+
+    wR, wS = 8 + widths(btypes), 8 + widths(ptypes)
+    wO = 8 + widths(btypes) + widths(ptypes)
+    b_alg = R * wR + S * wS + out_rows * wO
+    tuples_per_s = (R + S) / (ms_per_step / 1e3)
+    best_e2e = min(e2e)
+
+    # Dominant kernel and its algorithmic bytes per launch.
+    n = args.steps
+    per_kernel_bytes = {
+        # two passes per relation: every pass reads and writes each row once
+        "radix_scatter": 2 * (R * wR + S * wS) * 2 / (4 * n) * n,
+        # leaf join: build rows and probe rows read once, output written once
+        "leaf_join": R * wR + S * wS + out_rows * wO,
+        # histogram: keys read once per pass
+        "radix_hist": 2 * 8 * (R + S) / 4,
+    }
+    dom = max(ktimes, key=lambda k: ktimes[k][0]) if ktimes else None
+    roofline = None
+    if dom in per_kernel_bytes:
+        tot_ms, launches = ktimes[dom]
+        avg_ms = tot_ms / launches
+        if dom == "radix_scatter":
+            bytes_per_launch = 2 * (R * wR + S * wS) * 2 / 4  # 4 launches per step: R, S x 2 passes (mean)
+        elif dom == "leaf_join":
+            bytes_per_launch = per_kernel_bytes["leaf_join"]
+        else:
+            bytes_per_launch = per_kernel_bytes["radix_hist"]
+        achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tf) and args.scale_log2 == 0:
+            traffic = json.load(open(tf)).get(dom)
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": traffic, "kernel": dom,
+                    "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        try:
+            log(f"CPU baseline: compiled reference runPHJ on {threads} threads")
+            cpu = cpu_baseline(args.scale_log2, threads)
+        except Exception as e:  # reported, not fatal
+            log(f"CPU baseline failed: {e}")
+
+    line = {
+        "metric": METRIC,
+        "value": tuples_per_s,
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (device-generated, seed 42)",
+        "config": {"workload": f"PHJ-PURE 2^{rb} x 2^{rp}, build u32+u64, probe u64+u128, 50% match, "
+                               f"pass_bits {{8,8}}, LF 0.5, 16-byte cells",
+                   "build_rows": R, "probe_rows": S, "pass_bits": pass_bits, "match_rate": 0.5},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "e2e_ms_best": best_e2e,
+        "e2e_roofline_frac": b_alg / (best_e2e / 1e3) / HBM_PEAK,
+        "step_roofline_frac": b_alg / (ms_per_step / 1e3) / HBM_PEAK,
+        "algorithmic_bytes": b_alg,
+        "out_rows": out_rows,
+        "parity_checksum": parity,
+        "phase_ms_last": dict(zip(("build_shuffle", "probe_shuffle", "build", "probe"), phases[-1])),
+        "kernel_ms_per_step": {k: round(v[0] / n, 4) for k, v in sorted(ktimes.items())},
+    }
+    print(json.dumps(line), flush=True)
+    build.release()
+    probe.release()
+    ctx.close()
+    if not parity:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

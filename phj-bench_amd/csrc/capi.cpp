@@ -1,0 +1,1473 @@
+// capi.cpp — the C ABI of libphj_hip.so: context, device arena, relation
+// staging, and the host orchestration of the three schemes.
+//
+// Scheme orchestration follows the reference's phase structure:
+//   runPHJ    (src/PHJ.cpp:19-150):    build shuffle -> probe shuffle -> per-leaf build+probe
+//   runCHJ    (src/CHJ.cpp:16-119):    sub-table build -> probe
+//   runPhjBep (src/PHJBep.cpp:113-790): build shuffle -> bounded-budget probe partitioning
+//                                       with per-leaf join as buffers fill -> final drain
+// Every kernel is launched on the context's one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/phj_gpu.h"
+#include "gen.h"
+#include "kernels.h"
+
+using namespace phjd;
+
+namespace {
+
+struct PhjError : std::runtime_error {
+    phj_status code;
+    PhjError(phj_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (expr);                                                                                        \
+        if (e_ != hipSuccess)                                                                                          \
+            throw PhjError(e_ == hipErrorOutOfMemory ? PHJ_ERR_OUT_OF_MEMORY : PHJ_ERR_HIP,                           \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));                                         \
+    } while (0)
+
+void invalid(const std::string &m) { throw PhjError(PHJ_ERR_INVALID_ARGUMENT, m); }
+
+// ---------------------------------------------------------------- device arena
+// Caching allocator: freed blocks are kept per rounded size and reused by
+// later allocations on the same stream (stream order makes reuse safe).
+class Arena {
+public:
+    ~Arena() { release_cached(); }
+
+    void *alloc(uint64_t bytes) {
+        const uint64_t sz = round(bytes);
+        auto it = free_.find(sz);
+        void *p = nullptr;
+        if (it != free_.end()) {
+            p = it->second;
+            free_.erase(it);
+            cached_ -= sz;
+        } else {
+            hipError_t e = hipMalloc(&p, sz);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                release_cached();
+                e = hipMalloc(&p, sz);
+                if (e != hipSuccess) {
+                    (void)hipGetLastError();
+                    throw PhjError(PHJ_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(sz) + " bytes failed");
+                }
+            }
+        }
+        live_[p] = sz;
+        current_ += sz;
+        peak_ = std::max(peak_, current_);
+        return p;
+    }
+
+    void free(void *p) {
+        if (p == nullptr)
+            return;
+        auto it = live_.find(p);
+        if (it == live_.end())
+            throw PhjError(PHJ_ERR_INVALID_ARGUMENT, "pointer not allocated by this context");
+        current_ -= it->second;
+        cached_ += it->second;
+        free_.emplace(it->second, p);
+        live_.erase(it);
+    }
+
+    bool owns(const void *p) const { return live_.count(const_cast<void *>(p)) != 0; }
+
+    void release_cached() {
+        if (free_.empty())
+            return;
+        (void)hipDeviceSynchronize();
+        for (auto &kv : free_)
+            (void)hipFree(kv.second);
+        free_.clear();
+        cached_ = 0;
+    }
+
+    uint64_t current() const { return current_; }
+    uint64_t peak() const { return peak_; }
+    void reset_peak() { peak_ = current_; }
+
+private:
+    static uint64_t round(uint64_t b) {
+        if (b == 0)
+            b = 1;
+        if (b >= (2ull << 20))
+            return (b + (2ull << 20) - 1) & ~((2ull << 20) - 1);
+        uint64_t p = 256;
+        while (p < b)
+            p <<= 1;
+        return p;
+    }
+    std::multimap<uint64_t, void *> free_;
+    std::unordered_map<void *, uint64_t> live_;
+    uint64_t current_ = 0, peak_ = 0, cached_ = 0;
+};
+
+// ---------------------------------------------------------------- kernel timer
+class EventTimer : public KernelTimer {
+public:
+    ~EventTimer() override {
+        for (auto e : pool_)
+            (void)hipEventDestroy(e);
+    }
+    void begin(const char *name, hipStream_t s) override {
+        hipEvent_t a = get();
+        (void)hipEventRecord(a, s);
+        open_.push_back({name, a, nullptr});
+    }
+    void end(hipStream_t s) override {
+        hipEvent_t b = get();
+        (void)hipEventRecord(b, s);
+        open_.back().b = b;
+        pending_.push_back(open_.back());
+        open_.pop_back();
+    }
+    // After the stream has synchronised.
+    void resolve() {
+        for (auto &p : pending_) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                auto &t = totals_[p.name];
+                t.first += ms;
+                t.second += 1;
+            }
+            free_.push_back(p.a);
+            free_.push_back(p.b);
+        }
+        pending_.clear();
+    }
+    void reset() { totals_.clear(); }
+    bool get_total(const std::string &n, double &ms, uint64_t &cnt) const {
+        auto it = totals_.find(n);
+        if (it == totals_.end())
+            return false;
+        ms = it->second.first;
+        cnt = it->second.second;
+        return true;
+    }
+    std::string names() const {
+        std::string s;
+        for (auto &kv : totals_) {
+            if (!s.empty())
+                s += ",";
+            s += kv.first;
+        }
+        return s;
+    }
+
+private:
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    hipEvent_t get() {
+        if (!free_.empty()) {
+            hipEvent_t e = free_.back();
+            free_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        pool_.push_back(e);
+        return e;
+    }
+    std::vector<hipEvent_t> pool_, free_;
+    std::vector<Rec> open_, pending_;
+    std::map<std::string, std::pair<double, uint64_t>> totals_;
+};
+
+} // namespace
+
+struct phj_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Arena arena;
+    EventTimer timer;
+    bool timing = false;
+    std::unordered_map<uint64_t, std::vector<void *>> owned; // handle -> buffers
+    uint64_t next_handle = 1;
+
+    KernelTimer *tm() { return timing ? &timer : nullptr; }
+    void sync() {
+        HIPCHK(hipGetLastError()); // launch failures (bad configuration, LDS size)
+        HIPCHK(hipStreamSynchronize(stream));
+        if (timing)
+            timer.resolve();
+    }
+    template <class T> T *alloc(uint64_t n) { return static_cast<T *>(arena.alloc(n * sizeof(T))); }
+};
+
+namespace {
+
+// Buffers freed when the scope ends (returned to the arena cache).
+struct Scratch {
+    phj_ctx *ctx;
+    std::vector<void *> bufs;
+    explicit Scratch(phj_ctx *c) : ctx(c) {}
+    ~Scratch() {
+        for (void *p : bufs)
+            ctx->arena.free(p);
+    }
+    template <class T> T *get(uint64_t n) {
+        T *p = ctx->alloc<T>(std::max<uint64_t>(n, 1));
+        bufs.push_back(p);
+        return p;
+    }
+    void release(void *p) {
+        auto it = std::find(bufs.begin(), bufs.end(), p);
+        if (it != bufs.end()) {
+            ctx->arena.free(p);
+            bufs.erase(it);
+        }
+    }
+};
+
+uint32_t tsize(uint32_t t) {
+    if (t > 4)
+        invalid("payload type code must be 0..4");
+    return type_size(t);
+}
+
+// A relation resident on the device (copied from the host when needed).
+struct DevRel {
+    uint64_t rows = 0;
+    const uint64_t *keys = nullptr;
+    ColSet cols{};
+};
+
+DevRel stage_relation(phj_ctx *ctx, Scratch &scr, const phj_relation &r) {
+    if (r.n_cols > PHJ_MAX_COLS)
+        invalid("at most 8 payload columns");
+    if (r.rows > 0xFFFFFFFEull)
+        invalid("relations are limited to 2^32-2 rows");
+    if (r.rows > 0 && r.keys == nullptr)
+        invalid("null key column");
+    DevRel d;
+    d.rows = r.rows;
+    d.cols.n = r.n_cols;
+    for (uint32_t c = 0; c < r.n_cols; ++c) {
+        d.cols.types[c] = r.cols[c].type;
+        tsize(r.cols[c].type);
+        if (r.rows > 0 && r.cols[c].data == nullptr)
+            invalid("null payload column");
+    }
+    if (r.on_device) {
+        d.keys = r.keys;
+        for (uint32_t c = 0; c < r.n_cols; ++c)
+            d.cols.ptr[c] = r.cols[c].data;
+        return d;
+    }
+    uint64_t *k = scr.get<uint64_t>(r.rows);
+    if (r.rows)
+        HIPCHK(hipMemcpyAsync(k, r.keys, r.rows * 8, hipMemcpyHostToDevice, ctx->stream));
+    d.keys = k;
+    for (uint32_t c = 0; c < r.n_cols; ++c) {
+        const uint64_t bytes = r.rows * tsize(r.cols[c].type);
+        void *p = scr.get<uint8_t>(bytes);
+        if (bytes)
+            HIPCHK(hipMemcpyAsync(p, r.cols[c].data, bytes, hipMemcpyHostToDevice, ctx->stream));
+        d.cols.ptr[c] = p;
+    }
+    return d;
+}
+
+// src/RadixPartition.cpp:554-560 and src/CLI.cpp:126,350-355.
+uint32_t validate_pass_bits(const uint8_t *pass_bits, uint32_t n) {
+    if (n == 0 || pass_bits == nullptr)
+        invalid("radixShuffle: at least one pass required");
+    if (n > PHJ_MAX_PASSES)
+        invalid("at most 8 passes");
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (pass_bits[i] == 0 || pass_bits[i] > 16)
+            invalid("partition bits per pass must be in [1, 16]");
+        total += pass_bits[i];
+    }
+    if (total == 0 || total > 32)
+        invalid("radixShuffle: total partition bits must be in [1, 32]");
+    return total;
+}
+
+// Device pass plan: the leaves depend only on the total bit count (the top
+// `total` hash bits, consumed top-down), so the device splits them into passes
+// of at most 8 bits, each as even as possible.
+std::vector<uint32_t> device_passes(uint32_t total) {
+    if (total > 24)
+        throw PhjError(PHJ_ERR_UNSUPPORTED, "more than 24 partition bits (16M leaves) is not supported on the device");
+    const uint32_t n = (total + 7) / 8;
+    std::vector<uint32_t> v;
+    for (uint32_t p = 0; p < n; ++p)
+        v.push_back(total / n + (p < total % n ? 1 : 0));
+    return v;
+}
+
+struct Partitioned {
+    uint64_t rows = 0;
+    uint64_t *keys = nullptr;
+    OutColSet cols{};
+    uint32_t *off = nullptr; // parts + 1
+    uint64_t parts = 0;
+};
+
+OutColSet alloc_cols(phj_ctx *ctx, const ColSet &like, uint64_t rows, std::vector<void *> &track) {
+    OutColSet o{};
+    o.n = like.n;
+    for (uint32_t c = 0; c < like.n; ++c) {
+        o.types[c] = like.types[c];
+        o.ptr[c] = ctx->arena.alloc(std::max<uint64_t>(rows * type_size(like.types[c]), 1));
+        track.push_back(o.ptr[c]);
+    }
+    return o;
+}
+
+ColSet as_in(const OutColSet &o) {
+    ColSet c{};
+    c.n = o.n;
+    for (uint32_t i = 0; i < o.n; ++i) {
+        c.types[i] = o.types[i];
+        c.ptr[i] = o.ptr[i];
+    }
+    return c;
+}
+
+// Multi-pass radix partition of `in` (segments `in_off` with `n_seg` entries,
+// or the whole relation when in_off == nullptr) by `passes` starting at bit
+// position `consumed` from the top. Result buffers are tracked in `keep`.
+Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_off, uint32_t n_seg, uint32_t consumed,
+                            const std::vector<uint32_t> &passes, std::vector<void *> &keep, const char *) {
+    Scratch scr(ctx);
+    const uint64_t rows = in.rows;
+    uint32_t *seg = in_off ? const_cast<uint32_t *>(in_off) : nullptr;
+    if (!seg) {
+        seg = scr.get<uint32_t>(2);
+        launch_seg_single(ctx->stream, seg, static_cast<uint32_t>(rows));
+        n_seg = 1;
+    }
+    const uint64_t *cur_keys = in.keys;
+    ColSet cur_cols = in.cols;
+    std::vector<void *> prev_bufs; // buffers of the previous pass output
+    Partitioned out;
+    out.rows = rows;
+    uint32_t *cur_seg = seg;
+    uint32_t cur_nseg = n_seg;
+    for (size_t p = 0; p < passes.size(); ++p) {
+        const uint32_t bits = passes[p];
+        consumed += bits;
+        const uint32_t shift = 64 - consumed;
+        const uint64_t B = 1ull << bits;
+        const uint64_t out_nseg = static_cast<uint64_t>(cur_nseg) * B;
+        std::vector<void *> bufs;
+        uint64_t *nkeys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
+        bufs.push_back(nkeys);
+        OutColSet ncols = alloc_cols(ctx, cur_cols, rows, bufs);
+        uint32_t *nseg = ctx->alloc<uint32_t>(out_nseg + 1);
+        bufs.push_back(nseg);
+        const uint64_t max_tiles = radix_max_tiles(rows, cur_nseg);
+        RadixPassScratch rs;
+        rs.seg_tile_first = scr.get<uint32_t>(cur_nseg + 1);
+        rs.hist = scr.get<uint32_t>(max_tiles * B);
+        rs.block_sums = scr.get<uint32_t>(radix_scan_blocks(max_tiles * B) + 1);
+        radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, shift, bits, nseg, rs, ctx->tm());
+        // previous pass output is dead once this pass has consumed it
+        for (void *b : prev_bufs)
+            ctx->arena.free(b);
+        prev_bufs = bufs;
+        cur_keys = nkeys;
+        cur_cols = as_in(ncols);
+        cur_seg = nseg;
+        cur_nseg = static_cast<uint32_t>(out_nseg);
+        out.keys = nkeys;
+        out.cols = ncols;
+        out.off = nseg;
+        out.parts = out_nseg;
+    }
+    for (void *b : prev_bufs)
+        keep.push_back(b);
+    return out;
+}
+
+// Output buffers of a join result (capacity rows).
+struct OutBuffers {
+    uint64_t cap = 0;
+    uint64_t *keys = nullptr;
+    OutColSet left{}, right{};
+    std::vector<void *> bufs;
+};
+
+OutBuffers alloc_output(phj_ctx *ctx, const ColSet &l, const ColSet &r, uint64_t cap) {
+    OutBuffers o;
+    o.cap = cap;
+    o.keys = ctx->alloc<uint64_t>(std::max<uint64_t>(cap, 1));
+    o.bufs.push_back(o.keys);
+    o.left = alloc_cols(ctx, l, cap, o.bufs);
+    o.right = alloc_cols(ctx, r, cap, o.bufs);
+    return o;
+}
+
+void free_output(phj_ctx *ctx, OutBuffers &o) {
+    for (void *p : o.bufs)
+        ctx->arena.free(p);
+    o.bufs.clear();
+}
+
+OutSink sink_of(const OutBuffers &o, unsigned long long *cursor) {
+    OutSink s{};
+    s.cursor = cursor;
+    s.cap = o.cap;
+    s.keys = o.keys;
+    s.left = o.left;
+    s.right = o.right;
+    return s;
+}
+
+template <class T> T read_scalar(phj_ctx *ctx, const T *dev) {
+    T v{};
+    HIPCHK(hipMemcpyAsync(&v, dev, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    return v;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    float ms() const {
+        float m = 0;
+        HIPCHK(hipEventElapsedTime(&m, a, b));
+        return m;
+    }
+};
+
+void fill_result_cols(phj_join_result *res, const OutBuffers &o, uint64_t rows) {
+    res->out_rows = rows;
+    res->keys = o.keys;
+    res->n_left = o.left.n;
+    res->n_right = o.right.n;
+    for (uint32_t c = 0; c < o.left.n; ++c) {
+        res->left_types[c] = o.left.types[c];
+        res->left[c] = o.left.ptr[c];
+    }
+    for (uint32_t c = 0; c < o.right.n; ++c) {
+        res->right_types[c] = o.right.types[c];
+        res->right[c] = o.right.ptr[c];
+    }
+}
+
+void set_phase(phj_phase_timing &p, double ms, uint64_t rows) {
+    p.wall_ms = ms;
+    p.ns_per_row = rows == 0 ? 0.0 : ms * 1e6 / static_cast<double>(rows);
+}
+
+uint64_t own_result(phj_ctx *ctx, OutBuffers &o) {
+    const uint64_t h = ctx->next_handle++;
+    ctx->owned[h] = o.bufs;
+    o.bufs.clear();
+    return h;
+}
+
+// ---------------------------------------------------------------- leaf join
+struct LeafJoinPlan {
+    uint32_t load_pct = 50;
+    uint32_t chunk_rows = 16384;
+    uint32_t threads = 512;
+    uint32_t lds_limit_rows = 0;
+    bool force_global = false;
+};
+
+LeafJoinPlan plan_of(const phj_tuning &t) {
+    LeafJoinPlan p;
+    if (t.ht_load_pct)
+        p.load_pct = std::min<uint32_t>(std::max<uint32_t>(t.ht_load_pct, 10), 90);
+    if (t.probe_chunk_rows)
+        p.chunk_rows = t.probe_chunk_rows;
+    if (t.join_threads)
+        p.threads = t.join_threads;
+    p.lds_limit_rows = t.lds_leaf_rows;
+    p.force_global = (t.flags & PHJ_TUNE_FORCE_GLOBAL_HT) != 0;
+    return p;
+}
+
+// Joins partitioned R and S leaf by leaf. Returns the number of output rows
+// (rows beyond out.cap are counted but not written).
+uint64_t join_leaves(phj_ctx *ctx, const Partitioned &R, const Partitioned &S, const LeafJoinPlan &plan,
+                     const OutBuffers &out, unsigned long long *cursor, double *build_ms) {
+    Scratch scr(ctx);
+    const uint32_t n_leaves = static_cast<uint32_t>(R.parts);
+    OutSink sink = sink_of(out, cursor);
+
+    // LDS capacity: leaves up to lds_max rows use the LDS path.
+    uint32_t *dmax = scr.get<uint32_t>(1);
+    leaf_max_rows(ctx->stream, R.off, n_leaves, dmax);
+    const uint32_t max_leaf = read_scalar(ctx, dmax);
+    const uint32_t per_row_bytes = 16 * 100 / plan.load_pct + 4;
+    uint32_t lds_cap_rows = (150u * 1024u) / (per_row_bytes + 1);
+    if (plan.lds_limit_rows)
+        lds_cap_rows = std::min(lds_cap_rows, plan.lds_limit_rows);
+    const uint32_t lds_max = plan.force_global ? 0 : std::min(max_leaf, lds_cap_rows);
+
+    LeafJoinArgs a{};
+    a.r_keys = R.keys;
+    a.r_cols = as_in(R.cols);
+    a.r_off = R.off;
+    a.s_keys = S.keys;
+    a.s_cols = as_in(S.cols);
+    a.s_off = S.off;
+    a.n_leaves = n_leaves;
+    a.chunk_rows = plan.chunk_rows;
+    a.lds_max_rows = std::max<uint32_t>(lds_max, 1);
+    a.load_pct = plan.load_pct;
+    a.out = sink;
+    uint32_t *work_first = scr.get<uint32_t>(n_leaves + 1);
+    uint32_t *scan_tmp = scr.get<uint32_t>(radix_scan_blocks(n_leaves + 1) + 1);
+    a.work_first = work_first;
+    if (lds_max > 0) {
+        leaf_work_plan(ctx->stream, a, work_first, scan_tmp, ctx->tm());
+        leaf_join_lds(ctx->stream, a, leaf_max_work(S.rows, n_leaves, plan.chunk_rows), plan.threads, ctx->tm());
+    }
+
+    *build_ms = 0;
+    if (max_leaf > lds_max) {
+        // HBM tables for the oversize leaves: table b = b-th listed leaf.
+        uint32_t *leaf_of = scr.get<uint32_t>(n_leaves);
+        uint32_t *cnt = scr.get<uint32_t>(1);
+        leaf_big_list(ctx->stream, R.off, S.off, n_leaves, lds_max, leaf_of, cnt);
+        const uint32_t nb = read_scalar(ctx, cnt);
+        if (nb > 0) {
+            std::vector<uint32_t> h_leaf(nb), h_roff(n_leaves + 1), h_soff(n_leaves + 1);
+            HIPCHK(hipMemcpyAsync(h_leaf.data(), leaf_of, nb * 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipMemcpyAsync(h_roff.data(), R.off, (n_leaves + 1) * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipMemcpyAsync(h_soff.data(), S.off, (n_leaves + 1) * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+            ctx->sync();
+            std::sort(h_leaf.begin(), h_leaf.end());
+            std::vector<uint64_t> cell_off(nb + 1, 0);
+            std::vector<uint32_t> rfirst(nb + 1, 0), sfirst(nb + 1, 0);
+            for (uint32_t b = 0; b < nb; ++b) {
+                const uint32_t L = h_leaf[b];
+                const uint64_t nr = h_roff[L + 1] - h_roff[L];
+                uint64_t cap = 2;
+                while (cap < nr * 100 / plan.load_pct + 1)
+                    cap <<= 1;
+                cell_off[b + 1] = cell_off[b] + cap;
+                rfirst[b + 1] = rfirst[b] + static_cast<uint32_t>(nr);
+                sfirst[b + 1] = sfirst[b] + (h_soff[L + 1] - h_soff[L]);
+            }
+            uint64_t *d_cell_off = scr.get<uint64_t>(nb + 1);
+            uint32_t *d_leaf = scr.get<uint32_t>(nb);
+            uint32_t *d_rfirst = scr.get<uint32_t>(nb + 1);
+            uint32_t *d_sfirst = scr.get<uint32_t>(nb + 1);
+            HIPCHK(hipMemcpyAsync(d_cell_off, cell_off.data(), (nb + 1) * 8ull, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(d_leaf, h_leaf.data(), nb * 4ull, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(d_rfirst, rfirst.data(), (nb + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(d_sfirst, sfirst.data(), (nb + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
+            GlobalHT ht{};
+            ht.cells = scr.get<Cell16>(cell_off[nb]);
+            ht.cell_off = d_cell_off;
+            ht.next = scr.get<uint32_t>(R.rows);
+            ht.side_head = scr.get<uint32_t>(nb);
+            ht.n_tables = nb;
+            ht.block_rows = 1;
+            ht.build_seg_off = R.off;
+            EventPair ev;
+            HIPCHK(hipEventRecord(ev.a, ctx->stream));
+            ht_init(ctx->stream, ht, cell_off[nb], R.rows, ctx->tm());
+            HtRows hr{};
+            hr.keys = R.keys;
+            hr.rows = R.rows;
+            hr.seg_off = R.off;
+            hr.leaf_of = d_leaf;
+            hr.first = d_rfirst;
+            ht_build(ctx->stream, ht, kHtLeaves, hr, rfirst[nb], ctx->tm(), "leaf_ht_build");
+            HIPCHK(hipEventRecord(ev.b, ctx->stream));
+            HtRows hp{};
+            hp.keys = S.keys;
+            hp.rows = S.rows;
+            hp.seg_off = S.off;
+            hp.leaf_of = d_leaf;
+            hp.first = d_sfirst;
+            ht_probe_emit(ctx->stream, ht, kHtLeaves, hp, sfirst[nb], as_in(R.cols), as_in(S.cols), sink, ctx->tm(),
+                          "leaf_ht_probe");
+            ctx->sync();
+            *build_ms = ev.ms();
+        }
+    }
+    return read_scalar(ctx, reinterpret_cast<const uint64_t *>(cursor));
+}
+
+} // namespace
+
+namespace {
+
+void validate_rel(const phj_relation *r) {
+    if (r == nullptr)
+        invalid("null relation");
+    if (r->n_cols > PHJ_MAX_COLS)
+        invalid("at most 8 payload columns");
+    for (uint32_t c = 0; c < r->n_cols; ++c)
+        tsize(r->cols[c].type);
+}
+
+struct ResultBuilder {
+    phj_ctx *ctx;
+    phj_join_result *res;
+    uint64_t build_rows, probe_rows;
+    void finish(OutBuffers &out, uint64_t rows) {
+        fill_result_cols(res, out, rows);
+        res->peak_mem_bytes = ctx->arena.peak();
+        res->handle = own_result(ctx, out);
+    }
+};
+
+// Joins with the output capacity grown to the exact row count on overflow.
+template <class F> uint64_t join_with_capacity(phj_ctx *ctx, const ColSet &l, const ColSet &r, uint64_t cap, OutBuffers &out, F &&run) {
+    Scratch scr(ctx);
+    unsigned long long *cursor = scr.get<unsigned long long>(1);
+    out = alloc_output(ctx, l, r, cap);
+    launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(cursor), 1, 0);
+    uint64_t n = run(out, cursor);
+    if (n > out.cap) {
+        free_output(ctx, out);
+        out = alloc_output(ctx, l, r, n);
+        launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(cursor), 1, 0);
+        n = run(out, cursor);
+    }
+    return n;
+}
+
+// ---- PHJ-PURE: src/PHJ.cpp:19-150 ----
+void run_phj_pure(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, const uint8_t *pb, uint32_t np,
+                  const phj_tuning &tun, phj_join_result *res) {
+    validate_rel(&b);
+    validate_rel(&p);
+    const uint32_t total = validate_pass_bits(pb, np);
+    const std::vector<uint32_t> passes = device_passes(total);
+    std::memset(res, 0, sizeof(*res));
+    Scratch scr(ctx);
+    const DevRel R = stage_relation(ctx, scr, b);
+    const DevRel S = stage_relation(ctx, scr, p);
+    ctx->arena.reset_peak();
+    const LeafJoinPlan plan = plan_of(tun);
+
+    EventPair e2e, ebs, eps, ej;
+    HIPCHK(hipEventRecord(e2e.a, ctx->stream));
+    HIPCHK(hipEventRecord(ebs.a, ctx->stream));
+    std::vector<void *> keep;
+    const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build");
+    HIPCHK(hipEventRecord(ebs.b, ctx->stream));
+    HIPCHK(hipEventRecord(eps.a, ctx->stream));
+    const Partitioned SP = radix_partition(ctx, S, nullptr, 1, 0, passes, keep, "probe");
+    HIPCHK(hipEventRecord(eps.b, ctx->stream));
+    HIPCHK(hipEventRecord(ej.a, ctx->stream));
+    double big_build_ms = 0;
+    OutBuffers out;
+    const uint64_t n = join_with_capacity(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1), out,
+                                          [&](const OutBuffers &o, unsigned long long *cur) {
+                                              return join_leaves(ctx, RP, SP, plan, o, cur, &big_build_ms);
+                                          });
+    HIPCHK(hipEventRecord(ej.b, ctx->stream));
+    HIPCHK(hipEventRecord(e2e.b, ctx->stream));
+    ctx->sync();
+    for (void *k : keep)
+        ctx->arena.free(k);
+    set_phase(res->build_shuffle, ebs.ms(), R.rows);
+    set_phase(res->probe_shuffle, eps.ms(), S.rows);
+    // Leaf build and probe run fused in one kernel; the fused time is
+    // reported as probe, and build carries only separately built HBM tables.
+    set_phase(res->build, big_build_ms, R.rows);
+    set_phase(res->probe, ej.ms() - big_build_ms, S.rows);
+    res->e2e_wall_ms = e2e.ms();
+    ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, n);
+}
+
+// ---- CHJ: src/CHJ.cpp:16-119 with 256 lock-free sub-tables ----
+void run_chj(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, phj_join_result *res) {
+    validate_rel(&b);
+    validate_rel(&p);
+    std::memset(res, 0, sizeof(*res));
+    Scratch scr(ctx);
+    const DevRel R = stage_relation(ctx, scr, b);
+    const DevRel S = stage_relation(ctx, scr, p);
+    ctx->arena.reset_peak();
+    constexpr uint32_t kTables = 256;
+    EventPair e2e, eb, ep;
+    HIPCHK(hipEventRecord(e2e.a, ctx->stream));
+    HIPCHK(hipEventRecord(eb.a, ctx->stream));
+    uint32_t *counts = scr.get<uint32_t>(kTables);
+    ht_chj_plan(ctx->stream, R.keys, R.rows, kTables, counts, ctx->tm());
+    std::vector<uint32_t> hc(kTables);
+    HIPCHK(hipMemcpyAsync(hc.data(), counts, kTables * 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    std::vector<uint64_t> cell_off(kTables + 1, 0);
+    for (uint32_t t = 0; t < kTables; ++t) {
+        uint64_t cap = 2; // JoinHashTable::reserve: bit_ceil(2 * rows), src/HashTable.h:72-77
+        while (cap < 2ull * hc[t])
+            cap <<= 1;
+        cell_off[t + 1] = cell_off[t] + cap;
+    }
+    uint64_t *d_off = scr.get<uint64_t>(kTables + 1);
+    HIPCHK(hipMemcpyAsync(d_off, cell_off.data(), (kTables + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+    GlobalHT ht{};
+    ht.cells = scr.get<Cell16>(cell_off[kTables]);
+    ht.cell_off = d_off;
+    ht.next = scr.get<uint32_t>(R.rows);
+    ht.side_head = scr.get<uint32_t>(kTables);
+    ht.n_tables = kTables;
+    ht.block_rows = PHJ_PIPELINE_BLOCK_ROWS;
+    ht_init(ctx->stream, ht, cell_off[kTables], R.rows, ctx->tm());
+    HtRows hr{};
+    hr.keys = R.keys;
+    hr.rows = R.rows;
+    ht_build(ctx->stream, ht, kHtChj, hr, R.rows, ctx->tm(), "chj_build");
+    HIPCHK(hipEventRecord(eb.b, ctx->stream));
+    HIPCHK(hipEventRecord(ep.a, ctx->stream));
+    HtRows hp{};
+    hp.keys = S.keys;
+    hp.rows = S.rows;
+    OutBuffers out;
+    const uint64_t n = join_with_capacity(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1), out,
+                                          [&](const OutBuffers &o, unsigned long long *cur) {
+                                              ht_probe_emit(ctx->stream, ht, kHtChj, hp, S.rows, R.cols, S.cols,
+                                                            sink_of(o, cur), ctx->tm(), "chj_probe");
+                                              return read_scalar(ctx, reinterpret_cast<const uint64_t *>(cur));
+                                          });
+    HIPCHK(hipEventRecord(ep.b, ctx->stream));
+    HIPCHK(hipEventRecord(e2e.b, ctx->stream));
+    ctx->sync();
+    set_phase(res->build, eb.ms(), R.rows);
+    set_phase(res->probe, ep.ms(), S.rows);
+    res->e2e_wall_ms = e2e.ms();
+    ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, n);
+}
+
+__global__ void k_count_nonempty(const uint32_t *off, uint32_t n, uint32_t *out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n && off[g + 1] > off[g])
+        atomicAdd(out, 1u);
+}
+
+// ---- PHJ (best-effort partitioning): src/PHJBep.cpp:113-790 ----
+// The build side is shuffled to full leaf depth exactly as PHJ-PURE. The probe
+// side is never materialised whole: it is consumed in slices sized so that one
+// partitioned slice (two pass buffers) fits the global budget; every slice is
+// partitioned to the leaves and joined against the leaves' tables at once (an
+// eviction of every leaf the slice touched), then its buffers are reused.
+void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, const uint8_t *pb, uint32_t np,
+                 uint64_t budget_bytes, const phj_tuning &tun, phj_join_result *res) {
+    validate_rel(&b);
+    validate_rel(&p);
+    const uint32_t total = validate_pass_bits(pb, np);
+    const std::vector<uint32_t> passes = device_passes(total);
+    std::memset(res, 0, sizeof(*res));
+    Scratch scr(ctx);
+    const DevRel R = stage_relation(ctx, scr, b);
+    const DevRel S = stage_relation(ctx, scr, p);
+    ctx->arena.reset_peak();
+    const LeafJoinPlan plan = plan_of(tun);
+    uint64_t row_bytes = 8;
+    for (uint32_t c = 0; c < S.cols.n; ++c)
+        row_bytes += type_size(S.cols.types[c]);
+    // Two pass buffers of the slice are live at once.
+    const uint64_t slice_rows = std::max<uint64_t>(budget_bytes / (2 * row_bytes), radix_tile_rows());
+
+    EventPair e2e, ebs;
+    HIPCHK(hipEventRecord(e2e.a, ctx->stream));
+    HIPCHK(hipEventRecord(ebs.a, ctx->stream));
+    std::vector<void *> keep;
+    const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build");
+    HIPCHK(hipEventRecord(ebs.b, ctx->stream));
+
+    double shuffle_ms = 0, probe_ms = 0, big_build_ms = 0;
+    uint64_t evictions = 0, refinements = 0, peak_bytes = 0;
+    uint32_t *nonempty = scr.get<uint32_t>(1);
+    const uint32_t p1_parts = 1u << passes[0];
+    OutBuffers out;
+    auto probe_all = [&](const OutBuffers &o, unsigned long long *cur) -> uint64_t {
+        shuffle_ms = probe_ms = 0;
+        evictions = refinements = peak_bytes = 0;
+        for (uint64_t a = 0; a < S.rows; a += slice_rows) {
+            DevRel slice = S;
+            slice.rows = std::min(slice_rows, S.rows - a);
+            slice.keys = S.keys + a;
+            for (uint32_t c = 0; c < S.cols.n; ++c)
+                slice.cols.ptr[c] = static_cast<const uint8_t *>(S.cols.ptr[c]) + a * type_size(S.cols.types[c]);
+            EventPair es, ej;
+            HIPCHK(hipEventRecord(es.a, ctx->stream));
+            std::vector<void *> sk;
+            const Partitioned SP = radix_partition(ctx, slice, nullptr, 1, 0, passes, sk, "probe");
+            HIPCHK(hipEventRecord(es.b, ctx->stream));
+            HIPCHK(hipEventRecord(ej.a, ctx->stream));
+            double bb = 0;
+            join_leaves(ctx, RP, SP, plan, o, cur, &bb);
+            HIPCHK(hipEventRecord(ej.b, ctx->stream));
+            launch_fill_u32(ctx->stream, nonempty, 1, 0);
+            k_count_nonempty<<<static_cast<uint32_t>((SP.parts + 255) / 256), 256, 0, ctx->stream>>>(
+                SP.off, static_cast<uint32_t>(SP.parts), nonempty);
+            const uint32_t ne = read_scalar(ctx, nonempty);
+            evictions += ne;
+            refinements += passes.size() > 1 ? p1_parts : 0;
+            peak_bytes = std::max<uint64_t>(peak_bytes, 2 * slice.rows * row_bytes);
+            shuffle_ms += es.ms();
+            probe_ms += ej.ms() - bb;
+            big_build_ms += bb;
+            for (void *k : sk)
+                ctx->arena.free(k);
+        }
+        return read_scalar(ctx, reinterpret_cast<const uint64_t *>(cur));
+    };
+    const uint64_t n = join_with_capacity(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1), out, probe_all);
+    HIPCHK(hipEventRecord(e2e.b, ctx->stream));
+    ctx->sync();
+    for (void *k : keep)
+        ctx->arena.free(k);
+    set_phase(res->build_shuffle, ebs.ms(), R.rows);
+    set_phase(res->build, big_build_ms, R.rows);
+    set_phase(res->probe_shuffle, shuffle_ms, S.rows);
+    set_phase(res->probe, probe_ms, S.rows);
+    set_phase(res->eviction_overhead, 0.0, S.rows);
+    res->e2e_wall_ms = e2e.ms();
+    res->bep_budget_mib = budget_bytes >> 20;
+    res->bep_evictions = evictions;
+    res->bep_refinements = refinements;
+    res->bep_peak_bytes = peak_bytes;
+    res->bep_build_skip_retries = 0;
+    ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, n);
+}
+
+__global__ void k_u64_to_u32(const uint64_t *in, uint32_t *out, uint64_t n) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = static_cast<uint32_t>(in[i]);
+}
+
+__global__ void k_rows_to_refs(const uint32_t *rows, uint64_t n, uint32_t block_rows, phj_row_ref *out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const uint32_t r = rows[i];
+        phj_row_ref ref;
+        ref.block_no = r == kInvalidRow ? PHJ_INVALID_BLOCK : r / block_rows;
+        ref.row_no = r == kInvalidRow ? 0u : r % block_rows;
+        out[i] = ref;
+    }
+}
+
+void fill_partitioned(phj_ctx *ctx, const Partitioned &P, std::vector<void *> &keep, phj_partitioned *out) {
+    std::memset(out, 0, sizeof(*out));
+    uint64_t *off64 = ctx->alloc<uint64_t>(P.parts + 1);
+    keep.push_back(off64);
+    launch_u32_to_u64(ctx->stream, P.off, off64, P.parts + 1);
+    out->rows = P.rows;
+    out->parts = P.parts;
+    out->keys = P.keys;
+    out->offsets = off64;
+    out->n_cols = P.cols.n;
+    for (uint32_t c = 0; c < P.cols.n; ++c) {
+        out->col_types[c] = P.cols.types[c];
+        out->cols[c] = P.cols.ptr[c];
+    }
+    const uint64_t h = ctx->next_handle++;
+    ctx->owned[h] = keep;
+    out->handle = h;
+    ctx->sync();
+}
+
+GenParams make_gen_params(phj_ctx *ctx, const phj_gen_spec &s, Scratch &scr) {
+    GenParams g{};
+    g.seed = s.seed;
+    g.build_rows = s.build_rows;
+    g.build_total = s.build_rows;
+    g.probe_rows = s.probe_rows;
+    g.dist = s.distribution;
+    g.match_ppm = s.match_ppm;
+    if (s.build_n_cols > PHJ_MAX_COLS || s.probe_n_cols > PHJ_MAX_COLS)
+        invalid("at most 8 payload columns");
+    if (s.distribution > 2)
+        invalid("unknown distribution");
+    if (s.distribution == 1 && s.build_rows == 0 && s.probe_rows > 0)
+        invalid("reference streams need build rows");
+    g.n_build_cols = s.build_n_cols;
+    g.n_probe_cols = s.probe_n_cols;
+    for (uint32_t c = 0; c < PHJ_MAX_COLS; ++c) {
+        g.build_types[c] = s.build_types[c];
+        g.probe_types[c] = s.probe_types[c];
+        if (c < s.build_n_cols)
+            tsize(s.build_types[c]);
+        if (c < s.probe_n_cols)
+            tsize(s.probe_types[c]);
+        g.b_bpay[c] = stream_base(s.seed, kStreamBuildPay + c);
+        g.b_bpayhi[c] = stream_base(s.seed, kStreamBuildPayHi + c);
+        g.b_ppay[c] = stream_base(s.seed, kStreamProbePay + c);
+        g.b_ppayhi[c] = stream_base(s.seed, kStreamProbePayHi + c);
+    }
+    g.b_sel = stream_base(s.seed, kStreamProbeSelect);
+    g.b_pick = stream_base(s.seed, kStreamProbePick);
+    g.b_miss = stream_base(s.seed, kStreamProbeMiss);
+    g.b_rank = stream_base(s.seed, kStreamZipfRank);
+    const uint64_t kb = stream_base(s.seed, kStreamBuildKey);
+    g.m1 = (rand64(kb, 0) | 1) & kMask63;
+    g.m2 = (rand64(kb, 1) | 1) & kMask63;
+    g.m3 = (rand64(kb, 2) | 1) & kMask63;
+    if (s.distribution == 2) {
+        // Zipf tables (53-bit CDF thresholds) and the duplicate prefix sums.
+        auto thresholds = [](uint64_t n, double sx) {
+            std::vector<double> cum(n);
+            double acc = 0.0;
+            for (uint64_t k = 1; k <= n; ++k) {
+                acc += std::pow(static_cast<double>(k), -sx);
+                cum[k - 1] = acc;
+            }
+            std::vector<uint64_t> t(n);
+            const double scale = 9007199254740992.0;
+            for (uint64_t k = 0; k < n; ++k)
+                t[k] = static_cast<uint64_t>(cum[k] / acc * scale);
+            if (n > 0)
+                t[n - 1] = 1ull << 53;
+            return t;
+        };
+        const uint64_t maxdup = s.zipf_build_max_dup == 0 ? 1 : s.zipf_build_max_dup;
+        const std::vector<uint64_t> dup_t = thresholds(maxdup, s.zipf_build_s_milli / 1000.0);
+        const std::vector<uint64_t> rank_t = thresholds(s.build_rows, s.zipf_probe_s_milli / 1000.0);
+        std::vector<uint64_t> prefix(s.build_rows + 1, 0);
+        const uint64_t bd = stream_base(s.seed, kStreamZipfDup);
+        for (uint64_t d = 0; d < s.build_rows; ++d) {
+            const uint64_t u = rand64(bd, d) >> 11;
+            const uint64_t k = static_cast<uint64_t>(std::upper_bound(dup_t.begin(), dup_t.end(), u) - dup_t.begin());
+            prefix[d + 1] = prefix[d] + 1 + k;
+        }
+        g.build_total = prefix[s.build_rows];
+        uint64_t *dp = scr.get<uint64_t>(prefix.size());
+        uint64_t *dr = scr.get<uint64_t>(rank_t.size());
+        HIPCHK(hipMemcpyAsync(dp, prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        if (!rank_t.empty())
+            HIPCHK(hipMemcpyAsync(dr, rank_t.data(), rank_t.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        g.prefix = dp;
+        g.rank_t = dr;
+        g.n_rank = static_cast<uint32_t>(rank_t.size());
+        const uint64_t sb = stream_base(s.seed, kStreamShuffle);
+        unsigned m = 0;
+        while ((1ull << m) < g.build_total && m < 63)
+            ++m;
+        g.sh_mask = m == 0 ? 0 : ((1ull << m) - 1);
+        g.sh_r = (m + 1) / 2 == 0 ? 1 : (m + 1) / 2;
+        g.sh_o1 = rand64(sb, 10) | 1;
+        g.sh_o2 = rand64(sb, 11) | 1;
+        g.sh_c1 = rand64(sb, 12);
+    }
+    return g;
+}
+
+void gen_side(phj_ctx *ctx, const GenParams &g, int side, uint64_t rows, uint32_t ncols, const uint32_t *types,
+              phj_relation *rel) {
+    std::memset(rel, 0, sizeof(*rel));
+    rel->rows = rows;
+    rel->on_device = 1;
+    rel->n_cols = ncols;
+    uint64_t *keys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
+    rel->keys = keys;
+    OutColSet oc{};
+    oc.n = ncols;
+    for (uint32_t c = 0; c < ncols; ++c) {
+        oc.types[c] = types[c];
+        oc.ptr[c] = ctx->arena.alloc(std::max<uint64_t>(rows * type_size(types[c]), 1));
+        rel->cols[c].type = types[c];
+        rel->cols[c].data = oc.ptr[c];
+    }
+    gen_fill(ctx->stream, g, side, rows, keys, oc);
+}
+
+ColSet colset_of_result(const phj_join_result *r, bool left) {
+    ColSet c{};
+    c.n = left ? r->n_left : r->n_right;
+    for (uint32_t i = 0; i < c.n; ++i) {
+        c.types[i] = left ? r->left_types[i] : r->right_types[i];
+        c.ptr[i] = left ? r->left[i] : r->right[i];
+    }
+    return c;
+}
+
+template <class F> phj_status guard(phj_ctx *ctx, F &&f) {
+    if (ctx == nullptr)
+        return PHJ_ERR_INVALID_ARGUMENT;
+    try {
+        HIPCHK(hipSetDevice(ctx->device));
+        f();
+        ctx->err.clear();
+        return PHJ_OK;
+    } catch (const PhjError &e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        ctx->err = "host allocation failed";
+        return PHJ_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return PHJ_ERR_HIP;
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+uint32_t phj_abi_version(void) { return PHJ_ABI_VERSION; }
+
+phj_status phj_ctx_create(int device, phj_ctx **out) {
+    if (out == nullptr)
+        return PHJ_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return PHJ_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n)
+        return PHJ_ERR_INVALID_ARGUMENT;
+    auto ctx = std::make_unique<phj_ctx>();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return PHJ_ERR_HIP;
+    *out = ctx.release();
+    return PHJ_OK;
+}
+
+phj_status phj_ctx_destroy(phj_ctx *ctx) {
+    if (ctx == nullptr)
+        return PHJ_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->owned)
+        for (void *p : kv.second)
+            ctx->arena.free(p);
+    ctx->owned.clear();
+    ctx->arena.release_cached();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PHJ_OK;
+}
+
+const char *phj_last_error(const phj_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+phj_status phj_synchronize(phj_ctx *ctx) {
+    return guard(ctx, [&] { ctx->sync(); });
+}
+
+phj_status phj_dev_alloc(phj_ctx *ctx, uint64_t bytes, void **out) {
+    return guard(ctx, [&] {
+        if (out == nullptr)
+            invalid("null output pointer");
+        *out = ctx->arena.alloc(bytes);
+    });
+}
+
+phj_status phj_dev_free(phj_ctx *ctx, void *ptr) {
+    return guard(ctx, [&] { ctx->arena.free(ptr); });
+}
+
+phj_status phj_memcpy_h2d(phj_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    return guard(ctx, [&] {
+        if (bytes)
+            HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        ctx->sync();
+    });
+}
+
+phj_status phj_memcpy_d2h(phj_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    return guard(ctx, [&] {
+        if (bytes)
+            HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    });
+}
+
+phj_status phj_mem_stats(phj_ctx *ctx, uint64_t *current_bytes, uint64_t *peak_bytes) {
+    return guard(ctx, [&] {
+        if (current_bytes)
+            *current_bytes = ctx->arena.current();
+        if (peak_bytes)
+            *peak_bytes = ctx->arena.peak();
+    });
+}
+
+phj_status phj_reset_peak(phj_ctx *ctx) {
+    return guard(ctx, [&] { ctx->arena.reset_peak(); });
+}
+
+phj_status phj_int_hash64_batch(phj_ctx *ctx, const uint64_t *in, uint64_t n, uint64_t *out, int on_device) {
+    return guard(ctx, [&] {
+        if (n && (in == nullptr || out == nullptr))
+            invalid("null buffer");
+        if (on_device) {
+            launch_hash_batch(ctx->stream, in, n, out, ctx->tm());
+            ctx->sync();
+            return;
+        }
+        Scratch scr(ctx);
+        uint64_t *d = scr.get<uint64_t>(n);
+        if (n)
+            HIPCHK(hipMemcpyAsync(d, in, n * 8, hipMemcpyHostToDevice, ctx->stream));
+        launch_hash_batch(ctx->stream, d, n, d, ctx->tm());
+        if (n)
+            HIPCHK(hipMemcpyAsync(out, d, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    });
+}
+
+phj_status phj_radix_shuffle(phj_ctx *ctx, const phj_relation *input, const uint8_t *pass_bits, uint32_t n_passes,
+                             phj_partitioned *out) {
+    return guard(ctx, [&] {
+        validate_rel(input);
+        if (out == nullptr)
+            invalid("null output");
+        const uint32_t total = validate_pass_bits(pass_bits, n_passes);
+        Scratch scr(ctx);
+        const DevRel R = stage_relation(ctx, scr, *input);
+        std::vector<void *> keep;
+        const Partitioned P = radix_partition(ctx, R, nullptr, 1, 0, device_passes(total), keep, "shuffle");
+        fill_partitioned(ctx, P, keep, out);
+    });
+}
+
+phj_status phj_refine_to_leaves(phj_ctx *ctx, const phj_partitioned *pass1, const uint8_t *pass_bits, uint32_t n_passes,
+                                phj_partitioned *out) {
+    return guard(ctx, [&] {
+        if (pass1 == nullptr || out == nullptr)
+            invalid("null partitioned input/output");
+        if (pass_bits == nullptr || n_passes == 0)
+            invalid("refineToLeaves: empty pass_bits"); // src/RadixPartition.cpp:425-426
+        const uint32_t total = validate_pass_bits(pass_bits, n_passes);
+        if (pass1->parts != (1ull << pass_bits[0]))
+            invalid("input partition count must equal 2^pass_bits[0]");
+        Scratch scr(ctx);
+        DevRel in;
+        in.rows = pass1->rows;
+        in.keys = pass1->keys;
+        in.cols.n = pass1->n_cols;
+        for (uint32_t c = 0; c < pass1->n_cols; ++c) {
+            in.cols.types[c] = pass1->col_types[c];
+            in.cols.ptr[c] = pass1->cols[c];
+        }
+        uint32_t *off32 = scr.get<uint32_t>(pass1->parts + 1);
+        k_u64_to_u32<<<static_cast<uint32_t>((pass1->parts + 1 + 255) / 256), 256, 0, ctx->stream>>>(pass1->offsets, off32,
+                                                                                                     pass1->parts + 1);
+        std::vector<void *> keep;
+        Partitioned P;
+        if (n_passes == 1) {
+            // One pass: the pass-1 buckets are the leaves (src/RadixPartition.cpp:437-447); copy them.
+            P.rows = in.rows;
+            P.parts = pass1->parts;
+            P.keys = ctx->alloc<uint64_t>(std::max<uint64_t>(in.rows, 1));
+            keep.push_back(P.keys);
+            launch_copy(ctx->stream, P.keys, in.keys, in.rows * 8, nullptr);
+            P.cols = alloc_cols(ctx, in.cols, in.rows, keep);
+            for (uint32_t c = 0; c < in.cols.n; ++c)
+                launch_copy(ctx->stream, P.cols.ptr[c], in.cols.ptr[c], in.rows * type_size(in.cols.types[c]), nullptr);
+            P.off = ctx->alloc<uint32_t>(P.parts + 1);
+            keep.push_back(P.off);
+            launch_copy(ctx->stream, P.off, off32, (P.parts + 1) * 4, nullptr);
+        } else {
+            const std::vector<uint32_t> passes = device_passes(total - pass_bits[0]);
+            P = radix_partition(ctx, in, off32, static_cast<uint32_t>(pass1->parts), pass_bits[0], passes, keep, "refine");
+        }
+        fill_partitioned(ctx, P, keep, out);
+    });
+}
+
+phj_status phj_partitioned_release(phj_ctx *ctx, phj_partitioned *p) {
+    return guard(ctx, [&] {
+        if (p == nullptr || p->handle == 0)
+            return;
+        auto it = ctx->owned.find(p->handle);
+        if (it != ctx->owned.end()) {
+            for (void *b : it->second)
+                ctx->arena.free(b);
+            ctx->owned.erase(it);
+        }
+        std::memset(p, 0, sizeof(*p));
+    });
+}
+
+phj_status phj_hashtable_build_find(phj_ctx *ctx, const uint64_t *build_keys, uint64_t n_build, const uint64_t *probe_keys,
+                                    uint64_t n_probe, uint32_t block_rows, int on_device, phj_row_ref *heads,
+                                    phj_row_ref *next) {
+    return guard(ctx, [&] {
+        if (block_rows == 0)
+            invalid("block_rows must be positive");
+        if (n_build > 0xFFFFFFFEull)
+            invalid("at most 2^32-2 build rows");
+        Scratch scr(ctx);
+        const uint64_t *bk = build_keys, *pk = probe_keys;
+        if (!on_device) {
+            uint64_t *b = scr.get<uint64_t>(n_build), *p = scr.get<uint64_t>(n_probe);
+            if (n_build)
+                HIPCHK(hipMemcpyAsync(b, build_keys, n_build * 8, hipMemcpyHostToDevice, ctx->stream));
+            if (n_probe)
+                HIPCHK(hipMemcpyAsync(p, probe_keys, n_probe * 8, hipMemcpyHostToDevice, ctx->stream));
+            bk = b;
+            pk = p;
+        }
+        uint64_t cap = 2; // src/HashTable.h:72-77 reserve: bit_ceil(2 * rows)
+        while (cap < 2 * n_build)
+            cap <<= 1;
+        std::vector<uint64_t> off = {0, cap};
+        uint64_t *d_off = scr.get<uint64_t>(2);
+        HIPCHK(hipMemcpyAsync(d_off, off.data(), 16, hipMemcpyHostToDevice, ctx->stream));
+        GlobalHT ht{};
+        ht.cells = scr.get<Cell16>(cap);
+        ht.cell_off = d_off;
+        ht.next = scr.get<uint32_t>(n_build);
+        ht.side_head = scr.get<uint32_t>(1);
+        ht.n_tables = 1;
+        ht.block_rows = block_rows;
+        ht_init(ctx->stream, ht, cap, n_build, ctx->tm());
+        HtRows hr{};
+        hr.keys = bk;
+        hr.rows = n_build;
+        ht_build(ctx->stream, ht, kHtSingle, hr, n_build, ctx->tm(), "ht_build");
+        uint32_t *hrows = scr.get<uint32_t>(n_probe);
+        ht_find(ctx->stream, ht, pk, n_probe, nullptr, hrows, ctx->tm());
+        phj_row_ref *dh = on_device ? heads : scr.get<phj_row_ref>(n_probe);
+        phj_row_ref *dn = on_device ? next : scr.get<phj_row_ref>(n_build);
+        if (n_probe)
+            k_rows_to_refs<<<static_cast<uint32_t>((n_probe + 255) / 256), 256, 0, ctx->stream>>>(hrows, n_probe, block_rows, dh);
+        if (n_build)
+            k_rows_to_refs<<<static_cast<uint32_t>((n_build + 255) / 256), 256, 0, ctx->stream>>>(ht.next, n_build, block_rows,
+                                                                                                  dn);
+        if (!on_device) {
+            if (n_probe && heads)
+                HIPCHK(hipMemcpyAsync(heads, dh, n_probe * 8, hipMemcpyDeviceToHost, ctx->stream));
+            if (n_build && next)
+                HIPCHK(hipMemcpyAsync(next, dn, n_build * 8, hipMemcpyDeviceToHost, ctx->stream));
+        }
+        ctx->sync();
+    });
+}
+
+phj_status phj_join(phj_ctx *ctx, const phj_join_desc *d, phj_join_result *result) {
+    return guard(ctx, [&] {
+        if (d == nullptr || result == nullptr)
+            invalid("null descriptor/result");
+        switch (d->scheme) {
+        case PHJ_SCHEME_CHJ: run_chj(ctx, d->build, d->probe, result); break;
+        case PHJ_SCHEME_PHJ_PURE:
+            run_phj_pure(ctx, d->build, d->probe, d->pass_bits, d->n_passes, d->tuning, result);
+            break;
+        case PHJ_SCHEME_PHJ:
+            run_phj_bep(ctx, d->build, d->probe, d->pass_bits, d->n_passes, d->bep_budget_bytes, d->tuning, result);
+            break;
+        default: invalid("unknown scheme");
+        }
+    });
+}
+
+phj_status phj_run_chj(phj_ctx *ctx, const phj_relation *build, const phj_relation *probe, uint32_t,
+                       phj_join_result *result) {
+    return guard(ctx, [&] {
+        if (!build || !probe || !result)
+            invalid("null argument");
+        run_chj(ctx, *build, *probe, result);
+    });
+}
+
+phj_status phj_run_phj_pure(phj_ctx *ctx, const phj_relation *build, const phj_relation *probe, const uint8_t *pass_bits,
+                            uint32_t n_passes, uint32_t, phj_join_result *result) {
+    return guard(ctx, [&] {
+        if (!build || !probe || !result)
+            invalid("null argument");
+        phj_tuning t{};
+        run_phj_pure(ctx, *build, *probe, pass_bits, n_passes, t, result);
+    });
+}
+
+phj_status phj_run_phj(phj_ctx *ctx, const phj_relation *build, const phj_relation *probe, const uint8_t *pass_bits,
+                       uint32_t n_passes, uint32_t, uint64_t bep_budget_mib, phj_join_result *result) {
+    return guard(ctx, [&] {
+        if (!build || !probe || !result)
+            invalid("null argument");
+        phj_tuning t{};
+        run_phj_bep(ctx, *build, *probe, pass_bits, n_passes, bep_budget_mib << 20, t, result);
+    });
+}
+
+phj_status phj_result_release(phj_ctx *ctx, phj_join_result *result) {
+    return guard(ctx, [&] {
+        if (result == nullptr || result->handle == 0)
+            return;
+        auto it = ctx->owned.find(result->handle);
+        if (it != ctx->owned.end()) {
+            for (void *b : it->second)
+                ctx->arena.free(b);
+            ctx->owned.erase(it);
+        }
+        std::memset(result, 0, sizeof(*result));
+    });
+}
+
+phj_status phj_result_copy_to_host(phj_ctx *ctx, const phj_join_result *r, uint64_t *keys, void *const *left,
+                                   void *const *right) {
+    return guard(ctx, [&] {
+        if (r == nullptr)
+            invalid("null result");
+        const uint64_t n = r->out_rows;
+        if (n == 0)
+            return;
+        if (keys)
+            HIPCHK(hipMemcpyAsync(keys, r->keys, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        for (uint32_t c = 0; c < r->n_left; ++c)
+            if (left && left[c])
+                HIPCHK(hipMemcpyAsync(left[c], r->left[c], n * type_size(r->left_types[c]), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        for (uint32_t c = 0; c < r->n_right; ++c)
+            if (right && right[c])
+                HIPCHK(hipMemcpyAsync(right[c], r->right[c], n * type_size(r->right_types[c]), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        ctx->sync();
+    });
+}
+
+phj_status phj_result_checksum(phj_ctx *ctx, const phj_join_result *r, phj_checksum *out) {
+    return guard(ctx, [&] {
+        if (r == nullptr || out == nullptr)
+            invalid("null argument");
+        Scratch scr(ctx);
+        unsigned long long *acc = scr.get<unsigned long long>(3);
+        checksum_columns(ctx->stream, r->out_rows, r->keys, colset_of_result(r, true), colset_of_result(r, false), acc,
+                         ctx->tm());
+        unsigned long long h[3];
+        HIPCHK(hipMemcpyAsync(h, acc, 24, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        out->rows = h[0];
+        out->lane0 = h[1];
+        out->lane1 = h[2];
+    });
+}
+
+phj_status phj_gen_relations(phj_ctx *ctx, const phj_gen_spec *spec, phj_relation *build, phj_relation *probe) {
+    return guard(ctx, [&] {
+        if (spec == nullptr || build == nullptr || probe == nullptr)
+            invalid("null argument");
+        if (spec->probe_rows > 0xFFFFFFFEull || spec->build_rows > 0xFFFFFFFEull)
+            invalid("relations are limited to 2^32-2 rows");
+        Scratch scr(ctx);
+        const GenParams g = make_gen_params(ctx, *spec, scr);
+        uint32_t bt[PHJ_MAX_COLS], pt[PHJ_MAX_COLS];
+        for (uint32_t c = 0; c < PHJ_MAX_COLS; ++c) {
+            bt[c] = spec->build_types[c];
+            pt[c] = spec->probe_types[c];
+        }
+        gen_side(ctx, g, 0, g.build_total, spec->build_n_cols, bt, build);
+        gen_side(ctx, g, 1, spec->probe_rows, spec->probe_n_cols, pt, probe);
+        ctx->sync();
+    });
+}
+
+phj_status phj_relation_release(phj_ctx *ctx, phj_relation *rel) {
+    return guard(ctx, [&] {
+        if (rel == nullptr)
+            return;
+        ctx->arena.free(const_cast<uint64_t *>(rel->keys));
+        for (uint32_t c = 0; c < rel->n_cols; ++c)
+            ctx->arena.free(const_cast<void *>(rel->cols[c].data));
+        std::memset(rel, 0, sizeof(*rel));
+    });
+}
+
+phj_status phj_gen_expected_checksum(phj_ctx *ctx, const phj_gen_spec *spec, phj_checksum *out) {
+    return guard(ctx, [&] {
+        if (spec == nullptr || out == nullptr)
+            invalid("null argument");
+        if (spec->distribution == 2)
+            throw PhjError(PHJ_ERR_UNSUPPORTED, "closed-form checksum needs distinct build keys");
+        Scratch scr(ctx);
+        const GenParams g = make_gen_params(ctx, *spec, scr);
+        unsigned long long *acc = scr.get<unsigned long long>(3);
+        gen_expected_checksum(ctx->stream, g, acc);
+        unsigned long long h[3];
+        HIPCHK(hipMemcpyAsync(h, acc, 24, hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        out->rows = h[0];
+        out->lane0 = h[1];
+        out->lane1 = h[2];
+    });
+}
+
+phj_status phj_set_kernel_timing(phj_ctx *ctx, int enabled) {
+    return guard(ctx, [&] {
+        ctx->sync();
+        ctx->timing = enabled != 0;
+    });
+}
+
+phj_status phj_kernel_times_reset(phj_ctx *ctx) {
+    return guard(ctx, [&] {
+        ctx->sync();
+        ctx->timer.reset();
+    });
+}
+
+phj_status phj_kernel_times(phj_ctx *ctx, const char *kernel, double *total_ms, uint64_t *launches) {
+    return guard(ctx, [&] {
+        if (kernel == nullptr)
+            invalid("null kernel name");
+        ctx->sync();
+        double ms = 0;
+        uint64_t n = 0;
+        ctx->timer.get_total(kernel, ms, n);
+        if (total_ms)
+            *total_ms = ms;
+        if (launches)
+            *launches = n;
+    });
+}
+
+phj_status phj_kernel_names(phj_ctx *ctx, char *buf, uint64_t cap) {
+    return guard(ctx, [&] {
+        if (buf == nullptr || cap == 0)
+            invalid("null buffer");
+        ctx->sync();
+        const std::string s = ctx->timer.names();
+        const uint64_t n = std::min<uint64_t>(s.size(), cap - 1);
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    });
+}
+
+phj_status phj_copy_bench(phj_ctx *ctx, void *dst, const void *src, uint64_t bytes, int reps, double *avg_ms) {
+    return guard(ctx, [&] {
+        if (reps <= 0)
+            invalid("reps must be positive");
+        launch_copy(ctx->stream, dst, src, bytes, nullptr);
+        EventPair ev;
+        HIPCHK(hipEventRecord(ev.a, ctx->stream));
+        for (int r = 0; r < reps; ++r)
+            launch_copy(ctx->stream, dst, src, bytes, ctx->tm());
+        HIPCHK(hipEventRecord(ev.b, ctx->stream));
+        ctx->sync();
+        if (avg_ms)
+            *avg_ms = ev.ms() / reps;
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,137 @@
+// k_misc.hip — streaming kernels: intHash64Batch, copy/fill, output checksum.
+#include "kernels.h"
+
+namespace phjd {
+
+namespace {
+
+// src/Hash.h:33-58 — out[i] = intHash64(in[i]); 16-byte vector loads/stores,
+// grid-stride, purely HBM-bound (16 B per key).
+__global__ __launch_bounds__(256) void k_hash_batch(const uint64_t *__restrict__ in, uint64_t n, uint64_t *__restrict__ out) {
+    const uint64_t pairs = n / 2;
+    const auto *in2 = reinterpret_cast<const ulonglong2 *>(in);
+    auto *out2 = reinterpret_cast<ulonglong2 *>(out);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < pairs; i += stride) {
+        const ulonglong2 v = in2[i];
+        out2[i] = make_ulonglong2(int_hash64(v.x), int_hash64(v.y));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1))
+        out[n - 1] = int_hash64(in[n - 1]);
+}
+
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        b[i] = a[i];
+}
+
+__global__ void k_copy1(const uint8_t *__restrict__ a, uint8_t *__restrict__ b, uint64_t n) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = a[i];
+}
+
+template <class T> __global__ void k_fill(T *__restrict__ p, uint64_t n, T v) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+// Order-independent checksum: every output row's bytes (key | build cols |
+// probe cols, src/Reference.cpp:22-43) are digested by two chained SplitMix64
+// lanes; lane sums and the row count are accumulated mod 2^64.
+__global__ __launch_bounds__(256) void k_checksum(uint64_t n, const uint64_t *__restrict__ keys, ColSet left, ColSet right,
+                                                  uint32_t row_size, unsigned long long *__restrict__ acc) {
+    __shared__ unsigned long long part[3][4];
+    uint64_t s0 = 0, s1 = 0, cnt = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        Packer p(row_size);
+        p.push_le8(keys[i], 8);
+        for (uint32_t c = 0; c < left.n; ++c) {
+            uint64_t lo, hi;
+            load_elem(left.ptr[c], left.types[c], i, lo, hi);
+            p.push(lo, hi, type_size(left.types[c]));
+        }
+        for (uint32_t c = 0; c < right.n; ++c) {
+            uint64_t lo, hi;
+            load_elem(right.ptr[c], right.types[c], i, lo, hi);
+            p.push(lo, hi, type_size(right.types[c]));
+        }
+        p.finish();
+        s0 += p.d.a;
+        s1 += p.d.c;
+        cnt += 1;
+    }
+    s0 = wave_sum_u64(s0);
+    s1 = wave_sum_u64(s1);
+    cnt = wave_sum_u64(cnt);
+    const uint32_t w = threadIdx.x / 64;
+    if (lane_id() == 0) {
+        part[0][w] = cnt;
+        part[1][w] = s0;
+        part[2][w] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long t = 0;
+        for (int k = 0; k < 4; ++k)
+            t += part[threadIdx.x][k];
+        atomicAdd(&acc[threadIdx.x], t);
+    }
+}
+
+} // namespace
+
+void launch_hash_batch(hipStream_t s, const uint64_t *in, uint64_t n, uint64_t *out, KernelTimer *tm) {
+    if (n == 0)
+        return;
+    TimedScope ts(tm, "hash_batch", s);
+    const uint64_t pairs = (n + 1) / 2;
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((pairs + 255) / 256, 256 * 64));
+    k_hash_batch<<<grid, 256, 0, s>>>(in, n, out);
+}
+
+void launch_copy(hipStream_t s, void *dst, const void *src, uint64_t bytes, KernelTimer *tm) {
+    TimedScope ts(tm, "copy", s);
+    const uint64_t n16 = bytes / 16;
+    if (n16 > 0) {
+        const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n16 + 255) / 256, 256 * 64));
+        k_copy16<<<grid, 256, 0, s>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
+    }
+    const uint64_t tail = bytes - n16 * 16;
+    if (tail > 0)
+        k_copy1<<<1, 64, 0, s>>>(static_cast<const uint8_t *>(src) + n16 * 16, static_cast<uint8_t *>(dst) + n16 * 16, tail);
+}
+
+void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v) {
+    if (n == 0)
+        return;
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 256 * 32));
+    k_fill<uint32_t><<<grid, 256, 0, s>>>(p, n, v);
+}
+
+void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v) {
+    if (n == 0)
+        return;
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 256 * 32));
+    k_fill<uint64_t><<<grid, 256, 0, s>>>(p, n, v);
+}
+
+void checksum_columns(hipStream_t s, uint64_t n, const uint64_t *keys, const ColSet &left, const ColSet &right,
+                      unsigned long long *acc3, KernelTimer *tm) {
+    uint32_t row_size = 8;
+    for (uint32_t c = 0; c < left.n; ++c)
+        row_size += type_size(left.types[c]);
+    for (uint32_t c = 0; c < right.n; ++c)
+        row_size += type_size(right.types[c]);
+    k_fill<unsigned long long><<<1, 64, 0, s>>>(acc3, 3, 0ull);
+    if (n == 0)
+        return;
+    TimedScope ts(tm, "checksum", s);
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 256 * 16));
+    k_checksum<<<grid, 256, 0, s>>>(n, keys, left, right, row_size, acc3);
+}
+
+} // namespace phjd

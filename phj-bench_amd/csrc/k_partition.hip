@@ -1,0 +1,404 @@
+// k_partition.hip — radix partitioning on gfx950.
+//
+// Restates the behaviour of scatterBatch / radixShuffle / refineToLeaves
+// (src/RadixPartition.cpp:316-393, :417-539, :546-577): rows are grouped by
+// the hash bit window h >> (64 - consumed - bits) taken from the top of
+// intHash64, one pass at a time, each pass splitting every segment of the
+// previous pass. The CPU version appends to per-partition OutBlock chains; here
+// every pass is histogram -> device-wide exclusive scan -> scatter into one
+// contiguous column per field, so partition p of pass k occupies a contiguous
+// row range and no chains are needed.
+//
+// Kernels per pass:
+//   k_seg_tiles  tiles per input segment (exclusive scan, one workgroup)
+//   k_hist       per-tile digit histogram from per-wavefront LDS sub-histograms
+//   scan         exclusive scan over (segment, digit, tile) counts
+//   k_seg_out    output segment offsets
+//   k_scatter    wavefront ballot/mbcnt ranking into LDS, then each column is
+//                staged in LDS in digit order and leaves as contiguous runs
+#include "kernels.h"
+
+namespace phjd {
+
+namespace {
+
+constexpr int kPartThreads = 256;
+constexpr int kPartRpt = 16;
+constexpr int kTile = kPartThreads * kPartRpt; // rows per tile
+constexpr int kMaxDigits = 256;                // at most 8 bits per device pass
+constexpr int kScanThreads = 512;
+constexpr int kScanPer = 8;
+constexpr int kScanBlock = kScanThreads * kScanPer;
+
+struct TileLoc {
+    uint32_t seg, local, ntiles, first;
+    uint32_t row0, len;
+};
+
+__device__ __forceinline__ bool locate_tile(uint32_t t, const uint32_t *seg_off, const uint32_t *seg_tile_first,
+                                            uint32_t n_seg, TileLoc &L) {
+    if (t >= seg_tile_first[n_seg])
+        return false;
+    const uint32_t seg = upper_bound_idx(seg_tile_first, n_seg, t);
+    L.seg = seg;
+    L.first = seg_tile_first[seg];
+    L.ntiles = seg_tile_first[seg + 1] - L.first;
+    L.local = t - L.first;
+    L.row0 = seg_off[seg] + L.local * kTile;
+    const uint32_t end = seg_off[seg + 1];
+    L.len = min(static_cast<uint32_t>(kTile), end - L.row0);
+    return true;
+}
+
+// Tiles per segment, exclusively scanned (single workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t *__restrict__ seg_off, uint32_t n_seg,
+                                                    uint32_t *__restrict__ seg_tile_first) {
+    __shared__ uint32_t tmp[1024 / 64 + 1];
+    const uint32_t per = (n_seg + 1023) / 1024;
+    const uint32_t a = threadIdx.x * per;
+    const uint32_t b = min(n_seg, a + per);
+    uint32_t cnt = 0;
+    for (uint32_t s = a; s < b; ++s)
+        cnt += (seg_off[s + 1] - seg_off[s] + kTile - 1) / kTile;
+    uint32_t total;
+    uint32_t off = block_excl_scan<1024>(cnt, tmp, &total);
+    for (uint32_t s = a; s < b; ++s) {
+        seg_tile_first[s] = off;
+        off += (seg_off[s + 1] - seg_off[s] + kTile - 1) / kTile;
+    }
+    if (threadIdx.x == 0)
+        seg_tile_first[n_seg] = total;
+}
+
+__global__ void k_seg_single(uint32_t *seg_off, uint32_t rows) {
+    seg_off[0] = 0;
+    seg_off[1] = rows;
+}
+
+// Per-tile histogram; counts land at hist[first*B + d*ntiles + local] so one
+// flat exclusive scan yields each (segment, digit, tile) output offset.
+__global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ seg_off,
+                                                       const uint32_t *__restrict__ seg_tile_first, uint32_t n_seg,
+                                                       uint32_t shift, uint32_t bits, uint32_t *__restrict__ hist) {
+    constexpr int W = kPartThreads / 64;
+    __shared__ uint32_t sub[W][kMaxDigits];
+    TileLoc L;
+    if (!locate_tile(xcd_remap(blockIdx.x, gridDim.x), seg_off, seg_tile_first, n_seg, L))
+        return;
+    const uint32_t B = 1u << bits;
+    const uint32_t mask = B - 1;
+    for (uint32_t i = threadIdx.x; i < W * kMaxDigits; i += kPartThreads)
+        (&sub[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t wave = threadIdx.x / 64;
+#pragma unroll
+    for (int r = 0; r < kPartRpt; ++r) {
+        const uint32_t idx = r * kPartThreads + threadIdx.x;
+        if (idx < L.len) {
+            const uint64_t h = int_hash64(keys[L.row0 + idx]);
+            atomicAdd(&sub[wave][static_cast<uint32_t>(h >> shift) & mask], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < B; d += kPartThreads) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+            c += sub[w][d];
+        hist[L.first * B + d * L.ntiles + L.local] = c;
+    }
+}
+
+// ---- device-wide exclusive scan of uint32 (three kernels) ----
+__device__ __forceinline__ uint64_t scan_count(uint64_t max_n, const uint32_t *count_ptr, uint32_t mul) {
+    return count_ptr ? static_cast<uint64_t>(*count_ptr) * mul : max_n;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__restrict__ data, uint64_t max_n,
+                                                              const uint32_t *count_ptr, uint32_t mul,
+                                                              uint32_t *__restrict__ block_sums) {
+    __shared__ uint32_t tmp[kScanThreads / 64 + 1];
+    const uint64_t n = scan_count(max_n, count_ptr, mul);
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanBlock;
+    if (base >= n)
+        return;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const uint64_t i = base + static_cast<uint64_t>(k) * kScanThreads + threadIdx.x;
+        if (i < n)
+            s += data[i];
+    }
+    uint32_t total;
+    block_excl_scan<kScanThreads>(s, tmp, &total);
+    if (threadIdx.x == 0)
+        block_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block sums by one workgroup (serial over chunks).
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t *__restrict__ block_sums, uint64_t max_n,
+                                                   const uint32_t *count_ptr, uint32_t mul) {
+    __shared__ uint32_t tmp[1024 / 64 + 1];
+    const uint64_t n = scan_count(max_n, count_ptr, mul);
+    const uint32_t nb = static_cast<uint32_t>((n + kScanBlock - 1) / kScanBlock);
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? block_sums[i] : 0u;
+        uint32_t total;
+        const uint32_t off = block_excl_scan<1024>(v, tmp, &total);
+        if (i < nb)
+            block_sums[i] = carry + off;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict__ data, uint64_t max_n,
+                                                            const uint32_t *count_ptr, uint32_t mul,
+                                                            const uint32_t *__restrict__ block_sums) {
+    __shared__ uint32_t tmp[kScanThreads / 64 + 1];
+    const uint64_t n = scan_count(max_n, count_ptr, mul);
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanBlock;
+    if (base >= n)
+        return;
+    // Each thread owns kScanPer consecutive elements.
+    const uint64_t my = base + static_cast<uint64_t>(threadIdx.x) * kScanPer;
+    uint32_t v[kScanPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        v[k] = (my + k < n) ? data[my + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan<kScanThreads>(s, tmp, &total) + block_sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        if (my + k < n)
+            data[my + k] = off;
+        off += v[k];
+    }
+}
+
+// Output segment offsets: segment s*B + d starts at the scanned offset of its
+// first tile; a segment with no tiles is empty and starts at its input offset.
+__global__ void k_seg_out(const uint32_t *__restrict__ seg_off, const uint32_t *__restrict__ seg_tile_first, uint32_t n_seg,
+                          uint32_t bits, const uint32_t *__restrict__ scanned, uint32_t *__restrict__ out_off) {
+    const uint32_t B = 1u << bits;
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t total = static_cast<uint64_t>(n_seg) * B;
+    if (g < total) {
+        const uint32_t s = static_cast<uint32_t>(g >> bits);
+        const uint32_t d = static_cast<uint32_t>(g & (B - 1));
+        const uint32_t first = seg_tile_first[s];
+        const uint32_t nt = seg_tile_first[s + 1] - first;
+        out_off[g] = nt > 0 ? scanned[first * B + d * nt] : seg_off[s];
+    }
+    if (g == 0)
+        out_off[total] = seg_off[n_seg];
+}
+
+struct ScatterShared {
+    uint64_t stage[kTile];
+    uint8_t slot_digit[kTile];
+    uint32_t cursor[kMaxDigits];
+    uint32_t lstart[kMaxDigits];
+    uint32_t gbase[kMaxDigits];
+    uint32_t scan_tmp[kPartThreads / 64 + 1];
+};
+
+template <class T>
+__device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__restrict__ in, T *__restrict__ out,
+                                               const TileLoc &L, const uint16_t (&slot)[kPartRpt]) {
+    constexpr uint32_t kPerRound = sizeof(T) <= 8 ? kTile : kTile * 8 / sizeof(T);
+    constexpr uint32_t kRounds = kTile / kPerRound;
+    T v[kPartRpt];
+#pragma unroll
+    for (int r = 0; r < kPartRpt; ++r) {
+        const uint32_t idx = r * kPartThreads + threadIdx.x;
+        if (idx < L.len)
+            v[r] = in[L.row0 + idx];
+    }
+    T *st = reinterpret_cast<T *>(sh.stage);
+#pragma unroll
+    for (uint32_t round = 0; round < kRounds; ++round) {
+        const uint32_t lo = round * kPerRound;
+#pragma unroll
+        for (int r = 0; r < kPartRpt; ++r) {
+            const uint32_t idx = r * kPartThreads + threadIdx.x;
+            if (idx < L.len && slot[r] >= lo && slot[r] < lo + kPerRound)
+                st[slot[r] - lo] = v[r];
+        }
+        __syncthreads();
+        const uint32_t hi = min(L.len, lo + kPerRound);
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += kPartThreads) {
+            const uint32_t d = sh.slot_digit[k];
+            out[sh.gbase[d] + (k - sh.lstart[d])] = st[k - lo];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
+                                                          uint64_t *__restrict__ keys_out, OutColSet cout,
+                                                          const uint32_t *__restrict__ seg_off,
+                                                          const uint32_t *__restrict__ seg_tile_first, uint32_t n_seg,
+                                                          uint32_t shift, uint32_t bits,
+                                                          const uint32_t *__restrict__ scanned) {
+    __shared__ ScatterShared sh;
+    TileLoc L;
+    if (!locate_tile(xcd_remap(blockIdx.x, gridDim.x), seg_off, seg_tile_first, n_seg, L))
+        return;
+    const uint32_t B = 1u << bits;
+    const uint32_t mask = B - 1;
+    for (uint32_t i = threadIdx.x; i < B; i += kPartThreads)
+        sh.cursor[i] = 0;
+    __syncthreads();
+
+    uint32_t dig[kPartRpt];
+#pragma unroll
+    for (int r = 0; r < kPartRpt; ++r) {
+        const uint32_t idx = r * kPartThreads + threadIdx.x;
+        dig[r] = 0;
+        if (idx < L.len) {
+            const uint64_t h = int_hash64(keys_in[L.row0 + idx]);
+            dig[r] = static_cast<uint32_t>(h >> shift) & mask;
+        }
+    }
+    // Tile histogram with wavefront-aggregated LDS adds.
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int r = 0; r < kPartRpt; ++r) {
+        const uint32_t idx = r * kPartThreads + threadIdx.x;
+        const bool act = idx < L.len;
+        const uint64_t am = __ballot(act);
+        if (act) {
+            const uint64_t peers = match_any(dig[r], bits, am);
+            if (lane == static_cast<uint32_t>(__ffsll(static_cast<long long>(peers)) - 1))
+                atomicAdd(&sh.cursor[dig[r]], static_cast<uint32_t>(__popcll(peers)));
+        }
+    }
+    __syncthreads();
+    {
+        const uint32_t c = threadIdx.x < B ? sh.cursor[threadIdx.x] : 0u;
+        uint32_t total;
+        const uint32_t off = block_excl_scan<kPartThreads>(c, sh.scan_tmp, &total);
+        if (threadIdx.x < B) {
+            sh.lstart[threadIdx.x] = off;
+            sh.cursor[threadIdx.x] = off;
+            sh.gbase[threadIdx.x] = scanned[L.first * B + threadIdx.x * L.ntiles + L.local];
+        }
+    }
+    __syncthreads();
+    // Rank every row inside its digit: peers via ballots, offset via mbcnt,
+    // one LDS atomic per (wavefront, digit) group.
+    uint16_t slot[kPartRpt];
+#pragma unroll
+    for (int r = 0; r < kPartRpt; ++r) {
+        const uint32_t idx = r * kPartThreads + threadIdx.x;
+        const bool act = idx < L.len;
+        const uint64_t am = __ballot(act);
+        slot[r] = 0;
+        if (act) {
+            const uint32_t d = dig[r];
+            const uint64_t peers = match_any(d, bits, am);
+            const int leader = __ffsll(static_cast<long long>(peers)) - 1;
+            uint32_t base = 0;
+            if (lane == static_cast<uint32_t>(leader))
+                base = atomicAdd(&sh.cursor[d], static_cast<uint32_t>(__popcll(peers)));
+            base = __shfl(base, leader, 64);
+            const uint32_t s = base + mbcnt(peers);
+            slot[r] = static_cast<uint16_t>(s);
+            sh.slot_digit[s] = static_cast<uint8_t>(d);
+        }
+    }
+    __syncthreads();
+
+    scatter_column<uint64_t>(sh, keys_in, keys_out, L, slot);
+    for (uint32_t c = 0; c < cin.n; ++c) {
+        switch (cin.types[c]) {
+        case 0:
+            scatter_column<uint8_t>(sh, static_cast<const uint8_t *>(cin.ptr[c]), static_cast<uint8_t *>(cout.ptr[c]), L, slot);
+            break;
+        case 1:
+            scatter_column<uint16_t>(sh, static_cast<const uint16_t *>(cin.ptr[c]), static_cast<uint16_t *>(cout.ptr[c]), L,
+                                     slot);
+            break;
+        case 2:
+            scatter_column<uint32_t>(sh, static_cast<const uint32_t *>(cin.ptr[c]), static_cast<uint32_t *>(cout.ptr[c]), L,
+                                     slot);
+            break;
+        case 3:
+            scatter_column<uint64_t>(sh, static_cast<const uint64_t *>(cin.ptr[c]), static_cast<uint64_t *>(cout.ptr[c]), L,
+                                     slot);
+            break;
+        default:
+            scatter_column<ulonglong2>(sh, static_cast<const ulonglong2 *>(cin.ptr[c]), static_cast<ulonglong2 *>(cout.ptr[c]),
+                                       L, slot);
+            break;
+        }
+    }
+}
+
+__global__ void k_u32_to_u64(const uint32_t *__restrict__ in, uint64_t *__restrict__ out, uint64_t n) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = in[i];
+}
+
+} // namespace
+
+uint32_t radix_tile_rows() { return kTile; }
+
+uint64_t radix_max_tiles(uint64_t rows, uint32_t n_seg) { return (rows + kTile - 1) / kTile + n_seg; }
+
+uint64_t radix_scan_blocks(uint64_t elems) { return (elems + kScanBlock - 1) / kScanBlock; }
+
+void scan_u32(hipStream_t s, uint32_t *data, uint64_t max_n, const uint32_t *count_ptr, uint32_t mul,
+              uint32_t *block_sums, KernelTimer *tm, const char *name) {
+    const uint64_t nb = radix_scan_blocks(max_n);
+    if (nb == 0)
+        return;
+    TimedScope ts(tm, name, s);
+    k_scan_reduce<<<static_cast<uint32_t>(nb), kScanThreads, 0, s>>>(data, max_n, count_ptr, mul, block_sums);
+    k_scan_top<<<1, 1024, 0, s>>>(block_sums, max_n, count_ptr, mul);
+    k_scan_down<<<static_cast<uint32_t>(nb), kScanThreads, 0, s>>>(data, max_n, count_ptr, mul, block_sums);
+}
+
+void launch_seg_single(hipStream_t s, uint32_t *seg_off, uint32_t rows) { k_seg_single<<<1, 1, 0, s>>>(seg_off, rows); }
+
+void launch_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out, uint64_t n) {
+    if (n == 0)
+        return;
+    k_u32_to_u64<<<static_cast<uint32_t>((n + 255) / 256), 256, 0, s>>>(in, out, n);
+}
+
+void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
+                const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, uint32_t shift, uint32_t bits,
+                uint32_t *out_seg_off, const RadixPassScratch &scr, KernelTimer *tm) {
+    const uint32_t B = 1u << bits;
+    const uint64_t max_tiles = radix_max_tiles(rows, n_seg);
+    {
+        TimedScope ts(tm, "radix_plan", s);
+        k_seg_tiles<<<1, 1024, 0, s>>>(seg_off, n_seg, scr.seg_tile_first);
+    }
+    if (rows > 0) {
+        TimedScope ts(tm, "radix_hist", s);
+        k_hist<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, seg_off, scr.seg_tile_first, n_seg, shift,
+                                                                          bits, scr.hist);
+    }
+    scan_u32(s, scr.hist, max_tiles * B, scr.seg_tile_first + n_seg, B, scr.block_sums, tm, "radix_scan");
+    {
+        const uint64_t total = static_cast<uint64_t>(n_seg) * B + 1;
+        TimedScope ts(tm, "radix_plan", s);
+        k_seg_out<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(seg_off, scr.seg_tile_first, n_seg, bits,
+                                                                               scr.hist, out_seg_off);
+    }
+    if (rows > 0) {
+        TimedScope ts(tm, "radix_scatter", s);
+        k_scatter<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off,
+                                                                             scr.seg_tile_first, n_seg, shift, bits, scr.hist);
+    }
+}
+
+} // namespace phjd

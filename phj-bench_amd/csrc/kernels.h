@@ -1,0 +1,151 @@
+// kernels.h — host-side launchers of the gfx950 kernels (internal to libphj_hip.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace phjd {
+
+// Optional per-launch timing: events recorded around each named launch on
+// the launch stream, resolved after the stream synchronises.
+class KernelTimer {
+public:
+    virtual ~KernelTimer() = default;
+    virtual void begin(const char *name, hipStream_t s) = 0;
+    virtual void end(hipStream_t s) = 0;
+};
+
+struct TimedScope {
+    KernelTimer *t;
+    hipStream_t s;
+    TimedScope(KernelTimer *t_, const char *name, hipStream_t s_) : t(t_), s(s_) {
+        if (t)
+            t->begin(name, s);
+    }
+    ~TimedScope() {
+        if (t)
+            t->end(s);
+    }
+};
+
+// ---------------------------------------------------------------- hashing
+void launch_hash_batch(hipStream_t s, const uint64_t *in, uint64_t n, uint64_t *out, KernelTimer *tm);
+void launch_copy(hipStream_t s, void *dst, const void *src, uint64_t bytes, KernelTimer *tm);
+void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
+void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
+
+// ---------------------------------------------------------------- radix partition
+// One partition pass: every input segment [seg_off[s], seg_off[s+1]) is split
+// by hash bits [shift, shift+bits) into 2^bits output segments (output segment
+// s*2^bits + d). Rows stay inside their segment's range.
+struct RadixPassScratch {
+    uint32_t *seg_tile_first; // n_seg + 1
+    uint32_t *hist;           // max_tiles * 2^bits
+    uint32_t *block_sums;     // scan partials
+};
+uint32_t radix_tile_rows();
+uint64_t radix_max_tiles(uint64_t rows, uint32_t n_seg);
+uint64_t radix_scan_blocks(uint64_t elems);
+void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
+                const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, uint32_t shift, uint32_t bits,
+                uint32_t *out_seg_off, const RadixPassScratch &scr, KernelTimer *tm);
+
+// Exclusive scan of n uint32 (device count: n = *count_ptr * mul when count_ptr != nullptr).
+void scan_u32(hipStream_t s, uint32_t *data, uint64_t max_n, const uint32_t *count_ptr, uint32_t mul,
+              uint32_t *block_sums, KernelTimer *tm, const char *name);
+void launch_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out, uint64_t n);
+void launch_seg_single(hipStream_t s, uint32_t *seg_off, uint32_t rows);
+
+// ---------------------------------------------------------------- joins
+// Output sink shared by all probe kernels: rows are appended at an atomic
+// cursor; rows at or beyond `cap` are counted but not written.
+struct OutSink {
+    unsigned long long *cursor;
+    uint64_t cap;
+    uint64_t *keys;
+    OutColSet left;
+    OutColSet right;
+};
+
+struct LeafJoinArgs {
+    const uint64_t *r_keys;
+    ColSet r_cols;
+    const uint32_t *r_off; // n_leaves + 1
+    const uint64_t *s_keys;
+    ColSet s_cols;
+    const uint32_t *s_off; // n_leaves + 1
+    uint32_t n_leaves;
+    uint32_t chunk_rows;   // probe rows per work item
+    uint32_t lds_max_rows; // leaves with more build rows use the HBM table path
+    uint32_t load_pct;
+    const uint32_t *work_first; // n_leaves + 1, exclusive scan of work items per leaf
+    OutSink out;
+};
+// Work items per leaf for the LDS path (0 for empty or oversize leaves), scanned.
+void leaf_work_plan(hipStream_t s, const LeafJoinArgs &a, uint32_t *work_first, uint32_t *scan_tmp, KernelTimer *tm);
+uint64_t leaf_max_work(uint64_t probe_rows, uint32_t n_leaves, uint32_t chunk_rows);
+void leaf_join_lds(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_t threads, KernelTimer *tm);
+size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct);
+
+// Global-memory multi-table hash table with 16-byte (key, RowRefCell) cells
+// (src/HashTable.h:35-40): CHJ's 256 sub-tables, oversize PHJ leaves and the
+// standalone JoinHashTable entry point. Table t owns cells
+// [cell_off[t], cell_off[t] + cap[t]) with cap[t] a power of two.
+struct Cell16 {
+    uint64_t key;
+    uint32_t block_no;
+    uint32_t row_no;
+};
+static_assert(sizeof(Cell16) == 16, "cell is 16 bytes");
+
+struct GlobalHT {
+    Cell16 *cells;
+    const uint64_t *cell_off; // n_tables + 1
+    uint32_t *next;           // chain successor per build row (row-index space)
+    uint32_t *side_head;      // per table: head row of key == kEmptyKey
+    uint32_t n_tables;
+    uint32_t block_rows;      // RowRefCell{row / block_rows, row % block_rows}
+    // kHtLeaves: RowRefCell{leaf, row - build_seg_off[leaf]}
+    const uint32_t *build_seg_off;
+};
+
+// Largest segment (rows) of an offsets array, atomically maxed into *out.
+void leaf_max_rows(hipStream_t s, const uint32_t *off, uint32_t n, uint32_t *out);
+// Leaves with more than lds_max build rows and a non-empty probe side.
+void leaf_big_list(hipStream_t s, const uint32_t *r_off, const uint32_t *s_off, uint32_t n, uint32_t lds_max,
+                   uint32_t *leaf_of, uint32_t *count);
+
+// Table selection: CHJ picks table = hash & (n_tables - 1) and homes on
+// hash >> 8; the leaf mode maps contiguous row ranges (leaf segments) to tables
+// and homes on the low hash bits.
+enum HtMode : uint32_t { kHtSingle = 0, kHtChj = 1, kHtLeaves = 2 };
+
+struct HtRows {
+    const uint64_t *keys;
+    uint64_t rows;
+    // kHtLeaves: table b covers rows [seg_off[leaf_of[b]], seg_off[leaf_of[b]+1]);
+    // first[b] is the exclusive scan of those row counts.
+    const uint32_t *seg_off;
+    const uint32_t *leaf_of;
+    const uint32_t *first;
+};
+
+void ht_chj_plan(hipStream_t s, const uint64_t *keys, uint64_t rows, uint32_t n_tables, uint32_t *counts, KernelTimer *tm);
+void ht_init(hipStream_t s, const GlobalHT &ht, uint64_t total_cells, uint64_t build_rows, KernelTimer *tm);
+void ht_build(hipStream_t s, const GlobalHT &ht, HtMode mode, const HtRows &r, uint64_t max_rows, KernelTimer *tm,
+              const char *name);
+// Probe and emit (keys + build cols + probe cols) into `out`.
+void ht_probe_emit(hipStream_t s, const GlobalHT &ht, HtMode mode, const HtRows &probe, uint64_t max_rows,
+                   const ColSet &r_cols, const ColSet &s_cols, const OutSink &out, KernelTimer *tm, const char *name);
+// Probe only: heads per probe key (RowRefCell), plus next chain export.
+void ht_find(hipStream_t s, const GlobalHT &ht, const uint64_t *keys, uint64_t n, phjd::Cell16 *unused, uint32_t *heads_rows,
+             KernelTimer *tm);
+
+// ---------------------------------------------------------------- checksum / generators
+void checksum_columns(hipStream_t s, uint64_t n, const uint64_t *keys, const ColSet &left, const ColSet &right,
+                      unsigned long long *acc3, KernelTimer *tm);
+
+} // namespace phjd

@@ -1,0 +1,597 @@
+"""Python host mirror of the phj-bench scheme API over libphj_hip.so (gfx950).
+
+The names and argument meanings follow the reference's scheme functions:
+
+    runCHJ(build, probe, threads)                            src/CHJ.h:31
+    runPHJ(build, probe, cfg, threads)        (PHJ-PURE)     src/PHJ.h:31
+    runPhjBep(build, probe, cfg, threads, bep_budget_mib)    src/PHJBep.h:97-98
+    intHash64Batch(keys)                                     src/Hash.h:33-58
+    radixShuffle(input, cfg, threads)                        src/RadixPartition.h:187-199
+    refineToLeaves(pass1, pass_bits)                         src/RadixPartition.h:393-399
+
+Errors map as in the reference: invalid radix configurations and schemas
+raise ValueError (std::invalid_argument), allocation failures MemoryError
+(std::bad_alloc); device/runtime failures raise RuntimeError. The library is
+required: importing succeeds without it, but every call raises when the HIP
+library is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphj_hip.so")
+MAX_COLS = 8
+MAX_PASSES = 8
+INVALID_BLOCK = 0xFFFFFFFF
+PIPELINE_BLOCK_ROWS = 10_000  # src/Types.h:108
+
+# src/Types.h:27-34
+U8, U16, U32, U64, U128 = 0, 1, 2, 3, 4
+TYPE_SIZE = {U8: 1, U16: 2, U32: 4, U64: 8, U128: 16}
+TYPE_NAME = {U8: "u8", U16: "u16", U32: "u32", U64: "u64", U128: "u128"}
+NP_DTYPE = {U8: np.uint8, U16: np.uint16, U32: np.uint32, U64: np.uint64}
+
+# src/CLI.h:15-22
+SCHEME_CHJ, SCHEME_PHJ, SCHEME_PHJ_PURE = 1, 2, 3
+
+DIST_UNIFORM, DIST_REFERENCE, DIST_ZIPF = 0, 1, 2
+
+OK, ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_CAPACITY = range(7)
+
+
+class _Column(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("reserved", C.c_uint32), ("data", C.c_void_p)]
+
+
+class _Relation(C.Structure):
+    _fields_ = [
+        ("rows", C.c_uint64),
+        ("keys", C.c_void_p),
+        ("n_cols", C.c_uint32),
+        ("on_device", C.c_uint32),
+        ("cols", _Column * MAX_COLS),
+    ]
+
+
+class _Tuning(C.Structure):
+    _fields_ = [
+        ("tile_rows", C.c_uint32),
+        ("join_threads", C.c_uint32),
+        ("ht_load_pct", C.c_uint32),
+        ("lds_leaf_rows", C.c_uint32),
+        ("probe_chunk_rows", C.c_uint32),
+        ("flags", C.c_uint32),
+    ]
+
+
+class _JoinDesc(C.Structure):
+    _fields_ = [
+        ("scheme", C.c_uint32),
+        ("threads", C.c_uint32),
+        ("build", _Relation),
+        ("probe", _Relation),
+        ("n_passes", C.c_uint32),
+        ("pass_bits", C.c_uint8 * MAX_PASSES),
+        ("bep_budget_bytes", C.c_uint64),
+        ("tuning", _Tuning),
+    ]
+
+
+class _Phase(C.Structure):
+    _fields_ = [("wall_ms", C.c_double), ("ns_per_row", C.c_double)]
+
+
+class _JoinResult(C.Structure):
+    _fields_ = [
+        ("out_rows", C.c_uint64),
+        ("keys", C.c_void_p),
+        ("n_left", C.c_uint32),
+        ("n_right", C.c_uint32),
+        ("left_types", C.c_uint32 * MAX_COLS),
+        ("right_types", C.c_uint32 * MAX_COLS),
+        ("left", C.c_void_p * MAX_COLS),
+        ("right", C.c_void_p * MAX_COLS),
+        ("build_shuffle", _Phase),
+        ("build", _Phase),
+        ("probe_shuffle", _Phase),
+        ("probe", _Phase),
+        ("eviction_overhead", _Phase),
+        ("e2e_wall_ms", C.c_double),
+        ("peak_mem_bytes", C.c_uint64),
+        ("bep_budget_mib", C.c_uint64),
+        ("bep_evictions", C.c_uint64),
+        ("bep_refinements", C.c_uint64),
+        ("bep_peak_bytes", C.c_uint64),
+        ("bep_build_skip_retries", C.c_uint64),
+        ("handle", C.c_uint64),
+    ]
+
+
+class _Partitioned(C.Structure):
+    _fields_ = [
+        ("rows", C.c_uint64),
+        ("parts", C.c_uint64),
+        ("keys", C.c_void_p),
+        ("offsets", C.c_void_p),
+        ("n_cols", C.c_uint32),
+        ("col_types", C.c_uint32 * MAX_COLS),
+        ("cols", C.c_void_p * MAX_COLS),
+        ("handle", C.c_uint64),
+    ]
+
+
+class _Checksum(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("lane0", C.c_uint64), ("lane1", C.c_uint64)]
+
+
+class _GenSpec(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("build_rows", C.c_uint64),
+        ("probe_rows", C.c_uint64),
+        ("distribution", C.c_uint32),
+        ("match_ppm", C.c_uint32),
+        ("build_n_cols", C.c_uint32),
+        ("probe_n_cols", C.c_uint32),
+        ("build_types", C.c_uint8 * MAX_COLS),
+        ("probe_types", C.c_uint8 * MAX_COLS),
+        ("zipf_build_max_dup", C.c_uint32),
+        ("zipf_build_s_milli", C.c_uint32),
+        ("zipf_probe_s_milli", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class _RowRef(C.Structure):
+    _fields_ = [("block_no", C.c_uint32), ("row_no", C.c_uint32)]
+
+
+# Exported symbols with their argument types (the C header's declarations).
+_SIGNATURES = {
+    "phj_abi_version": ([], C.c_uint32),
+    "phj_ctx_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "phj_ctx_destroy": ([C.c_void_p], C.c_int),
+    "phj_last_error": ([C.c_void_p], C.c_char_p),
+    "phj_synchronize": ([C.c_void_p], C.c_int),
+    "phj_dev_alloc": ([C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+    "phj_dev_free": ([C.c_void_p, C.c_void_p], C.c_int),
+    "phj_memcpy_h2d": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
+    "phj_memcpy_d2h": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
+    "phj_mem_stats": ([C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "phj_reset_peak": ([C.c_void_p], C.c_int),
+    "phj_int_hash64_batch": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int], C.c_int),
+    "phj_radix_shuffle": ([C.c_void_p, C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.POINTER(_Partitioned)], C.c_int),
+    "phj_refine_to_leaves": ([C.c_void_p, C.POINTER(_Partitioned), C.c_void_p, C.c_uint32, C.POINTER(_Partitioned)],
+                             C.c_int),
+    "phj_partitioned_release": ([C.c_void_p, C.POINTER(_Partitioned)], C.c_int),
+    "phj_hashtable_build_find": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int,
+                                  C.c_void_p, C.c_void_p], C.c_int),
+    "phj_join": ([C.c_void_p, C.POINTER(_JoinDesc), C.POINTER(_JoinResult)], C.c_int),
+    "phj_run_chj": ([C.c_void_p, C.POINTER(_Relation), C.POINTER(_Relation), C.c_uint32, C.POINTER(_JoinResult)], C.c_int),
+    "phj_run_phj_pure": ([C.c_void_p, C.POINTER(_Relation), C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.c_uint32,
+                          C.POINTER(_JoinResult)], C.c_int),
+    "phj_run_phj": ([C.c_void_p, C.POINTER(_Relation), C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.c_uint32,
+                     C.c_uint64, C.POINTER(_JoinResult)], C.c_int),
+    "phj_result_release": ([C.c_void_p, C.POINTER(_JoinResult)], C.c_int),
+    "phj_result_copy_to_host": ([C.c_void_p, C.POINTER(_JoinResult), C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "phj_result_checksum": ([C.c_void_p, C.POINTER(_JoinResult), C.POINTER(_Checksum)], C.c_int),
+    "phj_gen_relations": ([C.c_void_p, C.POINTER(_GenSpec), C.POINTER(_Relation), C.POINTER(_Relation)], C.c_int),
+    "phj_relation_release": ([C.c_void_p, C.POINTER(_Relation)], C.c_int),
+    "phj_gen_expected_checksum": ([C.c_void_p, C.POINTER(_GenSpec), C.POINTER(_Checksum)], C.c_int),
+    "phj_set_kernel_timing": ([C.c_void_p, C.c_int], C.c_int),
+    "phj_kernel_times_reset": ([C.c_void_p], C.c_int),
+    "phj_kernel_times": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)], C.c_int),
+    "phj_kernel_names": ([C.c_void_p, C.c_char_p, C.c_uint64], C.c_int),
+    "phj_copy_bench": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)], C.c_int),
+}
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load_library():
+    """Load libphj_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libphj_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        lib = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+class PhjError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"phj error {code}: {msg}")
+        self.code = code
+
+
+def _raise(ctx_ptr, rc):
+    if rc == OK:
+        return
+    msg = load_library().phj_last_error(ctx_ptr)
+    msg = msg.decode() if msg else ""
+    if rc == ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if rc == ERR_OUT_OF_MEMORY:
+        raise MemoryError(msg)
+    raise PhjError(rc, msg)
+
+
+# ---------------------------------------------------------------- types
+@dataclass
+class RadixConfig:
+    """src/RadixPartition.h:20-33 — bits per pass, consumed from the top of the hash."""
+    pass_bits: list = field(default_factory=lambda: [8])
+
+    def total_bits(self):
+        return sum(self.pass_bits)
+
+    def partitions(self):
+        return 1 << self.total_bits()
+
+    @staticmethod
+    def from_partitions(partitions=256, passes=1):
+        """src/CLI.cpp:323-348 — total bits split evenly, remainder to the first passes."""
+        total = partitions.bit_length() - 1
+        if passes > total:
+            raise ValueError("passes cannot exceed log2(partitions)")
+        base, extra = divmod(total, passes)
+        return RadixConfig([base + (1 if p < extra else 0) for p in range(passes)])
+
+
+@dataclass
+class PhaseTiming:
+    wall_ms: float = 0.0
+    ns_per_row: float = 0.0
+
+
+class HostRelation:
+    """Column-major host relation: uint64 keys + payload columns (numpy).
+
+    A u128 column is an (n, 2) uint64 array of (low, high) (src/Types.h:17-24)."""
+
+    def __init__(self, keys, cols=(), types=()):
+        self.keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        self.cols = [np.ascontiguousarray(c) for c in cols]
+        self.types = list(types)
+        if len(self.cols) != len(self.types):
+            raise ValueError("schema and column count mismatch")
+        for c, t in zip(self.cols, self.types):
+            want = (self.rows, 2) if t == U128 else (self.rows,)
+            if c.shape != want or c.dtype != (np.uint64 if t == U128 else NP_DTYPE[t]):
+                raise ValueError(f"column of type {TYPE_NAME[t]} has shape {c.shape} dtype {c.dtype}")
+
+    @property
+    def rows(self):
+        return int(self.keys.shape[0])
+
+    def _c(self) -> _Relation:
+        r = _Relation()
+        r.rows = self.rows
+        r.keys = self.keys.ctypes.data
+        r.n_cols = len(self.cols)
+        r.on_device = 0
+        for i, (c, t) in enumerate(zip(self.cols, self.types)):
+            r.cols[i].type = t
+            r.cols[i].data = c.ctypes.data
+        return r
+
+
+class DeviceRelation:
+    """A relation resident in HBM (owned by the context)."""
+
+    def __init__(self, ctx, crel: _Relation):
+        self.ctx = ctx
+        self._rel = crel
+
+    @property
+    def rows(self):
+        return int(self._rel.rows)
+
+    @property
+    def types(self):
+        return [int(self._rel.cols[i].type) for i in range(self._rel.n_cols)]
+
+    def _c(self):
+        return self._rel
+
+    def to_host(self) -> HostRelation:
+        n = self.rows
+        keys = np.zeros(n, dtype=np.uint64)
+        self.ctx._d2h(keys, self._rel.keys)
+        cols = []
+        for i, t in enumerate(self.types):
+            a = np.zeros((n, 2), dtype=np.uint64) if t == U128 else np.zeros(n, dtype=NP_DTYPE[t])
+            self.ctx._d2h(a, self._rel.cols[i].data)
+            cols.append(a)
+        return HostRelation(keys, cols, self.types)
+
+    def release(self):
+        if self._rel.keys:
+            _raise(self.ctx._p, load_library().phj_relation_release(self.ctx._p, C.byref(self._rel)))
+
+
+class JoinOutput:
+    """Join output columns on the host: keys, build-side (left) and probe-side (right)
+    columns in schema order (src/JoinOutput.h:18-77)."""
+
+    def __init__(self, keys, left, left_types, right, right_types):
+        self.keys, self.left, self.left_types, self.right, self.right_types = keys, left, left_types, right, right_types
+
+    @property
+    def rows(self):
+        return int(self.keys.shape[0])
+
+    def blocks(self, block_rows=PIPELINE_BLOCK_ROWS):
+        """Chunks of at most block_rows rows, like the reference's OutputBlocks."""
+        for a in range(0, self.rows, block_rows):
+            b = min(self.rows, a + block_rows)
+            yield JoinOutput(self.keys[a:b], [c[a:b] for c in self.left], self.left_types,
+                             [c[a:b] for c in self.right], self.right_types)
+
+
+class JoinResult:
+    """ChjResult / PhjResult / PhjBepResult with the output left in HBM."""
+
+    def __init__(self, ctx, res: _JoinResult):
+        self.ctx = ctx
+        self._res = res
+        self.build_shuffle = PhaseTiming(res.build_shuffle.wall_ms, res.build_shuffle.ns_per_row)
+        self.build = PhaseTiming(res.build.wall_ms, res.build.ns_per_row)
+        self.probe_shuffle = PhaseTiming(res.probe_shuffle.wall_ms, res.probe_shuffle.ns_per_row)
+        self.probe = PhaseTiming(res.probe.wall_ms, res.probe.ns_per_row)
+        self.eviction_overhead = PhaseTiming(res.eviction_overhead.wall_ms, res.eviction_overhead.ns_per_row)
+        self.e2e_wall_ms = res.e2e_wall_ms
+        self.peak_mem_bytes = res.peak_mem_bytes
+        self.bep_budget_mib = res.bep_budget_mib
+        self.bep_evictions = res.bep_evictions
+        self.bep_refinements = res.bep_refinements
+        self.bep_peak_bytes = res.bep_peak_bytes
+        self.bep_build_skip_retries = res.bep_build_skip_retries
+
+    @property
+    def out_rows(self):
+        return int(self._res.out_rows)
+
+    def checksum(self):
+        cs = _Checksum()
+        _raise(self.ctx._p, load_library().phj_result_checksum(self.ctx._p, C.byref(self._res), C.byref(cs)))
+        return (int(cs.rows), int(cs.lane0), int(cs.lane1))
+
+    def to_host(self) -> JoinOutput:
+        r = self._res
+        n = int(r.out_rows)
+
+        def alloc(t):
+            return np.zeros((n, 2), dtype=np.uint64) if t == U128 else np.zeros(n, dtype=NP_DTYPE[t])
+
+        lt = [int(r.left_types[i]) for i in range(r.n_left)]
+        rt = [int(r.right_types[i]) for i in range(r.n_right)]
+        keys = np.zeros(n, dtype=np.uint64)
+        left = [alloc(t) for t in lt]
+        right = [alloc(t) for t in rt]
+        lp = (C.c_void_p * MAX_COLS)(*([a.ctypes.data for a in left] + [None] * (MAX_COLS - len(left))))
+        rp = (C.c_void_p * MAX_COLS)(*([a.ctypes.data for a in right] + [None] * (MAX_COLS - len(right))))
+        _raise(self.ctx._p, load_library().phj_result_copy_to_host(self.ctx._p, C.byref(r), keys.ctypes.data, lp, rp))
+        return JoinOutput(keys, left, lt, right, rt)
+
+    def release(self):
+        if self._res.handle:
+            _raise(self.ctx._p, load_library().phj_result_release(self.ctx._p, C.byref(self._res)))
+
+
+class Partitioned:
+    """Radix shuffle output: contiguous columns grouped by partition; offsets[p]..offsets[p+1]."""
+
+    def __init__(self, ctx, p: _Partitioned):
+        self.ctx = ctx
+        self._p = p
+
+    @property
+    def parts(self):
+        return int(self._p.parts)
+
+    @property
+    def rows(self):
+        return int(self._p.rows)
+
+    def offsets(self):
+        a = np.zeros(self.parts + 1, dtype=np.uint64)
+        self.ctx._d2h(a, self._p.offsets)
+        return a
+
+    def to_host(self):
+        n = self.rows
+        keys = np.zeros(n, dtype=np.uint64)
+        self.ctx._d2h(keys, self._p.keys)
+        cols = []
+        for i in range(self._p.n_cols):
+            t = int(self._p.col_types[i])
+            a = np.zeros((n, 2), dtype=np.uint64) if t == U128 else np.zeros(n, dtype=NP_DTYPE[t])
+            self.ctx._d2h(a, self._p.cols[i])
+            cols.append(a)
+        return HostRelation(keys, cols, [int(self._p.col_types[i]) for i in range(self._p.n_cols)]), self.offsets()
+
+    def release(self):
+        if self._p.handle:
+            _raise(self.ctx._p, load_library().phj_partitioned_release(self.ctx._p, C.byref(self._p)))
+
+
+def gen_spec(seed=42, build_rows=1 << 20, probe_rows=1 << 24, distribution=DIST_UNIFORM, match_rate=1.0,
+             build_types=(U64,), probe_types=(U64,), zipf_max_dup=64, zipf_build_s=1.2, zipf_probe_s=0.8) -> _GenSpec:
+    s = _GenSpec()
+    s.seed = seed
+    s.build_rows = build_rows
+    s.probe_rows = probe_rows
+    s.distribution = distribution
+    s.match_ppm = int(round(match_rate * 1_000_000))
+    s.build_n_cols = len(build_types)
+    s.probe_n_cols = len(probe_types)
+    for i, t in enumerate(build_types):
+        s.build_types[i] = t
+    for i, t in enumerate(probe_types):
+        s.probe_types[i] = t
+    s.zipf_build_max_dup = zipf_max_dup
+    s.zipf_build_s_milli = int(round(zipf_build_s * 1000))
+    s.zipf_probe_s_milli = int(round(zipf_probe_s * 1000))
+    return s
+
+
+def _pass_array(pass_bits):
+    pb = list(pass_bits)
+    return (C.c_uint8 * max(len(pb), 1))(*[min(max(int(b), 0), 255) for b in pb]), len(pb)
+
+
+class Context:
+    """One device and one HIP stream; owns the device arena."""
+
+    def __init__(self, device=0):
+        lib = load_library()
+        p = C.c_void_p()
+        rc = lib.phj_ctx_create(device, C.byref(p))
+        if rc != OK:
+            raise PhjError(rc, "no usable HIP device" if rc == ERR_NO_DEVICE else "context creation failed")
+        self._p = p
+
+    def close(self):
+        if self._p:
+            load_library().phj_ctx_destroy(self._p)
+            self._p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- memory
+    def _d2h(self, arr: np.ndarray, dev_ptr):
+        if arr.nbytes:
+            _raise(self._p, load_library().phj_memcpy_d2h(self._p, arr.ctypes.data, dev_ptr, arr.nbytes))
+
+    def mem_stats(self):
+        cur, peak = C.c_uint64(), C.c_uint64()
+        _raise(self._p, load_library().phj_mem_stats(self._p, C.byref(cur), C.byref(peak)))
+        return int(cur.value), int(peak.value)
+
+    def synchronize(self):
+        _raise(self._p, load_library().phj_synchronize(self._p))
+
+    # -- primitives
+    def intHash64Batch(self, keys) -> np.ndarray:
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.empty_like(keys)
+        _raise(self._p, load_library().phj_int_hash64_batch(self._p, keys.ctypes.data, keys.size, out.ctypes.data, 0))
+        return out
+
+    def radixShuffle(self, rel, cfg: RadixConfig, threads=1) -> Partitioned:
+        out = _Partitioned()
+        pb, n = _pass_array(cfg.pass_bits)
+        r = rel._c()
+        _raise(self._p, load_library().phj_radix_shuffle(self._p, C.byref(r), pb, n, C.byref(out)))
+        return Partitioned(self, out)
+
+    def refineToLeaves(self, pass1: Partitioned, pass_bits) -> Partitioned:
+        out = _Partitioned()
+        pb, n = _pass_array(pass_bits)
+        _raise(self._p, load_library().phj_refine_to_leaves(self._p, C.byref(pass1._p), pb, n, C.byref(out)))
+        return Partitioned(self, out)
+
+    def hashtable_build_find(self, build_keys, probe_keys, block_rows=PIPELINE_BLOCK_ROWS):
+        """JoinHashTable insert + batchFind with BlockStore chains: (heads, next) as (n, 2) uint32
+        arrays of RowRefCell{block_no, row_no}."""
+        bk = np.ascontiguousarray(build_keys, dtype=np.uint64)
+        pk = np.ascontiguousarray(probe_keys, dtype=np.uint64)
+        heads = np.zeros((pk.size, 2), dtype=np.uint32)
+        nxt = np.zeros((bk.size, 2), dtype=np.uint32)
+        _raise(self._p, load_library().phj_hashtable_build_find(self._p, bk.ctypes.data, bk.size, pk.ctypes.data, pk.size,
+                                                                block_rows, 0, heads.ctypes.data, nxt.ctypes.data))
+        return heads, nxt
+
+    # -- schemes
+    def join(self, scheme, build, probe, cfg: RadixConfig | None = None, threads=1, bep_budget_mib=512,
+             tuning: dict | None = None) -> JoinResult:
+        d = _JoinDesc()
+        d.scheme = scheme
+        d.threads = threads
+        d.build = build._c()
+        d.probe = probe._c()
+        pb = list((cfg or RadixConfig()).pass_bits)
+        if len(pb) > MAX_PASSES:
+            raise ValueError("at most 8 passes")
+        d.n_passes = len(pb)
+        for i, b in enumerate(pb):
+            d.pass_bits[i] = min(max(int(b), 0), 255)
+        d.bep_budget_bytes = int(bep_budget_mib) << 20
+        for k, v in (tuning or {}).items():
+            setattr(d.tuning, k, v)
+        res = _JoinResult()
+        _raise(self._p, load_library().phj_join(self._p, C.byref(d), C.byref(res)))
+        return JoinResult(self, res)
+
+    def runCHJ(self, build, probe, threads=1, **kw) -> JoinResult:
+        return self.join(SCHEME_CHJ, build, probe, None, threads, **kw)
+
+    def runPHJ(self, build, probe, cfg: RadixConfig, threads=1, **kw) -> JoinResult:
+        return self.join(SCHEME_PHJ_PURE, build, probe, cfg, threads, **kw)
+
+    def runPhjBep(self, build, probe, cfg: RadixConfig, threads=1, bep_budget_mib=512, **kw) -> JoinResult:
+        return self.join(SCHEME_PHJ, build, probe, cfg, threads, bep_budget_mib=bep_budget_mib, **kw)
+
+    # -- workloads
+    def generate(self, spec: _GenSpec):
+        b, p = _Relation(), _Relation()
+        _raise(self._p, load_library().phj_gen_relations(self._p, C.byref(spec), C.byref(b), C.byref(p)))
+        return DeviceRelation(self, b), DeviceRelation(self, p)
+
+    def expected_checksum(self, spec: _GenSpec):
+        cs = _Checksum()
+        _raise(self._p, load_library().phj_gen_expected_checksum(self._p, C.byref(spec), C.byref(cs)))
+        return (int(cs.rows), int(cs.lane0), int(cs.lane1))
+
+    # -- measurement
+    def set_kernel_timing(self, on=True):
+        _raise(self._p, load_library().phj_set_kernel_timing(self._p, 1 if on else 0))
+
+    def reset_kernel_times(self):
+        _raise(self._p, load_library().phj_kernel_times_reset(self._p))
+
+    def kernel_times(self):
+        buf = C.create_string_buffer(4096)
+        _raise(self._p, load_library().phj_kernel_names(self._p, buf, 4096))
+        out = {}
+        for name in filter(None, buf.value.decode().split(",")):
+            ms, n = C.c_double(), C.c_uint64()
+            _raise(self._p, load_library().phj_kernel_times(self._p, name.encode(), C.byref(ms), C.byref(n)))
+            out[name] = (float(ms.value), int(n.value))
+        return out
+
+    def copy_bench(self, nbytes, reps=5):
+        lib = load_library()
+        a, b = C.c_void_p(), C.c_void_p()
+        _raise(self._p, lib.phj_dev_alloc(self._p, nbytes, C.byref(a)))
+        _raise(self._p, lib.phj_dev_alloc(self._p, nbytes, C.byref(b)))
+        try:
+            ms = C.c_double()
+            _raise(self._p, lib.phj_copy_bench(self._p, b, a, nbytes, reps, C.byref(ms)))
+            return float(ms.value)
+        finally:
+            lib.phj_dev_free(self._p, a)
+            lib.phj_dev_free(self._p, b)

@@ -1,0 +1,281 @@
+"""GPU parity tests: every hot-path primitive and scheme through the C ABI,
+checked against the CPU oracle (oracle/) on the same seeded inputs."""
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import phj_gpu as G
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def host_rel(o: O.Relation) -> G.HostRelation:
+    return G.HostRelation(o.keys, o.cols, o.types)
+
+
+def orc_rel(h: G.HostRelation) -> O.Relation:
+    return O.Relation(h.keys, h.cols, h.types)
+
+
+def sorted_rows_of(out: G.JoinOutput):
+    return O.serialize_sorted(O.JoinRows(out.keys, out.left, out.left_types, out.right, out.right_types))
+
+
+def random_relation(rng, n, types, key_pool=None):
+    keys = rng.integers(0, 2**64, n, dtype=np.uint64) if key_pool is None else rng.choice(key_pool, n)
+    cols = []
+    for t in types:
+        if t == G.U128:
+            cols.append(rng.integers(0, 2**64, (n, 2), dtype=np.uint64))
+        else:
+            cols.append(rng.integers(0, np.iinfo(G.NP_DTYPE[t]).max, n, dtype=G.NP_DTYPE[t], endpoint=True))
+    return O.Relation(keys, cols, list(types))
+
+
+# ---------------------------------------------------------------- a1/a2 hash
+@pytest.mark.parametrize("n", [0, 1, 3, 1000, 1 << 20])
+def test_int_hash64_batch(ctx, n):
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 2**64, n, dtype=np.uint64)
+    if n >= 3:
+        x[:3] = [0, 2**64 - 1, 42]
+    got = ctx.intHash64Batch(x)
+    assert np.array_equal(got, O.int_hash64(x))
+
+
+# ---------------------------------------------------------------- generators
+@pytest.mark.parametrize("dist,kw", [
+    (G.DIST_UNIFORM, dict(match_rate=0.5, build_types=(2, 3), probe_types=(3, 4))),
+    (G.DIST_REFERENCE, dict(build_types=(0, 1, 2), probe_types=(4,))),
+    (G.DIST_ZIPF, dict(match_rate=0.9, build_types=(0, 1, 3), probe_types=(3,))),
+])
+def test_generator_matches_oracle(ctx, dist, kw):
+    args = dict(seed=7, build_rows=3000, probe_rows=20000, **kw)
+    b, p = ctx.generate(G.gen_spec(distribution=dist, **args))
+    ob, op = O.gen(O.make_spec(distribution=dist, **args))
+    for dev, ref in ((b, ob), (p, op)):
+        h = dev.to_host()
+        assert h.rows == ref.rows
+        assert np.array_equal(h.keys, ref.keys)
+        for a, c in zip(h.cols, ref.cols):
+            assert np.array_equal(a, c)
+    b.release()
+    p.release()
+
+
+# ---------------------------------------------------------------- a3-a7 partition
+def leaf_ids(keys, total_bits):
+    return (O.int_hash64(keys) >> np.uint64(64 - total_bits)).astype(np.int64)
+
+
+def rows_multiset(rel: G.HostRelation, idx=None):
+    o = orc_rel(rel) if idx is None else orc_rel(rel).take(idx)
+    j = O.JoinRows(o.keys, o.cols, o.types, [], [])
+    return O.serialize_sorted(j)
+
+
+@pytest.mark.parametrize("pass_bits", [[8], [4, 4], [8, 8], [3, 3, 3], [5], [16], [1], [12, 7]])
+@pytest.mark.parametrize("n", [0, 1, 4097, 100_000])
+def test_radix_shuffle_partitions(ctx, pass_bits, n):
+    rng = np.random.default_rng(len(pass_bits) * 1000 + n)
+    rel = random_relation(rng, n, [G.U8, G.U32, G.U128])
+    part = ctx.radixShuffle(host_rel(rel), G.RadixConfig(pass_bits))
+    out, off = part.to_host()
+    total = sum(pass_bits)
+    assert part.parts == 1 << total
+    assert off[0] == 0 and off[-1] == n and np.all(np.diff(off.astype(np.int64)) >= 0)
+    # every row sits in the partition of its top `total` hash bits
+    lid = leaf_ids(out.keys, total)
+    expect = np.repeat(np.arange(part.parts), np.diff(off.astype(np.int64)))
+    assert np.array_equal(lid, expect)
+    # histogram equals the brute-force top-bit count
+    counts = np.bincount(leaf_ids(rel.keys, total), minlength=part.parts) if n else np.zeros(part.parts)
+    assert np.array_equal(np.diff(off.astype(np.int64)), counts)
+    # the multiset of rows is preserved
+    assert np.array_equal(rows_multiset(out), rows_multiset(host_rel(rel)))
+    part.release()
+
+
+def test_radix_shuffle_rejects_bad_config(ctx):
+    rel = host_rel(random_relation(np.random.default_rng(1), 10, [G.U64]))
+    for bad in ([], [0], [17], [16, 16, 1]):
+        with pytest.raises(ValueError):
+            ctx.radixShuffle(rel, G.RadixConfig(bad))
+
+
+@pytest.mark.parametrize("pass_bits", [[4, 4], [6, 5], [3, 3, 3], [8]])
+def test_refine_to_leaves(ctx, pass_bits):
+    rng = np.random.default_rng(sum(pass_bits))
+    rel = random_relation(rng, 50_000, [G.U16, G.U64])
+    p1 = ctx.radixShuffle(host_rel(rel), G.RadixConfig(pass_bits[:1]))
+    leaves = ctx.refineToLeaves(p1, pass_bits)
+    direct = ctx.radixShuffle(host_rel(rel), G.RadixConfig(pass_bits))
+    a, offa = leaves.to_host()
+    b, offb = direct.to_host()
+    assert np.array_equal(offa, offb)
+    for L in range(0, leaves.parts, max(1, leaves.parts // 64)):
+        ia = np.arange(offa[L], offa[L + 1])
+        ib = np.arange(offb[L], offb[L + 1])
+        assert np.array_equal(rows_multiset(a, ia), rows_multiset(b, ib))
+    for x in (p1, leaves, direct):
+        x.release()
+
+
+# ---------------------------------------------------------------- a9-a12 hash table
+def test_hashtable_chains(ctx):
+    rng = np.random.default_rng(5)
+    pool = rng.integers(0, 2**64, 3000, dtype=np.uint64)
+    pool[:2] = [2**64 - 1, 0]  # the empty-slot sentinel and zero are ordinary keys
+    bkeys = rng.choice(pool, 20_000)
+    pkeys = np.concatenate([pool, rng.integers(0, 2**64, 1000, dtype=np.uint64)])
+    block_rows = 10_000
+    heads, nxt = ctx.hashtable_build_find(bkeys, pkeys, block_rows)
+
+    def row(ref):
+        return None if ref[0] == G.INVALID_BLOCK else int(ref[0]) * block_rows + int(ref[1])
+
+    where = {}
+    for i, k in enumerate(bkeys.tolist()):
+        where.setdefault(k, set()).add(i)
+    for j, k in enumerate(pkeys.tolist()):
+        chain = set()
+        r = row(heads[j])
+        steps = 0
+        while r is not None:
+            assert r not in chain
+            chain.add(r)
+            r = row(nxt[r])
+            steps += 1
+            assert steps <= len(bkeys)
+        assert chain == where.get(k, set()), j
+
+
+# ---------------------------------------------------------------- schemes
+SCHEMES = [("chj", None), ("phj_pure", [8]), ("phj_pure", [8, 8]), ("phj_pure", [4, 4]), ("phj_pure", [2]),
+           ("phj", [8, 8]), ("phj", [6])]
+
+
+def run_scheme(ctx, name, cfg, b, p, **kw):
+    if name == "chj":
+        return ctx.runCHJ(b, p, **kw)
+    if name == "phj_pure":
+        return ctx.runPHJ(b, p, G.RadixConfig(cfg), **kw)
+    return ctx.runPhjBep(b, p, G.RadixConfig(cfg), bep_budget_mib=1, **kw)
+
+
+def datasets():
+    rng = np.random.default_rng(11)
+    out = {}
+    s = O.make_spec(seed=3, build_rows=5000, probe_rows=40000, distribution=0, match_rate=0.5, build_types=(2, 3),
+                    probe_types=(3, 4))
+    out["uniform_half"] = O.gen(s)
+    s = O.make_spec(seed=4, build_rows=3000, probe_rows=30000, distribution=1, build_types=(0, 4), probe_types=(1, 2))
+    out["reference_streams"] = O.gen(s)
+    s = O.make_spec(seed=5, build_rows=500, probe_rows=30000, distribution=2, match_rate=0.9, build_types=(0, 1, 3),
+                    probe_types=(3,))
+    out["zipf_multi"] = O.gen(s)
+    pool = rng.integers(0, 2**64, 200, dtype=np.uint64)
+    pool[:3] = [2**64 - 1, 0, 1]
+    out["sentinel_dups"] = (random_relation(rng, 2000, [G.U64], pool), random_relation(rng, 5000, [G.U8], pool))
+    out["empty_build"] = (random_relation(rng, 0, [G.U32]), random_relation(rng, 1000, [G.U64]))
+    out["empty_probe"] = (random_relation(rng, 1000, [G.U32]), random_relation(rng, 0, [G.U64]))
+    out["all_miss"] = (random_relation(rng, 1000, [G.U64]), random_relation(rng, 3000, [G.U64]))
+    out["no_payload"] = (random_relation(rng, 1000, [], pool), random_relation(rng, 3000, [], pool))
+    out["wide_payload"] = (random_relation(rng, 3000, [0, 1, 2, 3, 4, 4, 3, 0], pool),
+                           random_relation(rng, 4000, [4, 3, 2, 1, 0, 4, 4, 4], pool))
+    return out
+
+
+DATA = None
+
+
+def data():
+    global DATA
+    if DATA is None:
+        DATA = datasets()
+    return DATA
+
+
+@pytest.mark.parametrize("dsname", ["uniform_half", "reference_streams", "zipf_multi", "sentinel_dups", "empty_build",
+                                    "empty_probe", "all_miss", "no_payload", "wide_payload"])
+@pytest.mark.parametrize("scheme,cfg", SCHEMES)
+def test_join_matches_reference(ctx, dsname, scheme, cfg):
+    ob, op = data()[dsname]
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    res = run_scheme(ctx, scheme, cfg, host_rel(ob), host_rel(op))
+    out = res.to_host()
+    assert out.rows == want.shape[0]
+    assert np.array_equal(sorted_rows_of(out), want)
+    assert res.checksum() == O.checksum_rows(want)
+    res.release()
+
+
+@pytest.mark.parametrize("flags", [dict(flags=1), dict(lds_leaf_rows=8), dict(ht_load_pct=80),
+                                   dict(join_threads=256), dict(join_threads=1024), dict(probe_chunk_rows=1000)])
+def test_phj_pure_tuning_paths(ctx, flags):
+    ob, op = data()["zipf_multi"]
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig([4, 4]), tuning=flags)
+    assert np.array_equal(sorted_rows_of(res.to_host()), want)
+    res.release()
+
+
+def test_join_rejects_bad_radix(ctx):
+    ob, op = data()["uniform_half"]
+    with pytest.raises(ValueError):
+        ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig([20, 20]))
+    with pytest.raises(ValueError):
+        ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig([]))
+
+
+# ---------------------------------------------------------------- checksums
+def test_expected_checksum_closed_form(ctx):
+    args = dict(seed=9, build_rows=1 << 16, probe_rows=1 << 19, match_rate=0.5, build_types=(2, 3), probe_types=(3, 4))
+    spec = G.gen_spec(distribution=G.DIST_UNIFORM, **args)
+    gpu_expected = ctx.expected_checksum(spec)
+    cpu_expected = O.expected_checksum(O.make_spec(distribution=0, **args), threads=8)
+    assert gpu_expected == cpu_expected
+    b, p = ctx.generate(spec)
+    for scheme, cfg in (("phj_pure", [8, 8]), ("chj", None), ("phj", [8, 8])):
+        res = run_scheme(ctx, scheme, cfg, b, p)
+        assert res.checksum() == cpu_expected, scheme
+        res.release()
+    b.release()
+    p.release()
+
+
+# ---------------------------------------------------------------- C++ adapter (drop-in signatures)
+def _write_side(f, rel: O.Relation):
+    f.write(struct.pack("<QI", rel.rows, len(rel.types)))
+    for t in rel.types:
+        f.write(struct.pack("<I", t))
+    f.write(rel.keys.tobytes())
+    for c in rel.cols:
+        f.write(np.ascontiguousarray(c).tobytes())
+
+
+@pytest.mark.parametrize("scheme,pass_bits", [(1, [8]), (3, [8]), (3, [4, 4]), (2, [4, 4])])
+def test_cpp_adapter_drop_in(scheme, pass_bits):
+    exe = os.path.join(ROOT, "phj-bench_amd", "host", "adapter_selftest")
+    ob, op = data()["reference_streams"]
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    with tempfile.TemporaryDirectory() as td:
+        inp, outp = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(inp, "wb") as f:
+            pb = bytes(pass_bits + [0] * (8 - len(pass_bits)))
+            f.write(struct.pack("<QII", 0x31464C45534A4850, scheme, len(pass_bits)) + pb + struct.pack("<QQ", 1, 4))
+            _write_side(f, ob)
+            _write_side(f, op)
+        r = subprocess.run([exe, inp, outp], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        with open(outp, "rb") as f:
+            n, rs = struct.unpack("<QQ", f.read(16))
+            got = np.frombuffer(f.read(), dtype=np.uint8).reshape(n, rs)
+    assert np.array_equal(got, want)
