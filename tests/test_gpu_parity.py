@@ -121,8 +121,8 @@ def test_refine_to_leaves(ctx, pass_bits):
     b, offb = direct.to_host()
     assert np.array_equal(offa, offb)
     for L in range(0, leaves.parts, max(1, leaves.parts // 64)):
-        ia = np.arange(offa[L], offa[L + 1])
-        ib = np.arange(offb[L], offb[L + 1])
+        ia = np.arange(int(offa[L]), int(offa[L + 1]))
+        ib = np.arange(int(offb[L]), int(offb[L + 1]))
         assert np.array_equal(rows_multiset(a, ia), rows_multiset(b, ib))
     for x in (p1, leaves, direct):
         x.release()
