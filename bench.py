@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--scale-log2", type=int, default=0, help="shrink both sides by 2^k (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one plain device copy of this size (HBM counter calibration under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the join on a single MI355X; "
@@ -105,6 +107,10 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step (wall), e2e events {['%.2f' % x for x in e2e]}")
 
+    if args.calib_copy_mib:
+        ms = ctx.copy_bench(args.calib_copy_mib << 20, reps=1)
+        log(f"calibration copy of {args.calib_copy_mib} MiB: {ms:.3f} ms")
+
     out_rows = last.out_rows
     got = last.checksum()
     want = ctx.expected_checksum(spec)
@@ -118,27 +124,22 @@ def main():
     tuples_per_s = (R + S) / (ms_per_step / 1e3)
     best_e2e = min(e2e)
 
-    # Dominant kernel and its algorithmic bytes per launch.
+    # Dominant kernel and its algorithmic bytes per launch (mean over the
+    # launches of one step). Per step: 2 passes x 2 relations of k_hist and
+    # k_scatter, one k_leaf_join.
     n = args.steps
-    per_kernel_bytes = {
-        # two passes per relation: every pass reads and writes each row once
-        "radix_scatter": 2 * (R * wR + S * wS) * 2 / (4 * n) * n,
-        # leaf join: build rows and probe rows read once, output written once
-        "leaf_join": R * wR + S * wS + out_rows * wO,
-        # histogram: keys read once per pass
-        "radix_hist": 2 * 8 * (R + S) / 4,
+    passes = 2
+    per_step_bytes = {
+        "radix_scatter": passes * 2 * (R * wR + S * wS),  # every pass reads and writes each row once
+        "radix_hist": passes * 8 * (R + S),                # every pass reads the keys once
+        "leaf_join": R * wR + S * wS + out_rows * wO,      # rows read once, output written once
     }
     dom = max(ktimes, key=lambda k: ktimes[k][0]) if ktimes else None
     roofline = None
-    if dom in per_kernel_bytes:
+    if dom in per_step_bytes:
         tot_ms, launches = ktimes[dom]
         avg_ms = tot_ms / launches
-        if dom == "radix_scatter":
-            bytes_per_launch = 2 * (R * wR + S * wS) * 2 / 4  # 4 launches per step: R, S x 2 passes (mean)
-        elif dom == "leaf_join":
-            bytes_per_launch = per_kernel_bytes["leaf_join"]
-        else:
-            bytes_per_launch = per_kernel_bytes["radix_hist"]
+        bytes_per_launch = per_step_bytes[dom] * n / launches
         achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
         traffic = None
         tf = os.path.join(ROOT, "profiles", "traffic.json")

@@ -383,9 +383,9 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
         bufs.push_back(nseg);
         const uint64_t max_tiles = radix_max_tiles(rows, cur_nseg);
         RadixPassScratch rs;
-        rs.seg_tile_first = scr.get<uint32_t>(cur_nseg + 1);
+        rs.seg_tile_first = scr.get<uint32_t>(2 * (static_cast<uint64_t>(cur_nseg) + 1));
         rs.hist = scr.get<uint32_t>(max_tiles * B);
-        rs.block_sums = scr.get<uint32_t>(radix_scan_blocks(max_tiles * B) + 1);
+        rs.block_sums = scr.get<uint32_t>(radix_max_chunks(rows, cur_nseg) * B);
         radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, shift, bits, nseg, rs, ctx->tm());
         // previous pass output is dead once this pass has consumed it
         for (void *b : prev_bufs)
@@ -493,7 +493,7 @@ uint64_t own_result(phj_ctx *ctx, OutBuffers &o) {
 // ---------------------------------------------------------------- leaf join
 struct LeafJoinPlan {
     uint32_t load_pct = 50;
-    uint32_t chunk_rows = 16384;
+    uint32_t chunk_rows = 0; // 0: twice the mean probe leaf, at least 16384
     uint32_t threads = 512;
     uint32_t lds_limit_rows = 0;
     bool force_global = false;
@@ -512,111 +512,169 @@ LeafJoinPlan plan_of(const phj_tuning &t) {
     return p;
 }
 
-// Joins partitioned R and S leaf by leaf. Returns the number of output rows
-// (rows beyond out.cap are counted but not written).
-uint64_t join_leaves(phj_ctx *ctx, const Partitioned &R, const Partitioned &S, const LeafJoinPlan &plan,
-                     const OutBuffers &out, unsigned long long *cursor, double *build_ms) {
-    Scratch scr(ctx);
-    const uint32_t n_leaves = static_cast<uint32_t>(R.parts);
-    OutSink sink = sink_of(out, cursor);
+// Joins partitioned R and S leaf by leaf in two phases: construction counts
+// the exact output rows (LDS leaves by a count pass, oversize leaves through
+// HBM tables probed in count-only mode); emit() then writes them at a given
+// base row of an output buffer.
+class LeafJoin {
+public:
+    LeafJoin(phj_ctx *ctx, const Partitioned &R, const Partitioned &S, const LeafJoinPlan &plan)
+        : ctx_(ctx), scr_(ctx), plan_(plan), R_(R), S_(S) {
+        const uint32_t n_leaves = static_cast<uint32_t>(R.parts);
+        uint32_t *dmax = scr_.get<uint32_t>(1);
+        leaf_max_rows(ctx->stream, R.off, n_leaves, dmax);
+        const uint32_t max_leaf = read_scalar(ctx, dmax);
+        // Two workgroups per CU when the largest leaf allows; otherwise one.
+        uint32_t lds_cap_rows = leaf_join_lds_max_rows(150u * 1024u, plan.load_pct);
+        const uint32_t two_per_cu = leaf_join_lds_max_rows(78u * 1024u, plan.load_pct);
+        if (max_leaf <= two_per_cu)
+            lds_cap_rows = two_per_cu;
+        if (plan.lds_limit_rows)
+            lds_cap_rows = std::min(lds_cap_rows, plan.lds_limit_rows);
+        lds_max_ = plan.force_global ? 0 : std::min(max_leaf, lds_cap_rows);
 
-    // LDS capacity: leaves up to lds_max rows use the LDS path.
-    uint32_t *dmax = scr.get<uint32_t>(1);
-    leaf_max_rows(ctx->stream, R.off, n_leaves, dmax);
-    const uint32_t max_leaf = read_scalar(ctx, dmax);
-    const uint32_t per_row_bytes = 16 * 100 / plan.load_pct + 4;
-    uint32_t lds_cap_rows = (150u * 1024u) / (per_row_bytes + 1);
-    if (plan.lds_limit_rows)
-        lds_cap_rows = std::min(lds_cap_rows, plan.lds_limit_rows);
-    const uint32_t lds_max = plan.force_global ? 0 : std::min(max_leaf, lds_cap_rows);
-
-    LeafJoinArgs a{};
-    a.r_keys = R.keys;
-    a.r_cols = as_in(R.cols);
-    a.r_off = R.off;
-    a.s_keys = S.keys;
-    a.s_cols = as_in(S.cols);
-    a.s_off = S.off;
-    a.n_leaves = n_leaves;
-    a.chunk_rows = plan.chunk_rows;
-    a.lds_max_rows = std::max<uint32_t>(lds_max, 1);
-    a.load_pct = plan.load_pct;
-    a.out = sink;
-    uint32_t *work_first = scr.get<uint32_t>(n_leaves + 1);
-    uint32_t *scan_tmp = scr.get<uint32_t>(radix_scan_blocks(n_leaves + 1) + 1);
-    a.work_first = work_first;
-    if (lds_max > 0) {
-        leaf_work_plan(ctx->stream, a, work_first, scan_tmp, ctx->tm());
-        leaf_join_lds(ctx->stream, a, leaf_max_work(S.rows, n_leaves, plan.chunk_rows), plan.threads, ctx->tm());
+        a_.r_keys = R.keys;
+        a_.r_cols = as_in(R.cols);
+        a_.r_off = R.off;
+        a_.s_keys = S.keys;
+        a_.s_cols = as_in(S.cols);
+        a_.s_off = S.off;
+        a_.n_leaves = n_leaves;
+        a_.chunk_rows = plan.chunk_rows
+                            ? plan.chunk_rows
+                            : static_cast<uint32_t>(std::max<uint64_t>(16384, 2 * (S.rows / std::max<uint64_t>(n_leaves, 1))));
+        a_.lds_max_rows = std::max<uint32_t>(lds_max_, 1);
+        a_.load_pct = plan.load_pct;
+        if (lds_max_ > 0) {
+            uint32_t *work_first = scr_.get<uint32_t>(n_leaves + 1);
+            uint32_t *scan_tmp = scr_.get<uint32_t>(radix_scan_blocks(n_leaves + 1) + 1);
+            max_work_ = leaf_max_work(S.rows, n_leaves, a_.chunk_rows);
+            uint32_t *item_leaf = scr_.get<uint32_t>(max_work_);
+            a_.work_first = work_first;
+            a_.item_leaf = item_leaf;
+            leaf_work_plan(ctx->stream, a_, work_first, scan_tmp, item_leaf, ctx->tm());
+            uint32_t *counts = scr_.get<uint32_t>(max_work_ + 1);
+            launch_fill_u32(ctx->stream, counts, max_work_ + 1, 0);
+            a_.item_count = counts;
+            a_.heads = scr_.get<uint16_t>(S.rows);
+            a_.next16 = scr_.get<uint16_t>(R.rows);
+            a_.chain_flag = scr_.get<uint32_t>(1);
+            launch_fill_u32(ctx->stream, a_.chain_flag, 1, 0);
+            leaf_probe(ctx->stream, a_, max_work_, plan.threads, ctx->tm());
+            uint32_t *tmp = scr_.get<uint32_t>(radix_scan_blocks(max_work_ + 1) + 1);
+            scan_u32(ctx->stream, counts, max_work_ + 1, nullptr, 1, tmp, ctx->tm(), "join_plan");
+            a_.item_off = counts;
+            lds_rows_ = read_scalar(ctx, counts + max_work_);
+            chains_ = read_scalar(ctx, a_.chain_flag) != 0;
+        }
+        if (max_leaf > lds_max_)
+            plan_big_leaves(n_leaves);
     }
 
-    *build_ms = 0;
-    if (max_leaf > lds_max) {
-        // HBM tables for the oversize leaves: table b = b-th listed leaf.
-        uint32_t *leaf_of = scr.get<uint32_t>(n_leaves);
-        uint32_t *cnt = scr.get<uint32_t>(1);
-        leaf_big_list(ctx->stream, R.off, S.off, n_leaves, lds_max, leaf_of, cnt);
-        const uint32_t nb = read_scalar(ctx, cnt);
-        if (nb > 0) {
-            std::vector<uint32_t> h_leaf(nb), h_roff(n_leaves + 1), h_soff(n_leaves + 1);
-            HIPCHK(hipMemcpyAsync(h_leaf.data(), leaf_of, nb * 4, hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(h_roff.data(), R.off, (n_leaves + 1) * 4ull, hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(h_soff.data(), S.off, (n_leaves + 1) * 4ull, hipMemcpyDeviceToHost, ctx->stream));
-            ctx->sync();
-            std::sort(h_leaf.begin(), h_leaf.end());
-            std::vector<uint64_t> cell_off(nb + 1, 0);
-            std::vector<uint32_t> rfirst(nb + 1, 0), sfirst(nb + 1, 0);
-            for (uint32_t b = 0; b < nb; ++b) {
-                const uint32_t L = h_leaf[b];
-                const uint64_t nr = h_roff[L + 1] - h_roff[L];
-                uint64_t cap = 2;
-                while (cap < nr * 100 / plan.load_pct + 1)
-                    cap <<= 1;
-                cell_off[b + 1] = cell_off[b] + cap;
-                rfirst[b + 1] = rfirst[b] + static_cast<uint32_t>(nr);
-                sfirst[b + 1] = sfirst[b] + (h_soff[L + 1] - h_soff[L]);
-            }
-            uint64_t *d_cell_off = scr.get<uint64_t>(nb + 1);
-            uint32_t *d_leaf = scr.get<uint32_t>(nb);
-            uint32_t *d_rfirst = scr.get<uint32_t>(nb + 1);
-            uint32_t *d_sfirst = scr.get<uint32_t>(nb + 1);
-            HIPCHK(hipMemcpyAsync(d_cell_off, cell_off.data(), (nb + 1) * 8ull, hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(d_leaf, h_leaf.data(), nb * 4ull, hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(d_rfirst, rfirst.data(), (nb + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(d_sfirst, sfirst.data(), (nb + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
-            GlobalHT ht{};
-            ht.cells = scr.get<Cell16>(cell_off[nb]);
-            ht.cell_off = d_cell_off;
-            ht.next = scr.get<uint32_t>(R.rows);
-            ht.side_head = scr.get<uint32_t>(nb);
-            ht.n_tables = nb;
-            ht.block_rows = 1;
-            ht.build_seg_off = R.off;
-            EventPair ev;
-            HIPCHK(hipEventRecord(ev.a, ctx->stream));
-            ht_init(ctx->stream, ht, cell_off[nb], R.rows, ctx->tm());
-            HtRows hr{};
-            hr.keys = R.keys;
-            hr.rows = R.rows;
-            hr.seg_off = R.off;
-            hr.leaf_of = d_leaf;
-            hr.first = d_rfirst;
-            ht_build(ctx->stream, ht, kHtLeaves, hr, rfirst[nb], ctx->tm(), "leaf_ht_build");
-            HIPCHK(hipEventRecord(ev.b, ctx->stream));
-            HtRows hp{};
-            hp.keys = S.keys;
-            hp.rows = S.rows;
-            hp.seg_off = S.off;
-            hp.leaf_of = d_leaf;
-            hp.first = d_sfirst;
-            ht_probe_emit(ctx->stream, ht, kHtLeaves, hp, sfirst[nb], as_in(R.cols), as_in(S.cols), sink, ctx->tm(),
-                          "leaf_ht_probe");
-            ctx->sync();
-            *build_ms = ev.ms();
+    uint64_t total() const { return lds_rows_ + big_rows_; }
+    double big_build_ms() const { return big_build_ms_; }
+
+    void emit(const OutBuffers &out, uint64_t base) {
+        if (lds_max_ > 0 && lds_rows_ > 0) {
+            a_.out = sink_of(out, nullptr);
+            a_.out_base = base;
+            leaf_emit(ctx_->stream, a_, max_work_, chains_, ctx_->tm());
+        }
+        if (nb_ > 0 && big_rows_ > 0) {
+            unsigned long long *cursor = scr_.get<unsigned long long>(1);
+            launch_fill_u64(ctx_->stream, reinterpret_cast<uint64_t *>(cursor), 1, base + lds_rows_);
+            ht_probe_emit(ctx_->stream, ht_, kHtLeaves, hp_, probe_big_, as_in(R_.cols), as_in(S_.cols),
+                          sink_of(out, cursor), ctx_->tm(), "leaf_ht_probe");
         }
     }
-    return read_scalar(ctx, reinterpret_cast<const uint64_t *>(cursor));
-}
+
+private:
+    // HBM tables for the oversize leaves: table b = b-th listed leaf.
+    void plan_big_leaves(uint32_t n_leaves) {
+        uint32_t *leaf_of = scr_.get<uint32_t>(n_leaves);
+        uint32_t *cnt = scr_.get<uint32_t>(1);
+        leaf_big_list(ctx_->stream, R_.off, S_.off, n_leaves, lds_max_, leaf_of, cnt);
+        nb_ = read_scalar(ctx_, cnt);
+        if (nb_ == 0)
+            return;
+        std::vector<uint32_t> h_leaf(nb_), h_roff(n_leaves + 1), h_soff(n_leaves + 1);
+        HIPCHK(hipMemcpyAsync(h_leaf.data(), leaf_of, nb_ * 4ull, hipMemcpyDeviceToHost, ctx_->stream));
+        HIPCHK(hipMemcpyAsync(h_roff.data(), R_.off, (n_leaves + 1) * 4ull, hipMemcpyDeviceToHost, ctx_->stream));
+        HIPCHK(hipMemcpyAsync(h_soff.data(), S_.off, (n_leaves + 1) * 4ull, hipMemcpyDeviceToHost, ctx_->stream));
+        ctx_->sync();
+        std::sort(h_leaf.begin(), h_leaf.end());
+        std::vector<uint64_t> cell_off(nb_ + 1, 0);
+        std::vector<uint32_t> rfirst(nb_ + 1, 0), sfirst(nb_ + 1, 0);
+        for (uint32_t b = 0; b < nb_; ++b) {
+            const uint32_t L = h_leaf[b];
+            const uint64_t nr = h_roff[L + 1] - h_roff[L];
+            uint64_t cap = 2;
+            while (cap < nr * 100 / plan_.load_pct + 1)
+                cap <<= 1;
+            cell_off[b + 1] = cell_off[b] + cap;
+            rfirst[b + 1] = rfirst[b] + static_cast<uint32_t>(nr);
+            sfirst[b + 1] = sfirst[b] + (h_soff[L + 1] - h_soff[L]);
+        }
+        uint64_t *d_cell_off = scr_.get<uint64_t>(nb_ + 1);
+        uint32_t *d_leaf = scr_.get<uint32_t>(nb_);
+        uint32_t *d_rfirst = scr_.get<uint32_t>(nb_ + 1);
+        uint32_t *d_sfirst = scr_.get<uint32_t>(nb_ + 1);
+        HIPCHK(hipMemcpyAsync(d_cell_off, cell_off.data(), (nb_ + 1) * 8ull, hipMemcpyHostToDevice, ctx_->stream));
+        HIPCHK(hipMemcpyAsync(d_leaf, h_leaf.data(), nb_ * 4ull, hipMemcpyHostToDevice, ctx_->stream));
+        HIPCHK(hipMemcpyAsync(d_rfirst, rfirst.data(), (nb_ + 1) * 4ull, hipMemcpyHostToDevice, ctx_->stream));
+        HIPCHK(hipMemcpyAsync(d_sfirst, sfirst.data(), (nb_ + 1) * 4ull, hipMemcpyHostToDevice, ctx_->stream));
+        ht_.cells = scr_.get<Cell16>(cell_off[nb_]);
+        ht_.cell_off = d_cell_off;
+        ht_.next = scr_.get<uint32_t>(R_.rows);
+        ht_.side_head = scr_.get<uint32_t>(nb_);
+        ht_.n_tables = nb_;
+        ht_.block_rows = 1;
+        ht_.build_seg_off = R_.off;
+        EventPair ev;
+        HIPCHK(hipEventRecord(ev.a, ctx_->stream));
+        ht_init(ctx_->stream, ht_, cell_off[nb_], R_.rows, ctx_->tm());
+        HtRows hr{};
+        hr.keys = R_.keys;
+        hr.rows = R_.rows;
+        hr.seg_off = R_.off;
+        hr.leaf_of = d_leaf;
+        hr.first = d_rfirst;
+        ht_build(ctx_->stream, ht_, kHtLeaves, hr, rfirst[nb_], ctx_->tm(), "leaf_ht_build");
+        HIPCHK(hipEventRecord(ev.b, ctx_->stream));
+        hp_.keys = S_.keys;
+        hp_.rows = S_.rows;
+        hp_.seg_off = S_.off;
+        hp_.leaf_of = d_leaf;
+        hp_.first = d_sfirst;
+        probe_big_ = sfirst[nb_];
+        // count-only probe: a sink of capacity 0 counts without writing
+        unsigned long long *cursor = scr_.get<unsigned long long>(1);
+        launch_fill_u64(ctx_->stream, reinterpret_cast<uint64_t *>(cursor), 1, 0);
+        OutSink counter{};
+        counter.cursor = cursor;
+        counter.cap = 0;
+        ht_probe_emit(ctx_->stream, ht_, kHtLeaves, hp_, probe_big_, as_in(R_.cols), as_in(S_.cols), counter, ctx_->tm(),
+                      "leaf_ht_count");
+        big_rows_ = read_scalar(ctx_, reinterpret_cast<const uint64_t *>(cursor));
+        big_build_ms_ = ev.ms();
+    }
+
+    phj_ctx *ctx_;
+    Scratch scr_;
+    LeafJoinPlan plan_;
+    const Partitioned &R_, &S_;
+    LeafJoinArgs a_{};
+    uint32_t lds_max_ = 0;
+    uint64_t max_work_ = 0;
+    uint64_t lds_rows_ = 0;
+    bool chains_ = false;
+    uint32_t nb_ = 0;
+    GlobalHT ht_{};
+    HtRows hp_{};
+    uint64_t probe_big_ = 0;
+    uint64_t big_rows_ = 0;
+    double big_build_ms_ = 0;
+};
 
 } // namespace
 
@@ -629,6 +687,18 @@ void validate_rel(const phj_relation *r) {
         invalid("at most 8 payload columns");
     for (uint32_t c = 0; c < r->n_cols; ++c)
         tsize(r->cols[c].type);
+}
+
+// Re-allocates the output with `cap` rows, keeping the first `used` rows.
+void grow_output(phj_ctx *ctx, OutBuffers &out, uint64_t used, uint64_t cap) {
+    OutBuffers bigger = alloc_output(ctx, as_in(out.left), as_in(out.right), cap);
+    launch_copy(ctx->stream, bigger.keys, out.keys, used * 8, nullptr);
+    for (uint32_t c = 0; c < out.left.n; ++c)
+        launch_copy(ctx->stream, bigger.left.ptr[c], out.left.ptr[c], used * type_size(out.left.types[c]), nullptr);
+    for (uint32_t c = 0; c < out.right.n; ++c)
+        launch_copy(ctx->stream, bigger.right.ptr[c], out.right.ptr[c], used * type_size(out.right.types[c]), nullptr);
+    free_output(ctx, out);
+    out = std::move(bigger);
 }
 
 struct ResultBuilder {
@@ -684,10 +754,14 @@ void run_phj_pure(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, co
     HIPCHK(hipEventRecord(ej.a, ctx->stream));
     double big_build_ms = 0;
     OutBuffers out;
-    const uint64_t n = join_with_capacity(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1), out,
-                                          [&](const OutBuffers &o, unsigned long long *cur) {
-                                              return join_leaves(ctx, RP, SP, plan, o, cur, &big_build_ms);
-                                          });
+    uint64_t n = 0;
+    {
+        LeafJoin lj(ctx, RP, SP, plan);
+        n = lj.total();
+        big_build_ms = lj.big_build_ms();
+        out = alloc_output(ctx, R.cols, S.cols, n);
+        lj.emit(out, 0);
+    }
     HIPCHK(hipEventRecord(ej.b, ctx->stream));
     HIPCHK(hipEventRecord(e2e.b, ctx->stream));
     ctx->sync();
@@ -805,40 +879,46 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     uint32_t *nonempty = scr.get<uint32_t>(1);
     const uint32_t p1_parts = 1u << passes[0];
     OutBuffers out;
-    auto probe_all = [&](const OutBuffers &o, unsigned long long *cur) -> uint64_t {
-        shuffle_ms = probe_ms = 0;
-        evictions = refinements = peak_bytes = 0;
-        for (uint64_t a = 0; a < S.rows; a += slice_rows) {
-            DevRel slice = S;
-            slice.rows = std::min(slice_rows, S.rows - a);
-            slice.keys = S.keys + a;
-            for (uint32_t c = 0; c < S.cols.n; ++c)
-                slice.cols.ptr[c] = static_cast<const uint8_t *>(S.cols.ptr[c]) + a * type_size(S.cols.types[c]);
-            EventPair es, ej;
-            HIPCHK(hipEventRecord(es.a, ctx->stream));
-            std::vector<void *> sk;
-            const Partitioned SP = radix_partition(ctx, slice, nullptr, 1, 0, passes, sk, "probe");
-            HIPCHK(hipEventRecord(es.b, ctx->stream));
-            HIPCHK(hipEventRecord(ej.a, ctx->stream));
-            double bb = 0;
-            join_leaves(ctx, RP, SP, plan, o, cur, &bb);
-            HIPCHK(hipEventRecord(ej.b, ctx->stream));
-            launch_fill_u32(ctx->stream, nonempty, 1, 0);
-            k_count_nonempty<<<static_cast<uint32_t>((SP.parts + 255) / 256), 256, 0, ctx->stream>>>(
-                SP.off, static_cast<uint32_t>(SP.parts), nonempty);
-            const uint32_t ne = read_scalar(ctx, nonempty);
-            evictions += ne;
-            refinements += passes.size() > 1 ? p1_parts : 0;
-            peak_bytes = std::max<uint64_t>(peak_bytes, 2 * slice.rows * row_bytes);
-            shuffle_ms += es.ms();
-            probe_ms += ej.ms() - bb;
-            big_build_ms += bb;
-            for (void *k : sk)
-                ctx->arena.free(k);
+    // Output grows geometrically; each slice's exact row count is known
+    // before its rows are written.
+    out = alloc_output(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1));
+    uint64_t n = 0;
+    for (uint64_t a = 0; a < S.rows; a += slice_rows) {
+        DevRel slice = S;
+        slice.rows = std::min(slice_rows, S.rows - a);
+        slice.keys = S.keys + a;
+        for (uint32_t c = 0; c < S.cols.n; ++c)
+            slice.cols.ptr[c] = static_cast<const uint8_t *>(S.cols.ptr[c]) + a * type_size(S.cols.types[c]);
+        EventPair es, ej;
+        HIPCHK(hipEventRecord(es.a, ctx->stream));
+        std::vector<void *> sk;
+        const Partitioned SP = radix_partition(ctx, slice, nullptr, 1, 0, passes, sk, "probe");
+        HIPCHK(hipEventRecord(es.b, ctx->stream));
+        HIPCHK(hipEventRecord(ej.a, ctx->stream));
+        double bb = 0;
+        {
+            LeafJoin lj(ctx, RP, SP, plan);
+            bb = lj.big_build_ms();
+            const uint64_t add = lj.total();
+            if (n + add > out.cap)
+                grow_output(ctx, out, n, std::max(n + add, 2 * out.cap));
+            lj.emit(out, n);
+            n += add;
         }
-        return read_scalar(ctx, reinterpret_cast<const uint64_t *>(cur));
-    };
-    const uint64_t n = join_with_capacity(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1), out, probe_all);
+        HIPCHK(hipEventRecord(ej.b, ctx->stream));
+        launch_fill_u32(ctx->stream, nonempty, 1, 0);
+        k_count_nonempty<<<static_cast<uint32_t>((SP.parts + 255) / 256), 256, 0, ctx->stream>>>(
+            SP.off, static_cast<uint32_t>(SP.parts), nonempty);
+        const uint32_t ne = read_scalar(ctx, nonempty);
+        evictions += ne;
+        refinements += passes.size() > 1 ? p1_parts : 0;
+        peak_bytes = std::max<uint64_t>(peak_bytes, 2 * slice.rows * row_bytes);
+        shuffle_ms += es.ms();
+        probe_ms += ej.ms() - bb;
+        big_build_ms += bb;
+        for (void *k : sk)
+            ctx->arena.free(k);
+    }
     HIPCHK(hipEventRecord(e2e.b, ctx->stream));
     ctx->sync();
     for (void *k : keep)

@@ -214,4 +214,34 @@ struct OutColSet {
     void *ptr[kMaxCols];
 };
 
+// Field c of a by-value kernel argument with a wave-uniform c. Constant
+// indices in every case keep the argument in the kernarg segment instead of
+// copying it to scratch for a dynamic index.
+template <class S>
+__device__ __forceinline__ auto col_ptr(const S &s, uint32_t c) -> decltype(s.ptr[0]) {
+    switch (c) {
+    case 0: return s.ptr[0];
+    case 1: return s.ptr[1];
+    case 2: return s.ptr[2];
+    case 3: return s.ptr[3];
+    case 4: return s.ptr[4];
+    case 5: return s.ptr[5];
+    case 6: return s.ptr[6];
+    default: return s.ptr[7];
+    }
+}
+
+template <class S> __device__ __forceinline__ uint32_t col_type(const S &s, uint32_t c) {
+    switch (c) {
+    case 0: return s.types[0];
+    case 1: return s.types[1];
+    case 2: return s.types[2];
+    case 3: return s.types[3];
+    case 4: return s.types[4];
+    case 5: return s.types[5];
+    case 6: return s.types[6];
+    default: return s.types[7];
+    }
+}
+
 } // namespace phjd

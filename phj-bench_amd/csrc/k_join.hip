@@ -46,13 +46,15 @@ __device__ __forceinline__ void emit_row(const OutSink &out, uint64_t pos, uint6
     out.keys[pos] = key;
     for (uint32_t c = 0; c < rc.n; ++c) {
         uint64_t lo, hi;
-        load_elem(rc.ptr[c], rc.types[c], r, lo, hi);
-        store_elem(out.left.ptr[c], rc.types[c], pos, lo, hi);
+        const uint32_t t = col_type(rc, c);
+        load_elem(col_ptr(rc, c), t, r, lo, hi);
+        store_elem(col_ptr(out.left, c), t, pos, lo, hi);
     }
     for (uint32_t c = 0; c < sc.n; ++c) {
         uint64_t lo, hi;
-        load_elem(sc.ptr[c], sc.types[c], j, lo, hi);
-        store_elem(out.right.ptr[c], sc.types[c], pos, lo, hi);
+        const uint32_t t = col_type(sc, c);
+        load_elem(col_ptr(sc, c), t, j, lo, hi);
+        store_elem(col_ptr(out.right, c), t, pos, lo, hi);
     }
 }
 
@@ -68,42 +70,67 @@ __global__ void k_leaf_work_count(const uint32_t *__restrict__ r_off, const uint
     }
 }
 
+constexpr uint16_t kNoNext = 0xFFFFu;   // end of a chain / probe row without a match
+constexpr uint16_t kMore = 0x8000u;     // the referenced row has a successor in its chain
+constexpr uint16_t kRowMask = 0x7FFFu;  // leaf-local build row (leaves hold < 32768 rows)
+
+__global__ void k_leaf_items(const uint32_t *__restrict__ work_first, uint32_t n_leaves, uint32_t *__restrict__ item_leaf) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n_leaves)
+        for (uint32_t w = work_first[g]; w < work_first[g + 1]; ++w)
+            item_leaf[w] = g;
+}
+
+struct Item {
+    uint32_t leaf, item, r0, nr, s0, s1;
+};
+
+__device__ __forceinline__ bool locate_item(const LeafJoinArgs &a, uint32_t w, Item &it) {
+    if (w >= a.work_first[a.n_leaves])
+        return false;
+    it.leaf = a.item_leaf[w];
+    it.item = w - a.work_first[it.leaf];
+    it.r0 = a.r_off[it.leaf];
+    it.nr = a.r_off[it.leaf + 1] - it.r0;
+    it.s0 = a.s_off[it.leaf] + it.item * a.chunk_rows;
+    it.s1 = min(a.s_off[it.leaf + 1], it.s0 + a.chunk_rows);
+    return true;
+}
+
+// Probe pass over one (leaf, probe chunk) work item: build the leaf's table of
+// 16-byte cells in LDS (64-bit CAS on the key, exchange on the chain head),
+// look up every probe row, and record per probe row the matched build row
+// (heads, 16 bits with a "chain continues" flag) and the item's output rows.
+// Chains are exported as next16 so the emit pass needs no table.
 template <int THREADS, int U>
-__global__ __launch_bounds__(THREADS) void k_leaf_join_lds(LeafJoinArgs a) {
+__global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t tmp[THREADS / 64 + 1];
     __shared__ uint32_t side_head;
-    __shared__ unsigned long long out_base;
-
-    const uint32_t total = a.work_first[a.n_leaves];
+    __shared__ uint32_t blk_chain;
+    Item it;
     const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
-    if (w >= total)
+    if (!locate_item(a, w, it))
         return;
-    const uint32_t leaf = upper_bound_idx(a.work_first, a.n_leaves, w);
-    const uint32_t item = w - a.work_first[leaf];
-    const uint32_t r0 = a.r_off[leaf];
-    const uint32_t nr = a.r_off[leaf + 1] - r0;
-    const uint32_t s0 = a.s_off[leaf] + item * a.chunk_rows;
-    const uint32_t s1 = min(a.s_off[leaf + 1], s0 + a.chunk_rows);
-    const uint32_t cap = table_cap(nr, a.load_pct);
-
+    const uint32_t cap = table_cap(it.nr, a.load_pct);
     LdsCell *cells = reinterpret_cast<LdsCell *>(smem);
-    uint32_t *next = reinterpret_cast<uint32_t *>(smem + static_cast<size_t>(cap) * sizeof(LdsCell));
+    uint16_t *next = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * sizeof(LdsCell));
 
     for (uint32_t i = threadIdx.x; i < cap; i += THREADS) {
         LdsCell c;
         c.key = kEmptyKey;
-        c.block_no = leaf;
+        c.block_no = it.leaf;
         c.row_no = kInvalidRow;
         cells[i] = c;
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         side_head = kInvalidRow;
+        blk_chain = 0;
+    }
     __syncthreads();
 
-    // Build: claim the key slot with a 64-bit CAS, then swap the chain head.
-    for (uint32_t i = threadIdx.x; i < nr; i += THREADS) {
-        const uint64_t k = a.r_keys[r0 + i];
+    for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
+        const uint64_t k = a.r_keys[it.r0 + i];
         uint32_t prev;
         if (k == kEmptyKey) {
             prev = atomicExch(&side_head, i);
@@ -120,58 +147,223 @@ __global__ __launch_bounds__(THREADS) void k_leaf_join_lds(LeafJoinArgs a) {
                 pos = (pos + 1 == cap) ? 0 : pos + 1;
             }
         }
-        next[i] = prev;
+        next[i] = prev == kInvalidRow ? kNoNext : static_cast<uint16_t>(prev);
     }
     __syncthreads();
+    // Export the chains (first work item of the leaf only): successor plus
+    // whether the successor continues.
+    if (it.item == 0) {
+        for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
+            const uint16_t nx = next[i];
+            a.next16[it.r0 + i] = nx == kNoNext ? kNoNext : static_cast<uint16_t>(nx | (next[nx] != kNoNext ? kMore : 0));
+        }
+    }
 
-    // Probe in rounds of THREADS*U rows: find heads, count chain lengths,
-    // reserve one output range per round, expand and gather.
-    for (uint32_t base = s0; base < s1; base += THREADS * U) {
+    uint32_t cnt = 0;
+    bool saw_chain = false;
+    for (uint32_t base = it.s0; base < it.s1; base += THREADS * U) {
         uint64_t key[U];
-        uint32_t head[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = base + u * THREADS + threadIdx.x;
+            key[u] = j < it.s1 ? a.s_keys[j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t j = base + u * THREADS + threadIdx.x;
+            if (j >= it.s1)
+                continue;
+            const uint64_t k = key[u];
+            uint32_t hd = kInvalidRow;
+            if (k == kEmptyKey) {
+                hd = side_head;
+            } else {
+                uint32_t pos = home_slot(int_hash64(k), cap);
+                while (true) {
+                    const LdsCell c = cells[pos];
+                    if (c.key == k) {
+                        hd = c.row_no;
+                        break;
+                    }
+                    if (c.key == kEmptyKey)
+                        break;
+                    pos = (pos + 1 == cap) ? 0 : pos + 1;
+                }
+            }
+            uint16_t h16 = kNoNext;
+            if (hd != kInvalidRow) {
+                h16 = static_cast<uint16_t>(hd | (next[hd] != kNoNext ? kMore : 0));
+                if (h16 & kMore)
+                    saw_chain = true;
+                for (uint32_t x = hd; x != kInvalidRow;) {
+                    ++cnt;
+                    const uint16_t nx = next[x];
+                    x = nx == kNoNext ? kInvalidRow : nx;
+                }
+            }
+            a.heads[j] = h16;
+        }
+    }
+    if (saw_chain)
+        blk_chain = 1u;
+    uint32_t t;
+    block_excl_scan<THREADS>(cnt, tmp, &t);
+    if (threadIdx.x == 0)
+        a.item_count[w] = t;
+    if (threadIdx.x == 0 && blk_chain)
+        atomicOr(a.chain_flag, 1u);
+}
+
+template <class T> __device__ __forceinline__ const T *cptr(const ColSet &s, uint32_t c) {
+    return static_cast<const T *>(col_ptr(s, c));
+}
+template <class T> __device__ __forceinline__ T *optr(const OutColSet &s, uint32_t c) {
+    return static_cast<T *>(col_ptr(s, c));
+}
+
+// Column-major gather for the single-match rows of one round: every slot's
+// load is issued before any store, for one column at a time.
+template <int U, class T>
+__device__ __forceinline__ void gather_col(const T *__restrict__ src, T *__restrict__ dst, const uint32_t (&idx)[U],
+                                           const uint64_t (&pos)[U], const bool (&on)[U]) {
+    T v[U] = {};
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (on[u])
+            v[u] = src[idx[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (on[u])
+            dst[pos[u]] = v[u];
+}
+
+// 16-byte columns as two 8-byte halves (keeps the staging in registers).
+template <int U>
+__device__ __forceinline__ void gather_col128(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+                                              const uint32_t (&idx)[U], const uint64_t (&pos)[U], const bool (&on)[U]) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        uint64_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v[u] = 0;
+            if (on[u])
+                v[u] = src[2 * static_cast<uint64_t>(idx[u]) + half];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u])
+                dst[2 * pos[u] + half] = v[u];
+    }
+}
+
+template <int U>
+__device__ __forceinline__ void gather_set(const ColSet &in, const OutColSet &out, const uint32_t (&idx)[U],
+                                           const uint64_t (&pos)[U], const bool (&on)[U]) {
+    for (uint32_t c = 0; c < in.n; ++c) {
+        switch (col_type(in, c)) {
+        case 0: gather_col<U>(cptr<uint8_t>(in, c), optr<uint8_t>(out, c), idx, pos, on); break;
+        case 1: gather_col<U>(cptr<uint16_t>(in, c), optr<uint16_t>(out, c), idx, pos, on); break;
+        case 2: gather_col<U>(cptr<uint32_t>(in, c), optr<uint32_t>(out, c), idx, pos, on); break;
+        case 3: gather_col<U>(cptr<uint64_t>(in, c), optr<uint64_t>(out, c), idx, pos, on); break;
+        default: gather_col128<U>(cptr<uint64_t>(in, c), optr<uint64_t>(out, c), idx, pos, on); break;
+        }
+    }
+}
+
+// Emit pass: a streaming kernel over the probe rows of each work item using
+// the recorded heads; no table. Output row = probe key | build columns of the
+// matched row | probe columns (src/JoinOps.h:77-125), at the item's offset.
+// CHAINS == false writes every probe row with exactly one match (column by
+// column, all loads of a column issued before its stores) and leaves the
+// positions of multi-match rows unwritten; CHAINS == true recomputes the same
+// offsets and writes only the multi-match rows, walking their chains.
+template <int THREADS, int U, bool CHAINS>
+__global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
+    __shared__ uint32_t tmp[THREADS / 64 + 1];
+    Item it;
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
+    if (!locate_item(a, w, it))
+        return;
+    uint64_t pos0 = a.out_base + a.item_off[w];
+    for (uint32_t base = it.s0; base < it.s1; base += THREADS * U) {
+        uint16_t h[U];
         uint32_t cnt = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t j = base + u * THREADS + threadIdx.x;
-            head[u] = kInvalidRow;
-            key[u] = 0;
-            if (j < s1) {
-                const uint64_t k = a.s_keys[j];
-                key[u] = k;
-                uint32_t hd = kInvalidRow;
-                if (k == kEmptyKey) {
-                    hd = side_head;
-                } else {
-                    uint32_t pos = home_slot(int_hash64(k), cap);
-                    while (true) {
-                        const LdsCell c = cells[pos];
-                        if (c.key == k) {
-                            hd = c.row_no;
-                            break;
-                        }
-                        if (c.key == kEmptyKey)
-                            break;
-                        pos = (pos + 1 == cap) ? 0 : pos + 1;
-                    }
-                }
-                head[u] = hd;
-                for (uint32_t x = hd; x != kInvalidRow; x = next[x])
+            h[u] = j < it.s1 ? a.heads[j] : kNoNext;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (h[u] == kNoNext)
+                continue;
+            ++cnt;
+            if (h[u] & kMore) {
+                for (uint16_t x = a.next16[it.r0 + (h[u] & kRowMask)];; x = a.next16[it.r0 + (x & kRowMask)]) {
                     ++cnt;
+                    if (!(x & kMore))
+                        break;
+                }
             }
         }
         uint32_t round_total;
         const uint32_t off = block_excl_scan<THREADS>(cnt, tmp, &round_total);
         if (round_total == 0)
             continue;
-        if (threadIdx.x == 0)
-            out_base = atomicAdd(a.out.cursor, static_cast<unsigned long long>(round_total));
-        __syncthreads();
-        uint64_t pos = out_base + off;
+        uint64_t pos = pos0 + off;
+        pos0 += round_total;
+        if (!CHAINS) {
+            uint32_t ridx[U], sidx[U];
+            uint64_t opos[U];
+            bool on[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t j = base + u * THREADS + threadIdx.x;
-            for (uint32_t x = head[u]; x != kInvalidRow; x = next[x])
-                emit_row(a.out, pos++, key[u], a.r_cols, r0 + x, a.s_cols, j);
+            for (int u = 0; u < U; ++u) {
+                on[u] = h[u] != kNoNext && !(h[u] & kMore);
+                sidx[u] = base + u * THREADS + threadIdx.x;
+                ridx[u] = it.r0 + (h[u] & kRowMask);
+                opos[u] = pos;
+                if (h[u] != kNoNext) {
+                    pos += 1;
+                    if (h[u] & kMore)
+                        for (uint16_t x = a.next16[ridx[u]];; x = a.next16[it.r0 + (x & kRowMask)]) {
+                            pos += 1;
+                            if (!(x & kMore))
+                                break;
+                        }
+                }
+            }
+            {
+                uint64_t kv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (on[u])
+                        kv[u] = a.s_keys[sidx[u]];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (on[u])
+                        a.out.keys[opos[u]] = kv[u];
+            }
+            gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
+            gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
+        } else {
+            for (int u = 0; u < U; ++u) {
+                if (h[u] == kNoNext)
+                    continue;
+                const uint32_t j = base + u * THREADS + threadIdx.x;
+                if (!(h[u] & kMore)) {
+                    pos += 1;
+                    continue;
+                }
+                const uint64_t k = a.s_keys[j];
+                uint16_t x = h[u];
+                while (true) {
+                    emit_row(a.out, pos++, k, a.r_cols, it.r0 + (x & kRowMask), a.s_cols, j);
+                    if (!(x & kMore))
+                        break;
+                    x = a.next16[it.r0 + (x & kRowMask)];
+                }
+            }
         }
     }
 }
@@ -411,35 +603,60 @@ uint64_t leaf_max_work(uint64_t probe_rows, uint32_t n_leaves, uint32_t chunk_ro
 }
 
 size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct) {
-    return static_cast<size_t>(table_cap(lds_max_rows, load_pct)) * sizeof(LdsCell) + static_cast<size_t>(lds_max_rows) * 4;
+    return static_cast<size_t>(table_cap(lds_max_rows, load_pct)) * sizeof(LdsCell) +
+           ((static_cast<size_t>(lds_max_rows) * 2 + 15) & ~size_t{15});
 }
 
-void leaf_work_plan(hipStream_t s, const LeafJoinArgs &a, uint32_t *work_first, uint32_t *scan_tmp, KernelTimer *tm) {
-    {
-        TimedScope ts(tm, "join_plan", s);
-        k_leaf_work_count<<<(a.n_leaves + 1 + 255) / 256, 256, 0, s>>>(a.r_off, a.s_off, a.n_leaves, a.chunk_rows,
-                                                                        a.lds_max_rows, work_first);
+uint32_t leaf_join_lds_max_rows(uint32_t lds_bytes, uint32_t load_pct) {
+    uint32_t lo = 0, hi = 65535; // chains are 16-bit row indices
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (leaf_join_lds_bytes(mid, load_pct) <= lds_bytes)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
-    scan_u32(s, work_first, a.n_leaves + 1, nullptr, 1, scan_tmp, tm, "join_plan");
+    return lo;
 }
 
-void leaf_join_lds(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_t threads, KernelTimer *tm) {
+void leaf_work_plan(hipStream_t s, const LeafJoinArgs &a, uint32_t *work_first, uint32_t *scan_tmp, uint32_t *item_leaf,
+                    KernelTimer *tm) {
+    TimedScope ts(tm, "join_plan", s);
+    k_leaf_work_count<<<(a.n_leaves + 1 + 255) / 256, 256, 0, s>>>(a.r_off, a.s_off, a.n_leaves, a.chunk_rows,
+                                                                    a.lds_max_rows, work_first);
+    scan_u32(s, work_first, a.n_leaves + 1, nullptr, 1, scan_tmp, nullptr, "join_plan");
+    k_leaf_items<<<(a.n_leaves + 255) / 256, 256, 0, s>>>(work_first, a.n_leaves, item_leaf);
+}
+
+template <int T, int U> static void launch_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, size_t lds) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_probe<T, U>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(lds));
+    k_leaf_probe<T, U><<<grid8(max_work), T, lds, s>>>(a);
+}
+
+void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_t threads, KernelTimer *tm) {
     if (max_work == 0)
         return;
     const size_t lds = leaf_join_lds_bytes(a.lds_max_rows, a.load_pct);
-    TimedScope ts(tm, "leaf_join", s);
-    if (threads >= 1024) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join_lds<1024, 2>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        k_leaf_join_lds<1024, 2><<<grid8(max_work), 1024, lds, s>>>(a);
-    } else if (threads >= 512) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join_lds<512, 4>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        k_leaf_join_lds<512, 4><<<grid8(max_work), 512, lds, s>>>(a);
-    } else {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join_lds<256, 4>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        k_leaf_join_lds<256, 4><<<grid8(max_work), 256, lds, s>>>(a);
+    TimedScope ts(tm, "leaf_probe", s);
+    if (threads >= 1024)
+        launch_probe<1024, 8>(s, a, max_work, lds);
+    else if (threads >= 512)
+        launch_probe<512, 16>(s, a, max_work, lds);
+    else
+        launch_probe<256, 16>(s, a, max_work, lds);
+}
+
+void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm) {
+    if (max_work == 0)
+        return;
+    {
+        TimedScope ts(tm, "leaf_emit", s);
+        k_leaf_emit<256, 8, false><<<grid8(max_work), 256, 0, s>>>(a);
+    }
+    if (chains) {
+        TimedScope ts(tm, "leaf_emit_chains", s);
+        k_leaf_emit<256, 8, true><<<grid8(max_work), 256, 0, s>>>(a);
     }
 }
 

@@ -51,13 +51,15 @@ __global__ __launch_bounds__(256) void k_checksum(uint64_t n, const uint64_t *__
         p.push_le8(keys[i], 8);
         for (uint32_t c = 0; c < left.n; ++c) {
             uint64_t lo, hi;
-            load_elem(left.ptr[c], left.types[c], i, lo, hi);
-            p.push(lo, hi, type_size(left.types[c]));
+            const uint32_t t = col_type(left, c);
+            load_elem(col_ptr(left, c), t, i, lo, hi);
+            p.push(lo, hi, type_size(t));
         }
         for (uint32_t c = 0; c < right.n; ++c) {
             uint64_t lo, hi;
-            load_elem(right.ptr[c], right.types[c], i, lo, hi);
-            p.push(lo, hi, type_size(right.types[c]));
+            const uint32_t t = col_type(right, c);
+            load_elem(col_ptr(right, c), t, i, lo, hi);
+            p.push(lo, hi, type_size(t));
         }
         p.finish();
         s0 += p.d.a;

@@ -5,44 +5,50 @@
 // the hash bit window h >> (64 - consumed - bits) taken from the top of
 // intHash64, one pass at a time, each pass splitting every segment of the
 // previous pass. The CPU version appends to per-partition OutBlock chains; here
-// every pass is histogram -> device-wide exclusive scan -> scatter into one
-// contiguous column per field, so partition p of pass k occupies a contiguous
-// row range and no chains are needed.
+// every pass is histogram -> column scan -> scatter into one contiguous column
+// per field, so partition p of pass k occupies a contiguous row range and no
+// chains are needed.
 //
 // Kernels per pass:
-//   k_seg_tiles  tiles per input segment (exclusive scan, one workgroup)
-//   k_hist       per-tile digit histogram from per-wavefront LDS sub-histograms
-//   scan         exclusive scan over (segment, digit, tile) counts
-//   k_seg_out    output segment offsets
-//   k_scatter    wavefront ballot/mbcnt ranking into LDS, then each column is
-//                staged in LDS in digit order and leaves as contiguous runs
+//   k_seg_tiles     tiles and scan chunks per input segment (one workgroup)
+//   k_hist          per-tile digit counts from per-wavefront LDS sub-histograms,
+//                   stored tile-major (one contiguous row of 2^bits per tile)
+//   k_colscan_chunk per (segment, chunk of tiles): exclusive column scan of the
+//                   tile rows, chunk totals
+//   k_colscan_seg   per segment: chunk bases, digit bases, output segment offsets
+//   k_scatter       LDS-atomic ranking inside the tile, then each column is
+//                   staged in LDS in digit order and leaves as contiguous runs
 #include "kernels.h"
 
 namespace phjd {
 
 namespace {
 
-constexpr int kPartThreads = 256;
-constexpr int kPartRpt = 16;
+constexpr int kPartThreads = 512;
+constexpr int kPartRpt = 8;
 constexpr int kTile = kPartThreads * kPartRpt; // rows per tile
 constexpr int kMaxDigits = 256;                // at most 8 bits per device pass
+constexpr int kChunkTiles = 256;               // tiles per column-scan chunk
 constexpr int kScanThreads = 512;
 constexpr int kScanPer = 8;
 constexpr int kScanBlock = kScanThreads * kScanPer;
 
 struct TileLoc {
-    uint32_t seg, local, ntiles, first;
+    uint32_t tile, seg, local, ntiles, first;
     uint32_t row0, len;
 };
 
-__device__ __forceinline__ bool locate_tile(uint32_t t, const uint32_t *seg_off, const uint32_t *seg_tile_first,
-                                            uint32_t n_seg, TileLoc &L) {
-    if (t >= seg_tile_first[n_seg])
+// seg_first holds, for each segment, the first tile (entries 0..n_seg) and then
+// the first scan chunk (entries n_seg+1 .. 2*n_seg+1).
+__device__ __forceinline__ bool locate_tile(uint32_t t, const uint32_t *seg_off, const uint32_t *seg_first, uint32_t n_seg,
+                                            TileLoc &L) {
+    if (t >= seg_first[n_seg])
         return false;
-    const uint32_t seg = upper_bound_idx(seg_tile_first, n_seg, t);
+    const uint32_t seg = upper_bound_idx(seg_first, n_seg, t);
+    L.tile = t;
     L.seg = seg;
-    L.first = seg_tile_first[seg];
-    L.ntiles = seg_tile_first[seg + 1] - L.first;
+    L.first = seg_first[seg];
+    L.ntiles = seg_first[seg + 1] - L.first;
     L.local = t - L.first;
     L.row0 = seg_off[seg] + L.local * kTile;
     const uint32_t end = seg_off[seg + 1];
@@ -50,24 +56,37 @@ __device__ __forceinline__ bool locate_tile(uint32_t t, const uint32_t *seg_off,
     return true;
 }
 
-// Tiles per segment, exclusively scanned (single workgroup of 1024 threads).
+__device__ __forceinline__ uint32_t n_tiles_of(uint32_t rows) { return (rows + kTile - 1) / kTile; }
+__device__ __forceinline__ uint32_t n_chunks_of(uint32_t tiles) { return (tiles + kChunkTiles - 1) / kChunkTiles; }
+
+// Tiles and chunks per segment, exclusively scanned (single workgroup).
 __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t *__restrict__ seg_off, uint32_t n_seg,
-                                                    uint32_t *__restrict__ seg_tile_first) {
+                                                    uint32_t *__restrict__ seg_first) {
     __shared__ uint32_t tmp[1024 / 64 + 1];
     const uint32_t per = (n_seg + 1023) / 1024;
     const uint32_t a = threadIdx.x * per;
     const uint32_t b = min(n_seg, a + per);
-    uint32_t cnt = 0;
-    for (uint32_t s = a; s < b; ++s)
-        cnt += (seg_off[s + 1] - seg_off[s] + kTile - 1) / kTile;
-    uint32_t total;
-    uint32_t off = block_excl_scan<1024>(cnt, tmp, &total);
+    uint32_t nt = 0, nc = 0;
     for (uint32_t s = a; s < b; ++s) {
-        seg_tile_first[s] = off;
-        off += (seg_off[s + 1] - seg_off[s] + kTile - 1) / kTile;
+        const uint32_t t = n_tiles_of(seg_off[s + 1] - seg_off[s]);
+        nt += t;
+        nc += n_chunks_of(t);
     }
-    if (threadIdx.x == 0)
-        seg_tile_first[n_seg] = total;
+    uint32_t tot_t, tot_c;
+    uint32_t ot = block_excl_scan<1024>(nt, tmp, &tot_t);
+    uint32_t oc = block_excl_scan<1024>(nc, tmp, &tot_c);
+    uint32_t *chunk_first = seg_first + n_seg + 1;
+    for (uint32_t s = a; s < b; ++s) {
+        seg_first[s] = ot;
+        chunk_first[s] = oc;
+        const uint32_t t = n_tiles_of(seg_off[s + 1] - seg_off[s]);
+        ot += t;
+        oc += n_chunks_of(t);
+    }
+    if (threadIdx.x == 0) {
+        seg_first[n_seg] = tot_t;
+        chunk_first[n_seg] = tot_c;
+    }
 }
 
 __global__ void k_seg_single(uint32_t *seg_off, uint32_t rows) {
@@ -75,15 +94,14 @@ __global__ void k_seg_single(uint32_t *seg_off, uint32_t rows) {
     seg_off[1] = rows;
 }
 
-// Per-tile histogram; counts land at hist[first*B + d*ntiles + local] so one
-// flat exclusive scan yields each (segment, digit, tile) output offset.
+// Per-tile digit counts, hist[tile * B + d].
 __global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ seg_off,
-                                                       const uint32_t *__restrict__ seg_tile_first, uint32_t n_seg,
+                                                       const uint32_t *__restrict__ seg_first, uint32_t n_seg,
                                                        uint32_t shift, uint32_t bits, uint32_t *__restrict__ hist) {
     constexpr int W = kPartThreads / 64;
     __shared__ uint32_t sub[W][kMaxDigits];
     TileLoc L;
-    if (!locate_tile(xcd_remap(blockIdx.x, gridDim.x), seg_off, seg_tile_first, n_seg, L))
+    if (!locate_tile(xcd_remap(blockIdx.x, gridDim.x), seg_off, seg_first, n_seg, L))
         return;
     const uint32_t B = 1u << bits;
     const uint32_t mask = B - 1;
@@ -105,11 +123,68 @@ __global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restric
 #pragma unroll
         for (int w = 0; w < W; ++w)
             c += sub[w][d];
-        hist[L.first * B + d * L.ntiles + L.local] = c;
+        hist[L.tile * B + d] = c;
     }
 }
 
-// ---- device-wide exclusive scan of uint32 (three kernels) ----
+// Exclusive column scan of the tile rows of one chunk (within one segment);
+// the chunk's per-digit totals go to chunk_tot[chunk * B + d].
+__global__ __launch_bounds__(kMaxDigits) void k_colscan_chunk(const uint32_t *__restrict__ seg_first, uint32_t n_seg,
+                                                              uint32_t bits, uint32_t *__restrict__ hist,
+                                                              uint32_t *__restrict__ chunk_tot) {
+    const uint32_t *chunk_first = seg_first + n_seg + 1;
+    const uint32_t c = blockIdx.x;
+    if (c >= chunk_first[n_seg])
+        return;
+    const uint32_t B = 1u << bits;
+    const uint32_t s = upper_bound_idx(chunk_first, n_seg, c);
+    const uint32_t k = c - chunk_first[s];
+    const uint32_t t0 = seg_first[s] + k * kChunkTiles;
+    const uint32_t t1 = min(seg_first[s + 1], t0 + kChunkTiles);
+    const uint32_t d = threadIdx.x;
+    if (d >= B)
+        return;
+    uint32_t run = 0;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t v = hist[t * B + d];
+        hist[t * B + d] = run;
+        run += v;
+    }
+    chunk_tot[c * B + d] = run;
+}
+
+// Per segment: turn chunk totals into chunk bases (absolute row positions),
+// and write the output segment offsets out_off[s * B + d].
+__global__ __launch_bounds__(kMaxDigits) void k_colscan_seg(const uint32_t *__restrict__ seg_off,
+                                                            const uint32_t *__restrict__ seg_first, uint32_t n_seg,
+                                                            uint32_t bits, uint32_t *__restrict__ chunk_tot,
+                                                            uint32_t *__restrict__ out_off) {
+    __shared__ uint32_t tmp[kMaxDigits / 64 + 1];
+    const uint32_t *chunk_first = seg_first + n_seg + 1;
+    const uint32_t s = blockIdx.x;
+    const uint32_t B = 1u << bits;
+    const uint32_t d = threadIdx.x;
+    const uint32_t c0 = chunk_first[s], c1 = chunk_first[s + 1];
+    uint32_t run = 0;
+    if (d < B) {
+        for (uint32_t c = c0; c < c1; ++c) {
+            const uint32_t v = chunk_tot[c * B + d];
+            chunk_tot[c * B + d] = run;
+            run += v;
+        }
+    }
+    uint32_t total;
+    const uint32_t base = seg_off[s] + block_excl_scan<kMaxDigits>(d < B ? run : 0u, tmp, &total);
+    if (d < B) {
+        for (uint32_t c = c0; c < c1; ++c)
+            chunk_tot[c * B + d] += base;
+        out_off[s * B + d] = base;
+    }
+    if (s == n_seg - 1 && d == 0)
+        out_off[n_seg * B] = seg_off[n_seg];
+}
+
+// ---- generic device-wide exclusive scan of uint32 (three kernels) ----
 __device__ __forceinline__ uint64_t scan_count(uint64_t max_n, const uint32_t *count_ptr, uint32_t mul) {
     return count_ptr ? static_cast<uint64_t>(*count_ptr) * mul : max_n;
 }
@@ -135,7 +210,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *__
         block_sums[blockIdx.x] = total;
 }
 
-// Exclusive scan of the block sums by one workgroup (serial over chunks).
 __global__ __launch_bounds__(1024) void k_scan_top(uint32_t *__restrict__ block_sums, uint64_t max_n,
                                                    const uint32_t *count_ptr, uint32_t mul) {
     __shared__ uint32_t tmp[1024 / 64 + 1];
@@ -161,7 +235,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanBlock;
     if (base >= n)
         return;
-    // Each thread owns kScanPer consecutive elements.
     const uint64_t my = base + static_cast<uint64_t>(threadIdx.x) * kScanPer;
     uint32_t v[kScanPer];
     uint32_t s = 0;
@@ -180,24 +253,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict
     }
 }
 
-// Output segment offsets: segment s*B + d starts at the scanned offset of its
-// first tile; a segment with no tiles is empty and starts at its input offset.
-__global__ void k_seg_out(const uint32_t *__restrict__ seg_off, const uint32_t *__restrict__ seg_tile_first, uint32_t n_seg,
-                          uint32_t bits, const uint32_t *__restrict__ scanned, uint32_t *__restrict__ out_off) {
-    const uint32_t B = 1u << bits;
-    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const uint64_t total = static_cast<uint64_t>(n_seg) * B;
-    if (g < total) {
-        const uint32_t s = static_cast<uint32_t>(g >> bits);
-        const uint32_t d = static_cast<uint32_t>(g & (B - 1));
-        const uint32_t first = seg_tile_first[s];
-        const uint32_t nt = seg_tile_first[s + 1] - first;
-        out_off[g] = nt > 0 ? scanned[first * B + d * nt] : seg_off[s];
-    }
-    if (g == 0)
-        out_off[total] = seg_off[n_seg];
-}
-
 struct ScatterShared {
     uint64_t stage[kTile];
     uint8_t slot_digit[kTile];
@@ -207,28 +262,23 @@ struct ScatterShared {
     uint32_t scan_tmp[kPartThreads / 64 + 1];
 };
 
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+// Values already in registers are staged in LDS at their digit-ordered slot,
+// then leave in slot order so each digit's rows form one contiguous run.
 template <class T>
-__device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__restrict__ in, T *__restrict__ out,
-                                               const TileLoc &L, const uint16_t (&slot)[kPartRpt]) {
+__device__ __forceinline__ void write_staged(ScatterShared &sh, const T (&v)[kPartRpt], T *__restrict__ out,
+                                             const TileLoc &L, const uint32_t (&slot)[kPartRpt]) {
     constexpr uint32_t kPerRound = sizeof(T) <= 8 ? kTile : kTile * 8 / sizeof(T);
     constexpr uint32_t kRounds = kTile / kPerRound;
-    T v[kPartRpt];
-#pragma unroll
-    for (int r = 0; r < kPartRpt; ++r) {
-        const uint32_t idx = r * kPartThreads + threadIdx.x;
-        if (idx < L.len)
-            v[r] = in[L.row0 + idx];
-    }
     T *st = reinterpret_cast<T *>(sh.stage);
 #pragma unroll
     for (uint32_t round = 0; round < kRounds; ++round) {
         const uint32_t lo = round * kPerRound;
 #pragma unroll
-        for (int r = 0; r < kPartRpt; ++r) {
-            const uint32_t idx = r * kPartThreads + threadIdx.x;
-            if (idx < L.len && slot[r] >= lo && slot[r] < lo + kPerRound)
+        for (int r = 0; r < kPartRpt; ++r)
+            if (slot[r] - lo < kPerRound) // kNoSlot never qualifies
                 st[slot[r] - lo] = v[r];
-        }
         __syncthreads();
         const uint32_t hi = min(L.len, lo + kPerRound);
         for (uint32_t k = lo + threadIdx.x; k < hi; k += kPartThreads) {
@@ -239,15 +289,29 @@ __device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__res
     }
 }
 
+template <class T>
+__device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__restrict__ in, T *__restrict__ out,
+                                               const TileLoc &L, const uint32_t (&slot)[kPartRpt]) {
+    T v[kPartRpt];
+#pragma unroll
+    for (int r = 0; r < kPartRpt; ++r) {
+        const uint32_t idx = r * kPartThreads + threadIdx.x;
+        if (idx < L.len)
+            v[r] = in[L.row0 + idx];
+    }
+    write_staged<T>(sh, v, out, L, slot);
+}
+
 __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
                                                           uint64_t *__restrict__ keys_out, OutColSet cout,
                                                           const uint32_t *__restrict__ seg_off,
-                                                          const uint32_t *__restrict__ seg_tile_first, uint32_t n_seg,
+                                                          const uint32_t *__restrict__ seg_first, uint32_t n_seg,
                                                           uint32_t shift, uint32_t bits,
-                                                          const uint32_t *__restrict__ scanned) {
+                                                          const uint32_t *__restrict__ hist,
+                                                          const uint32_t *__restrict__ chunk_base) {
     __shared__ ScatterShared sh;
     TileLoc L;
-    if (!locate_tile(xcd_remap(blockIdx.x, gridDim.x), seg_off, seg_tile_first, n_seg, L))
+    if (!locate_tile(xcd_remap(blockIdx.x, gridDim.x), seg_off, seg_first, n_seg, L))
         return;
     const uint32_t B = 1u << bits;
     const uint32_t mask = B - 1;
@@ -255,27 +319,17 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
         sh.cursor[i] = 0;
     __syncthreads();
 
-    uint32_t dig[kPartRpt];
+    // Rank each row inside its digit with one LDS atomic (rank << 8 | digit).
+    uint64_t key[kPartRpt];
+    uint32_t slot[kPartRpt];
 #pragma unroll
     for (int r = 0; r < kPartRpt; ++r) {
         const uint32_t idx = r * kPartThreads + threadIdx.x;
-        dig[r] = 0;
+        slot[r] = kNoSlot;
         if (idx < L.len) {
-            const uint64_t h = int_hash64(keys_in[L.row0 + idx]);
-            dig[r] = static_cast<uint32_t>(h >> shift) & mask;
-        }
-    }
-    // Tile histogram with wavefront-aggregated LDS adds.
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int r = 0; r < kPartRpt; ++r) {
-        const uint32_t idx = r * kPartThreads + threadIdx.x;
-        const bool act = idx < L.len;
-        const uint64_t am = __ballot(act);
-        if (act) {
-            const uint64_t peers = match_any(dig[r], bits, am);
-            if (lane == static_cast<uint32_t>(__ffsll(static_cast<long long>(peers)) - 1))
-                atomicAdd(&sh.cursor[dig[r]], static_cast<uint32_t>(__popcll(peers)));
+            key[r] = keys_in[L.row0 + idx];
+            const uint32_t d = static_cast<uint32_t>(int_hash64(key[r]) >> shift) & mask;
+            slot[r] = (atomicAdd(&sh.cursor[d], 1u) << 8) | d;
         }
     }
     __syncthreads();
@@ -284,57 +338,45 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
         uint32_t total;
         const uint32_t off = block_excl_scan<kPartThreads>(c, sh.scan_tmp, &total);
         if (threadIdx.x < B) {
+            const uint32_t chunk = seg_first[n_seg + 1 + L.seg] + L.local / kChunkTiles;
             sh.lstart[threadIdx.x] = off;
-            sh.cursor[threadIdx.x] = off;
-            sh.gbase[threadIdx.x] = scanned[L.first * B + threadIdx.x * L.ntiles + L.local];
+            sh.gbase[threadIdx.x] = hist[L.tile * B + threadIdx.x] + chunk_base[chunk * B + threadIdx.x];
         }
     }
     __syncthreads();
-    // Rank every row inside its digit: peers via ballots, offset via mbcnt,
-    // one LDS atomic per (wavefront, digit) group.
-    uint16_t slot[kPartRpt];
 #pragma unroll
     for (int r = 0; r < kPartRpt; ++r) {
-        const uint32_t idx = r * kPartThreads + threadIdx.x;
-        const bool act = idx < L.len;
-        const uint64_t am = __ballot(act);
-        slot[r] = 0;
-        if (act) {
-            const uint32_t d = dig[r];
-            const uint64_t peers = match_any(d, bits, am);
-            const int leader = __ffsll(static_cast<long long>(peers)) - 1;
-            uint32_t base = 0;
-            if (lane == static_cast<uint32_t>(leader))
-                base = atomicAdd(&sh.cursor[d], static_cast<uint32_t>(__popcll(peers)));
-            base = __shfl(base, leader, 64);
-            const uint32_t s = base + mbcnt(peers);
-            slot[r] = static_cast<uint16_t>(s);
-            sh.slot_digit[s] = static_cast<uint8_t>(d);
+        if (slot[r] != kNoSlot) {
+            const uint32_t d = slot[r] & 0xFFu;
+            const uint32_t sl = sh.lstart[d] + (slot[r] >> 8);
+            sh.slot_digit[sl] = static_cast<uint8_t>(d);
+            slot[r] = sl;
         }
     }
     __syncthreads();
 
-    scatter_column<uint64_t>(sh, keys_in, keys_out, L, slot);
+    write_staged<uint64_t>(sh, key, keys_out, L, slot);
     for (uint32_t c = 0; c < cin.n; ++c) {
-        switch (cin.types[c]) {
+        switch (col_type(cin, c)) {
         case 0:
-            scatter_column<uint8_t>(sh, static_cast<const uint8_t *>(cin.ptr[c]), static_cast<uint8_t *>(cout.ptr[c]), L, slot);
+            scatter_column<uint8_t>(sh, static_cast<const uint8_t *>(col_ptr(cin, c)), static_cast<uint8_t *>(col_ptr(cout, c)),
+                                    L, slot);
             break;
         case 1:
-            scatter_column<uint16_t>(sh, static_cast<const uint16_t *>(cin.ptr[c]), static_cast<uint16_t *>(cout.ptr[c]), L,
-                                     slot);
+            scatter_column<uint16_t>(sh, static_cast<const uint16_t *>(col_ptr(cin, c)),
+                                     static_cast<uint16_t *>(col_ptr(cout, c)), L, slot);
             break;
         case 2:
-            scatter_column<uint32_t>(sh, static_cast<const uint32_t *>(cin.ptr[c]), static_cast<uint32_t *>(cout.ptr[c]), L,
-                                     slot);
+            scatter_column<uint32_t>(sh, static_cast<const uint32_t *>(col_ptr(cin, c)),
+                                     static_cast<uint32_t *>(col_ptr(cout, c)), L, slot);
             break;
         case 3:
-            scatter_column<uint64_t>(sh, static_cast<const uint64_t *>(cin.ptr[c]), static_cast<uint64_t *>(cout.ptr[c]), L,
-                                     slot);
+            scatter_column<uint64_t>(sh, static_cast<const uint64_t *>(col_ptr(cin, c)),
+                                     static_cast<uint64_t *>(col_ptr(cout, c)), L, slot);
             break;
         default:
-            scatter_column<ulonglong2>(sh, static_cast<const ulonglong2 *>(cin.ptr[c]), static_cast<ulonglong2 *>(cout.ptr[c]),
-                                       L, slot);
+            scatter_column<ulonglong2>(sh, static_cast<const ulonglong2 *>(col_ptr(cin, c)),
+                                       static_cast<ulonglong2 *>(col_ptr(cout, c)), L, slot);
             break;
         }
     }
@@ -351,6 +393,8 @@ __global__ void k_u32_to_u64(const uint32_t *__restrict__ in, uint64_t *__restri
 uint32_t radix_tile_rows() { return kTile; }
 
 uint64_t radix_max_tiles(uint64_t rows, uint32_t n_seg) { return (rows + kTile - 1) / kTile + n_seg; }
+
+uint64_t radix_max_chunks(uint64_t rows, uint32_t n_seg) { return radix_max_tiles(rows, n_seg) / kChunkTiles + n_seg; }
 
 uint64_t radix_scan_blocks(uint64_t elems) { return (elems + kScanBlock - 1) / kScanBlock; }
 
@@ -376,28 +420,27 @@ void launch_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out, uint64_
 void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
                 const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, uint32_t shift, uint32_t bits,
                 uint32_t *out_seg_off, const RadixPassScratch &scr, KernelTimer *tm) {
-    const uint32_t B = 1u << bits;
     const uint64_t max_tiles = radix_max_tiles(rows, n_seg);
+    const uint64_t max_chunks = radix_max_chunks(rows, n_seg);
     {
         TimedScope ts(tm, "radix_plan", s);
         k_seg_tiles<<<1, 1024, 0, s>>>(seg_off, n_seg, scr.seg_tile_first);
     }
     if (rows > 0) {
         TimedScope ts(tm, "radix_hist", s);
-        k_hist<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, seg_off, scr.seg_tile_first, n_seg, shift,
-                                                                          bits, scr.hist);
+        k_hist<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, seg_off, scr.seg_tile_first, n_seg, shift, bits, scr.hist);
     }
-    scan_u32(s, scr.hist, max_tiles * B, scr.seg_tile_first + n_seg, B, scr.block_sums, tm, "radix_scan");
     {
-        const uint64_t total = static_cast<uint64_t>(n_seg) * B + 1;
-        TimedScope ts(tm, "radix_plan", s);
-        k_seg_out<<<static_cast<uint32_t>((total + 255) / 256), 256, 0, s>>>(seg_off, scr.seg_tile_first, n_seg, bits,
-                                                                               scr.hist, out_seg_off);
+        TimedScope ts(tm, "radix_scan", s);
+        if (rows > 0)
+            k_colscan_chunk<<<static_cast<uint32_t>(max_chunks), kMaxDigits, 0, s>>>(scr.seg_tile_first, n_seg, bits, scr.hist,
+                                                                                      scr.block_sums);
+        k_colscan_seg<<<n_seg, kMaxDigits, 0, s>>>(seg_off, scr.seg_tile_first, n_seg, bits, scr.block_sums, out_seg_off);
     }
     if (rows > 0) {
         TimedScope ts(tm, "radix_scatter", s);
-        k_scatter<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off,
-                                                                             scr.seg_tile_first, n_seg, shift, bits, scr.hist);
+        k_scatter<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg,
+                                                             shift, bits, scr.hist, scr.block_sums);
     }
 }
 

@@ -42,12 +42,13 @@ void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
 // by hash bits [shift, shift+bits) into 2^bits output segments (output segment
 // s*2^bits + d). Rows stay inside their segment's range.
 struct RadixPassScratch {
-    uint32_t *seg_tile_first; // n_seg + 1
-    uint32_t *hist;           // max_tiles * 2^bits
-    uint32_t *block_sums;     // scan partials
+    uint32_t *seg_tile_first; // 2 * (n_seg + 1): first tile, then first scan chunk, per segment
+    uint32_t *hist;           // radix_max_tiles * 2^bits
+    uint32_t *block_sums;     // radix_max_chunks * 2^bits (chunk bases)
 };
 uint32_t radix_tile_rows();
 uint64_t radix_max_tiles(uint64_t rows, uint32_t n_seg);
+uint64_t radix_max_chunks(uint64_t rows, uint32_t n_seg);
 uint64_t radix_scan_blocks(uint64_t elems);
 void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
                 const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, uint32_t shift, uint32_t bits,
@@ -82,13 +83,25 @@ struct LeafJoinArgs {
     uint32_t lds_max_rows; // leaves with more build rows use the HBM table path
     uint32_t load_pct;
     const uint32_t *work_first; // n_leaves + 1, exclusive scan of work items per leaf
-    OutSink out;
+    const uint32_t *item_leaf;  // leaf of each work item
+    uint32_t *item_count;       // probe pass: output rows per work item (max_work + 1 entries)
+    const uint32_t *item_off;   // emit pass: exclusive scan of item_count
+    uint16_t *heads;            // per probe row: matched leaf-local build row | chain flag
+    uint16_t *next16;           // per build row: chain successor | chain flag
+    uint32_t *chain_flag;       // probe pass sets it when any probe row has several matches
+    OutSink out;                // emit pass: rows land at out_base + item_off[item] + ...
+    uint64_t out_base;
 };
-// Work items per leaf for the LDS path (0 for empty or oversize leaves), scanned.
-void leaf_work_plan(hipStream_t s, const LeafJoinArgs &a, uint32_t *work_first, uint32_t *scan_tmp, KernelTimer *tm);
+// Work items per leaf for the LDS path (0 for empty or oversize leaves),
+// scanned, and the leaf of each item.
+void leaf_work_plan(hipStream_t s, const LeafJoinArgs &a, uint32_t *work_first, uint32_t *scan_tmp, uint32_t *item_leaf,
+                    KernelTimer *tm);
 uint64_t leaf_max_work(uint64_t probe_rows, uint32_t n_leaves, uint32_t chunk_rows);
-void leaf_join_lds(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_t threads, KernelTimer *tm);
+// Probe pass: LDS tables, heads/next16, item_count. Emit pass: the rows.
+void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_t threads, KernelTimer *tm);
+void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm);
 size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct);
+uint32_t leaf_join_lds_max_rows(uint32_t lds_bytes, uint32_t load_pct);
 
 // Global-memory multi-table hash table with 16-byte (key, RowRefCell) cells
 // (src/HashTable.h:35-40): CHJ's 256 sub-tables, oversize PHJ leaves and the
