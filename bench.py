@@ -65,7 +65,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (HBM counter calibration under a profiler)")
+    ap.add_argument("--tune", default="", help="phj_tuning overrides, e.g. flags=12,join_threads=1024")
     args = ap.parse_args()
+    tuning = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(",") if kv)}
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the join on a single MI355X; "
               "multi-GPU runs are not part of it", file=sys.stderr)
@@ -84,7 +86,7 @@ def main():
     cfg = G.RadixConfig(pass_bits)
 
     for w in range(args.warmup):
-        r = ctx.runPHJ(build, probe, cfg)
+        r = ctx.runPHJ(build, probe, cfg, tuning=tuning)
         log(f"warmup {w}: e2e {r.e2e_wall_ms:.2f} ms out_rows {r.out_rows}")
         r.release()
 
@@ -97,7 +99,7 @@ def main():
     for k in range(args.steps):
         if last is not None:
             last.release()
-        last = ctx.runPHJ(build, probe, cfg)
+        last = ctx.runPHJ(build, probe, cfg, tuning=tuning)
         e2e.append(last.e2e_wall_ms)
         phases.append((last.build_shuffle.wall_ms, last.probe_shuffle.wall_ms, last.build.wall_ms, last.probe.wall_ms))
     ctx.synchronize()
@@ -126,13 +128,15 @@ def main():
 
     # Dominant kernel and its algorithmic bytes per launch (mean over the
     # launches of one step). Per step: 2 passes x 2 relations of k_hist and
-    # k_scatter, one k_leaf_join.
+    # k_scatter, one leaf probe pass and one emit pass.
     n = args.steps
     passes = 2
     per_step_bytes = {
         "radix_scatter": passes * 2 * (R * wR + S * wS),  # every pass reads and writes each row once
         "radix_hist": passes * 8 * (R + S),                # every pass reads the keys once
-        "leaf_join": R * wR + S * wS + out_rows * wO,      # rows read once, output written once
+        "leaf_probe": 8 * (R + S),                         # keys of both sides read once
+        # payloads of both sides read once, output written once
+        "leaf_emit": R * (wR - 8) + S * (wS - 8) + out_rows * wO,
     }
     dom = max(ktimes, key=lambda k: ktimes[k][0]) if ktimes else None
     roofline = None

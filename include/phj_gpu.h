@@ -120,6 +120,8 @@ typedef struct phj_tuning {
 
 #define PHJ_TUNE_FORCE_GLOBAL_HT 0x1u /* leaf tables in HBM even when they fit LDS */
 #define PHJ_TUNE_NO_FILTER 0x2u       /* reserved */
+#define PHJ_TUNE_CACHED_LOADS 0x4u    /* radix scatter reads its input with cached loads (default: nontemporal) */
+#define PHJ_TUNE_NT_STORES 0x8u       /* radix scatter writes with nontemporal stores */
 
 typedef struct phj_join_desc {
     uint32_t scheme; /* phj_scheme */

@@ -353,7 +353,8 @@ ColSet as_in(const OutColSet &o) {
 // or the whole relation when in_off == nullptr) by `passes` starting at bit
 // position `consumed` from the top. Result buffers are tracked in `keep`.
 Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_off, uint32_t n_seg, uint32_t consumed,
-                            const std::vector<uint32_t> &passes, std::vector<void *> &keep, const char *) {
+                            const std::vector<uint32_t> &passes, std::vector<void *> &keep, const char *,
+                            uint32_t flags = kRadixNtLoads) {
     Scratch scr(ctx);
     const uint64_t rows = in.rows;
     uint32_t *seg = in_off ? const_cast<uint32_t *>(in_off) : nullptr;
@@ -386,7 +387,8 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
         rs.seg_tile_first = scr.get<uint32_t>(2 * (static_cast<uint64_t>(cur_nseg) + 1));
         rs.hist = scr.get<uint32_t>(max_tiles * B);
         rs.block_sums = scr.get<uint32_t>(radix_max_chunks(rows, cur_nseg) * B);
-        radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, shift, bits, nseg, rs, ctx->tm());
+        radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, shift, bits, nseg, rs, flags,
+                   ctx->tm());
         // previous pass output is dead once this pass has consumed it
         for (void *b : prev_bufs)
             ctx->arena.free(b);
@@ -510,6 +512,12 @@ LeafJoinPlan plan_of(const phj_tuning &t) {
     p.lds_limit_rows = t.lds_leaf_rows;
     p.force_global = (t.flags & PHJ_TUNE_FORCE_GLOBAL_HT) != 0;
     return p;
+}
+
+// Scatter load/store variant of a tuning record: nontemporal loads unless the
+// caller asks for cached ones (the input is read once per pass).
+uint32_t radix_flags(const phj_tuning &t) {
+    return ((t.flags & PHJ_TUNE_CACHED_LOADS) ? 0u : kRadixNtLoads) | ((t.flags & PHJ_TUNE_NT_STORES) ? kRadixNtStores : 0u);
 }
 
 // Joins partitioned R and S leaf by leaf in two phases: construction counts
@@ -746,10 +754,10 @@ void run_phj_pure(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, co
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
     HIPCHK(hipEventRecord(ebs.a, ctx->stream));
     std::vector<void *> keep;
-    const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build");
+    const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
     HIPCHK(hipEventRecord(eps.a, ctx->stream));
-    const Partitioned SP = radix_partition(ctx, S, nullptr, 1, 0, passes, keep, "probe");
+    const Partitioned SP = radix_partition(ctx, S, nullptr, 1, 0, passes, keep, "probe", radix_flags(tun));
     HIPCHK(hipEventRecord(eps.b, ctx->stream));
     HIPCHK(hipEventRecord(ej.a, ctx->stream));
     double big_build_ms = 0;
@@ -871,7 +879,7 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
     HIPCHK(hipEventRecord(ebs.a, ctx->stream));
     std::vector<void *> keep;
-    const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build");
+    const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
 
     double shuffle_ms = 0, probe_ms = 0, big_build_ms = 0;
@@ -892,7 +900,7 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
         EventPair es, ej;
         HIPCHK(hipEventRecord(es.a, ctx->stream));
         std::vector<void *> sk;
-        const Partitioned SP = radix_partition(ctx, slice, nullptr, 1, 0, passes, sk, "probe");
+        const Partitioned SP = radix_partition(ctx, slice, nullptr, 1, 0, passes, sk, "probe", radix_flags(tun));
         HIPCHK(hipEventRecord(es.b, ctx->stream));
         HIPCHK(hipEventRecord(ej.a, ctx->stream));
         double bb = 0;

@@ -134,6 +134,15 @@ __host__ __device__ __forceinline__ uint32_t grid8(uint64_t n) {
     return static_cast<uint32_t>(((n + 7) / 8) * 8);
 }
 
+// Same idea when only `total` of the launched blocks carry work (total known
+// on the device, grid >= 8 * ceil(total / 8)): every XCD gets an equal,
+// contiguous share. Returns an index >= total for blocks without work.
+__device__ __forceinline__ uint32_t xcd_share(uint32_t b, uint32_t total) {
+    const uint32_t per = (total + 7) / 8;
+    const uint32_t q = b / 8;
+    return q < per ? (b % 8) * per + q : 0xFFFFFFFFu;
+}
+
 // ---- row digest (same definition as the oracle's) ----
 struct Digest {
     uint64_t a, c;

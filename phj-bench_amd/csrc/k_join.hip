@@ -109,7 +109,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
     __shared__ uint32_t side_head;
     __shared__ uint32_t blk_chain;
     Item it;
-    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t w = xcd_share(blockIdx.x, a.work_first[a.n_leaves]);
     if (!locate_item(a, w, it))
         return;
     const uint32_t cap = table_cap(it.nr, a.load_pct);
@@ -271,24 +271,73 @@ __device__ __forceinline__ void gather_set(const ColSet &in, const OutColSet &ou
     }
 }
 
+// Output layout of one round of THREADS*U probe rows: first every probe row
+// with exactly one match, slot-major (all lanes' slot 0, then slot 1, ...), so
+// each column store of a slot is one contiguous run; then the rows with
+// several matches, thread by thread. Both emit kernels derive the same layout.
+template <int THREADS, int U>
+struct RoundLayout {
+    uint32_t single_pos[U]; // round-relative position of this thread's slot u (single match)
+    uint32_t chain_pos;     // round-relative position of this thread's first multi-match row
+    uint32_t total;
+};
+
+template <int THREADS, int U>
+__device__ __forceinline__ void round_layout(const uint16_t (&h)[U], const uint32_t (&chainlen)[U],
+                                             uint32_t (*wave_cnt)[THREADS / 64], uint32_t *tmp,
+                                             RoundLayout<THREADS, U> &L) {
+    constexpr int W = THREADS / 64;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64;
+    uint32_t wpre[U];
+    uint32_t chain_cnt = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const bool single = h[u] != kNoNext && !(h[u] & kMore);
+        const uint64_t m = __ballot(single);
+        wpre[u] = mbcnt(m);
+        if (lane == 0)
+            wave_cnt[u][wave] = static_cast<uint32_t>(__popcll(m));
+        chain_cnt += chainlen[u];
+    }
+    uint32_t chain_total;
+    const uint32_t chain_off = block_excl_scan<THREADS>(chain_cnt, tmp, &chain_total); // barriers publish wave_cnt
+    uint32_t running = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t c = wave_cnt[u][w];
+            pre += static_cast<uint32_t>(w) < wave ? c : 0u;
+            tot += c;
+        }
+        L.single_pos[u] = running + pre + wpre[u];
+        running += tot;
+    }
+    L.chain_pos = running + chain_off;
+    L.total = running + chain_total;
+}
+
 // Emit pass: a streaming kernel over the probe rows of each work item using
 // the recorded heads; no table. Output row = probe key | build columns of the
 // matched row | probe columns (src/JoinOps.h:77-125), at the item's offset.
-// CHAINS == false writes every probe row with exactly one match (column by
-// column, all loads of a column issued before its stores) and leaves the
-// positions of multi-match rows unwritten; CHAINS == true recomputes the same
-// offsets and writes only the multi-match rows, walking their chains.
+// CHAINS == false writes the single-match rows column by column (every load of
+// a column issued before its stores); CHAINS == true derives the same layout
+// and writes only the multi-match rows, walking their chains.
 template <int THREADS, int U, bool CHAINS>
 __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     __shared__ uint32_t tmp[THREADS / 64 + 1];
+    __shared__ uint32_t wave_cnt[2][U][THREADS / 64];
     Item it;
-    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t w = xcd_share(blockIdx.x, a.work_first[a.n_leaves]);
     if (!locate_item(a, w, it))
         return;
-    uint64_t pos0 = a.out_base + a.item_off[w];
-    for (uint32_t base = it.s0; base < it.s1; base += THREADS * U) {
+    uint64_t round_base = a.out_base + a.item_off[w];
+    uint32_t parity = 0;
+    for (uint32_t base = it.s0; base < it.s1; base += THREADS * U, parity ^= 1) {
         uint16_t h[U];
-        uint32_t cnt = 0;
+        uint32_t chainlen[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t j = base + u * THREADS + threadIdx.x;
@@ -296,23 +345,21 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (h[u] == kNoNext)
-                continue;
-            ++cnt;
-            if (h[u] & kMore) {
+            chainlen[u] = 0;
+            if (h[u] != kNoNext && (h[u] & kMore)) {
+                uint32_t c = 1;
                 for (uint16_t x = a.next16[it.r0 + (h[u] & kRowMask)];; x = a.next16[it.r0 + (x & kRowMask)]) {
-                    ++cnt;
+                    ++c;
                     if (!(x & kMore))
                         break;
                 }
+                chainlen[u] = c;
             }
         }
-        uint32_t round_total;
-        const uint32_t off = block_excl_scan<THREADS>(cnt, tmp, &round_total);
-        if (round_total == 0)
+        RoundLayout<THREADS, U> L;
+        round_layout<THREADS, U>(h, chainlen, wave_cnt[parity], tmp, L);
+        if (L.total == 0)
             continue;
-        uint64_t pos = pos0 + off;
-        pos0 += round_total;
         if (!CHAINS) {
             uint32_t ridx[U], sidx[U];
             uint64_t opos[U];
@@ -322,23 +369,16 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                 on[u] = h[u] != kNoNext && !(h[u] & kMore);
                 sidx[u] = base + u * THREADS + threadIdx.x;
                 ridx[u] = it.r0 + (h[u] & kRowMask);
-                opos[u] = pos;
-                if (h[u] != kNoNext) {
-                    pos += 1;
-                    if (h[u] & kMore)
-                        for (uint16_t x = a.next16[ridx[u]];; x = a.next16[it.r0 + (x & kRowMask)]) {
-                            pos += 1;
-                            if (!(x & kMore))
-                                break;
-                        }
-                }
+                opos[u] = round_base + L.single_pos[u];
             }
             {
                 uint64_t kv[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u)
+                for (int u = 0; u < U; ++u) {
+                    kv[u] = 0;
                     if (on[u])
                         kv[u] = a.s_keys[sidx[u]];
+                }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                     if (on[u])
@@ -347,14 +387,11 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
             gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
         } else {
+            uint64_t pos = round_base + L.chain_pos;
             for (int u = 0; u < U; ++u) {
-                if (h[u] == kNoNext)
+                if (chainlen[u] == 0)
                     continue;
                 const uint32_t j = base + u * THREADS + threadIdx.x;
-                if (!(h[u] & kMore)) {
-                    pos += 1;
-                    continue;
-                }
                 const uint64_t k = a.s_keys[j];
                 uint16_t x = h[u];
                 while (true) {
@@ -365,6 +402,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                 }
             }
         }
+        round_base += L.total;
     }
 }
 

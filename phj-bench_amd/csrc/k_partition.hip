@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restric
     constexpr int W = kPartThreads / 64;
     __shared__ uint32_t sub[W][kMaxDigits];
     TileLoc L;
-    if (!locate_tile(xcd_remap(blockIdx.x, gridDim.x), seg_off, seg_first, n_seg, L))
+    if (!locate_tile(xcd_share(blockIdx.x, seg_first[n_seg]), seg_off, seg_first, n_seg, L))
         return;
     const uint32_t B = 1u << bits;
     const uint32_t mask = B - 1;
@@ -253,6 +253,33 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t *__restrict
     }
 }
 
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+
+template <bool NT, class T> __device__ __forceinline__ T ld(const T *p) {
+    if constexpr (!NT) {
+        return *p;
+    } else if constexpr (sizeof(T) == 16) {
+        const u64x2 v = __builtin_nontemporal_load(reinterpret_cast<const u64x2 *>(p));
+        T r;
+        __builtin_memcpy(&r, &v, 16);
+        return r;
+    } else {
+        return __builtin_nontemporal_load(p);
+    }
+}
+
+template <bool NT, class T> __device__ __forceinline__ void st(T *p, const T &v) {
+    if constexpr (!NT) {
+        *p = v;
+    } else if constexpr (sizeof(T) == 16) {
+        u64x2 x;
+        __builtin_memcpy(&x, &v, 16);
+        __builtin_nontemporal_store(x, reinterpret_cast<u64x2 *>(p));
+    } else {
+        __builtin_nontemporal_store(v, p);
+    }
+}
+
 struct ScatterShared {
     uint64_t stage[kTile];
     uint8_t slot_digit[kTile];
@@ -266,30 +293,30 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 // Values already in registers are staged in LDS at their digit-ordered slot,
 // then leave in slot order so each digit's rows form one contiguous run.
-template <class T>
+template <bool NTS, class T>
 __device__ __forceinline__ void write_staged(ScatterShared &sh, const T (&v)[kPartRpt], T *__restrict__ out,
                                              const TileLoc &L, const uint32_t (&slot)[kPartRpt]) {
     constexpr uint32_t kPerRound = sizeof(T) <= 8 ? kTile : kTile * 8 / sizeof(T);
     constexpr uint32_t kRounds = kTile / kPerRound;
-    T *st = reinterpret_cast<T *>(sh.stage);
+    T *stg = reinterpret_cast<T *>(sh.stage);
 #pragma unroll
     for (uint32_t round = 0; round < kRounds; ++round) {
         const uint32_t lo = round * kPerRound;
 #pragma unroll
         for (int r = 0; r < kPartRpt; ++r)
             if (slot[r] - lo < kPerRound) // kNoSlot never qualifies
-                st[slot[r] - lo] = v[r];
+                stg[slot[r] - lo] = v[r];
         __syncthreads();
         const uint32_t hi = min(L.len, lo + kPerRound);
         for (uint32_t k = lo + threadIdx.x; k < hi; k += kPartThreads) {
             const uint32_t d = sh.slot_digit[k];
-            out[sh.gbase[d] + (k - sh.lstart[d])] = st[k - lo];
+            st<NTS>(out + sh.gbase[d] + (k - sh.lstart[d]), stg[k - lo]);
         }
         __syncthreads();
     }
 }
 
-template <class T>
+template <bool NTL, bool NTS, class T>
 __device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__restrict__ in, T *__restrict__ out,
                                                const TileLoc &L, const uint32_t (&slot)[kPartRpt]) {
     T v[kPartRpt];
@@ -297,11 +324,12 @@ __device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__res
     for (int r = 0; r < kPartRpt; ++r) {
         const uint32_t idx = r * kPartThreads + threadIdx.x;
         if (idx < L.len)
-            v[r] = in[L.row0 + idx];
+            v[r] = ld<NTL>(in + L.row0 + idx);
     }
-    write_staged<T>(sh, v, out, L, slot);
+    write_staged<NTS, T>(sh, v, out, L, slot);
 }
 
+template <bool NTL, bool NTS>
 __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
                                                           uint64_t *__restrict__ keys_out, OutColSet cout,
                                                           const uint32_t *__restrict__ seg_off,
@@ -311,7 +339,7 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
                                                           const uint32_t *__restrict__ chunk_base) {
     __shared__ ScatterShared sh;
     TileLoc L;
-    if (!locate_tile(xcd_remap(blockIdx.x, gridDim.x), seg_off, seg_first, n_seg, L))
+    if (!locate_tile(xcd_share(blockIdx.x, seg_first[n_seg]), seg_off, seg_first, n_seg, L))
         return;
     const uint32_t B = 1u << bits;
     const uint32_t mask = B - 1;
@@ -327,7 +355,7 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
         const uint32_t idx = r * kPartThreads + threadIdx.x;
         slot[r] = kNoSlot;
         if (idx < L.len) {
-            key[r] = keys_in[L.row0 + idx];
+            key[r] = ld<NTL>(keys_in + L.row0 + idx);
             const uint32_t d = static_cast<uint32_t>(int_hash64(key[r]) >> shift) & mask;
             slot[r] = (atomicAdd(&sh.cursor[d], 1u) << 8) | d;
         }
@@ -355,27 +383,27 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
     }
     __syncthreads();
 
-    write_staged<uint64_t>(sh, key, keys_out, L, slot);
+    write_staged<NTS, uint64_t>(sh, key, keys_out, L, slot);
     for (uint32_t c = 0; c < cin.n; ++c) {
         switch (col_type(cin, c)) {
         case 0:
-            scatter_column<uint8_t>(sh, static_cast<const uint8_t *>(col_ptr(cin, c)), static_cast<uint8_t *>(col_ptr(cout, c)),
+            scatter_column<NTL, NTS, uint8_t>(sh, static_cast<const uint8_t *>(col_ptr(cin, c)), static_cast<uint8_t *>(col_ptr(cout, c)),
                                     L, slot);
             break;
         case 1:
-            scatter_column<uint16_t>(sh, static_cast<const uint16_t *>(col_ptr(cin, c)),
+            scatter_column<NTL, NTS, uint16_t>(sh, static_cast<const uint16_t *>(col_ptr(cin, c)),
                                      static_cast<uint16_t *>(col_ptr(cout, c)), L, slot);
             break;
         case 2:
-            scatter_column<uint32_t>(sh, static_cast<const uint32_t *>(col_ptr(cin, c)),
+            scatter_column<NTL, NTS, uint32_t>(sh, static_cast<const uint32_t *>(col_ptr(cin, c)),
                                      static_cast<uint32_t *>(col_ptr(cout, c)), L, slot);
             break;
         case 3:
-            scatter_column<uint64_t>(sh, static_cast<const uint64_t *>(col_ptr(cin, c)),
+            scatter_column<NTL, NTS, uint64_t>(sh, static_cast<const uint64_t *>(col_ptr(cin, c)),
                                      static_cast<uint64_t *>(col_ptr(cout, c)), L, slot);
             break;
         default:
-            scatter_column<ulonglong2>(sh, static_cast<const ulonglong2 *>(col_ptr(cin, c)),
+            scatter_column<NTL, NTS, ulonglong2>(sh, static_cast<const ulonglong2 *>(col_ptr(cin, c)),
                                        static_cast<ulonglong2 *>(col_ptr(cout, c)), L, slot);
             break;
         }
@@ -419,7 +447,7 @@ void launch_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out, uint64_
 
 void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
                 const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, uint32_t shift, uint32_t bits,
-                uint32_t *out_seg_off, const RadixPassScratch &scr, KernelTimer *tm) {
+                uint32_t *out_seg_off, const RadixPassScratch &scr, uint32_t flags, KernelTimer *tm) {
     const uint64_t max_tiles = radix_max_tiles(rows, n_seg);
     const uint64_t max_chunks = radix_max_chunks(rows, n_seg);
     {
@@ -439,8 +467,11 @@ void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint6
     }
     if (rows > 0) {
         TimedScope ts(tm, "radix_scatter", s);
-        k_scatter<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg,
-                                                             shift, bits, scr.hist, scr.block_sums);
+        const bool ntl = flags & kRadixNtLoads, nts = flags & kRadixNtStores;
+        auto k = ntl ? (nts ? k_scatter<true, true> : k_scatter<true, false>)
+                     : (nts ? k_scatter<false, true> : k_scatter<false, false>);
+        k<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg, shift,
+                                                     bits, scr.hist, scr.block_sums);
     }
 }
 

@@ -50,9 +50,12 @@ uint32_t radix_tile_rows();
 uint64_t radix_max_tiles(uint64_t rows, uint32_t n_seg);
 uint64_t radix_max_chunks(uint64_t rows, uint32_t n_seg);
 uint64_t radix_scan_blocks(uint64_t elems);
+// Scatter variants (phj_tuning.flags bits share these values).
+constexpr uint32_t kRadixNtLoads = 0x4;  // input columns read with nontemporal loads
+constexpr uint32_t kRadixNtStores = 0x8; // output columns written with nontemporal stores
 void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
                 const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, uint32_t shift, uint32_t bits,
-                uint32_t *out_seg_off, const RadixPassScratch &scr, KernelTimer *tm);
+                uint32_t *out_seg_off, const RadixPassScratch &scr, uint32_t flags, KernelTimer *tm);
 
 // Exclusive scan of n uint32 (device count: n = *count_ptr * mul when count_ptr != nullptr).
 void scan_u32(hipStream_t s, uint32_t *data, uint64_t max_n, const uint32_t *count_ptr, uint32_t mul,
