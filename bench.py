@@ -128,12 +128,15 @@ def main():
 
     # Dominant kernel and its algorithmic bytes per launch (mean over the
     # launches of one step). Per step: 2 passes x 2 relations of k_hist and
-    # k_scatter, one leaf probe pass and one emit pass.
+    # k_scatter, then the fused leaf join (or a probe pass and an emit pass).
     n = args.steps
     passes = 2
     per_step_bytes = {
         "radix_scatter": passes * 2 * (R * wR + S * wS),  # every pass reads and writes each row once
         "radix_hist": passes * 8 * (R + S),                # every pass reads the keys once
+        # fused leaf join: both sides' rows read once, output written once
+        "leaf_join": R * wR + S * wS + out_rows * wO,
+        # two-phase leaf join (repeated build keys or PHJ_TUNE_TWO_PHASE_JOIN)
         "leaf_probe": 8 * (R + S),                         # keys of both sides read once
         # payloads of both sides read once, output written once
         "leaf_emit": R * (wR - 8) + S * (wS - 8) + out_rows * wO,

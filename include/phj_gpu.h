@@ -119,7 +119,7 @@ typedef struct phj_tuning {
 } phj_tuning;
 
 #define PHJ_TUNE_FORCE_GLOBAL_HT 0x1u /* leaf tables in HBM even when they fit LDS */
-#define PHJ_TUNE_NO_FILTER 0x2u       /* reserved */
+#define PHJ_TUNE_TWO_PHASE_JOIN 0x2u  /* leaf join as a count pass plus an exact-offset emit pass */
 #define PHJ_TUNE_CACHED_LOADS 0x4u    /* radix scatter reads its input with cached loads (default: nontemporal) */
 #define PHJ_TUNE_NT_STORES 0x8u       /* radix scatter writes with nontemporal stores */
 

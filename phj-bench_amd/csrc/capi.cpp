@@ -499,6 +499,7 @@ struct LeafJoinPlan {
     uint32_t threads = 512;
     uint32_t lds_limit_rows = 0;
     bool force_global = false;
+    bool two_phase = false; // count pass + exact-offset emit even without repeated build keys
 };
 
 LeafJoinPlan plan_of(const phj_tuning &t) {
@@ -511,6 +512,7 @@ LeafJoinPlan plan_of(const phj_tuning &t) {
         p.threads = t.join_threads;
     p.lds_limit_rows = t.lds_leaf_rows;
     p.force_global = (t.flags & PHJ_TUNE_FORCE_GLOBAL_HT) != 0;
+    p.two_phase = (t.flags & PHJ_TUNE_TWO_PHASE_JOIN) != 0;
     return p;
 }
 
@@ -548,41 +550,47 @@ public:
         a_.s_cols = as_in(S.cols);
         a_.s_off = S.off;
         a_.n_leaves = n_leaves;
-        a_.chunk_rows = plan.chunk_rows
-                            ? plan.chunk_rows
-                            : static_cast<uint32_t>(std::max<uint64_t>(16384, 2 * (S.rows / std::max<uint64_t>(n_leaves, 1))));
         a_.lds_max_rows = std::max<uint32_t>(lds_max_, 1);
         a_.load_pct = plan.load_pct;
-        if (lds_max_ > 0) {
-            uint32_t *work_first = scr_.get<uint32_t>(n_leaves + 1);
-            uint32_t *scan_tmp = scr_.get<uint32_t>(radix_scan_blocks(n_leaves + 1) + 1);
-            max_work_ = leaf_max_work(S.rows, n_leaves, a_.chunk_rows);
-            uint32_t *item_leaf = scr_.get<uint32_t>(max_work_);
-            a_.work_first = work_first;
-            a_.item_leaf = item_leaf;
-            leaf_work_plan(ctx->stream, a_, work_first, scan_tmp, item_leaf, ctx->tm());
-            uint32_t *counts = scr_.get<uint32_t>(max_work_ + 1);
-            launch_fill_u32(ctx->stream, counts, max_work_ + 1, 0);
-            a_.item_count = counts;
-            a_.heads = scr_.get<uint16_t>(S.rows);
-            a_.next16 = scr_.get<uint16_t>(R.rows);
-            a_.chain_flag = scr_.get<uint32_t>(1);
-            launch_fill_u32(ctx->stream, a_.chain_flag, 1, 0);
-            leaf_probe(ctx->stream, a_, max_work_, plan.threads, ctx->tm());
-            uint32_t *tmp = scr_.get<uint32_t>(radix_scan_blocks(max_work_ + 1) + 1);
-            scan_u32(ctx->stream, counts, max_work_ + 1, nullptr, 1, tmp, ctx->tm(), "join_plan");
-            a_.item_off = counts;
-            lds_rows_ = read_scalar(ctx, counts + max_work_);
-            chains_ = read_scalar(ctx, a_.chain_flag) != 0;
+        max_leaf_ = max_leaf;
+        // Every leaf in LDS: try the fused join first. It is exact when no
+        // build key repeats (each probe row then has at most one match, so
+        // S.rows bounds the output); a repeated key sends emit() back here.
+        fused_ = !plan.two_phase && lds_max_ > 0 && max_leaf <= lds_max_;
+        if (fused_) {
+            plan_work(leaf_join_fused_chunk());
+            bound_ = S.rows;
+            return;
         }
-        if (max_leaf > lds_max_)
-            plan_big_leaves(n_leaves);
+        plan_two_phase();
     }
 
-    uint64_t total() const { return lds_rows_ + big_rows_; }
+    // Sentinel of emit(): the fused attempt met a repeated build key; the
+    // two-phase plan is now in place, bound() is exact, call emit() again.
+    static constexpr uint64_t kRetry = ~0ull;
+
+    // Output rows emit() may write: exact for the two-phase join, S.rows for
+    // the fused one.
+    uint64_t bound() const { return bound_; }
     double big_build_ms() const { return big_build_ms_; }
 
-    void emit(const OutBuffers &out, uint64_t base) {
+    // Writes the join's rows from row `base` of `out` (capacity >= base +
+    // bound()) and returns how many.
+    uint64_t emit(const OutBuffers &out, uint64_t base) {
+        if (fused_) {
+            unsigned long long *cursor = scr_.get<unsigned long long>(1);
+            launch_fill_u64(ctx_->stream, reinterpret_cast<uint64_t *>(cursor), 1, base);
+            a_.out = sink_of(out, cursor);
+            a_.chain_flag = scr_.get<uint32_t>(1);
+            launch_fill_u32(ctx_->stream, a_.chain_flag, 1, 0);
+            leaf_join_fused(ctx_->stream, a_, max_work_, ctx_->tm());
+            const uint64_t rows = read_scalar(ctx_, reinterpret_cast<const uint64_t *>(cursor)) - base;
+            if (read_scalar(ctx_, a_.chain_flag) == 0)
+                return rows;
+            fused_ = false;
+            plan_two_phase();
+            return kRetry;
+        }
         if (lds_max_ > 0 && lds_rows_ > 0) {
             a_.out = sink_of(out, nullptr);
             a_.out_base = base;
@@ -594,9 +602,50 @@ public:
             ht_probe_emit(ctx_->stream, ht_, kHtLeaves, hp_, probe_big_, as_in(R_.cols), as_in(S_.cols),
                           sink_of(out, cursor), ctx_->tm(), "leaf_ht_probe");
         }
+        return lds_rows_ + big_rows_;
     }
 
 private:
+    // Work items of the LDS leaves: probe chunks of `chunk` rows.
+    void plan_work(uint32_t chunk) {
+        const uint32_t n_leaves = a_.n_leaves;
+        a_.chunk_rows = chunk;
+        uint32_t *work_first = scr_.get<uint32_t>(n_leaves + 1);
+        uint32_t *scan_tmp = scr_.get<uint32_t>(radix_scan_blocks(n_leaves + 1) + 1);
+        max_work_ = leaf_max_work(S_.rows, n_leaves, chunk);
+        uint32_t *item_leaf = scr_.get<uint32_t>(max_work_);
+        a_.work_first = work_first;
+        a_.item_leaf = item_leaf;
+        leaf_work_plan(ctx_->stream, a_, work_first, scan_tmp, item_leaf, ctx_->tm());
+    }
+
+    // Count pass (LDS leaves: probe kernel and scan; oversize leaves: HBM
+    // tables probed in count-only mode) giving exact output offsets.
+    void plan_two_phase() {
+        const uint32_t n_leaves = a_.n_leaves;
+        if (lds_max_ > 0) {
+            plan_work(plan_.chunk_rows ? plan_.chunk_rows
+                                       : static_cast<uint32_t>(std::max<uint64_t>(
+                                             16384, 2 * (S_.rows / std::max<uint32_t>(n_leaves, 1)))));
+            uint32_t *counts = scr_.get<uint32_t>(max_work_ + 1);
+            launch_fill_u32(ctx_->stream, counts, max_work_ + 1, 0);
+            a_.item_count = counts;
+            a_.heads = scr_.get<uint16_t>(S_.rows);
+            a_.next16 = scr_.get<uint16_t>(R_.rows);
+            a_.chain_flag = scr_.get<uint32_t>(1);
+            launch_fill_u32(ctx_->stream, a_.chain_flag, 1, 0);
+            leaf_probe(ctx_->stream, a_, max_work_, plan_.threads, ctx_->tm());
+            uint32_t *tmp = scr_.get<uint32_t>(radix_scan_blocks(max_work_ + 1) + 1);
+            scan_u32(ctx_->stream, counts, max_work_ + 1, nullptr, 1, tmp, ctx_->tm(), "join_plan");
+            a_.item_off = counts;
+            lds_rows_ = read_scalar(ctx_, counts + max_work_);
+            chains_ = read_scalar(ctx_, a_.chain_flag) != 0;
+        }
+        if (max_leaf_ > lds_max_)
+            plan_big_leaves(n_leaves);
+        bound_ = lds_rows_ + big_rows_;
+    }
+
     // HBM tables for the oversize leaves: table b = b-th listed leaf.
     void plan_big_leaves(uint32_t n_leaves) {
         uint32_t *leaf_of = scr_.get<uint32_t>(n_leaves);
@@ -682,6 +731,9 @@ private:
     uint64_t probe_big_ = 0;
     uint64_t big_rows_ = 0;
     double big_build_ms_ = 0;
+    bool fused_ = false;
+    uint64_t bound_ = 0;
+    uint32_t max_leaf_ = 0;
 };
 
 } // namespace
@@ -765,10 +817,12 @@ void run_phj_pure(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, co
     uint64_t n = 0;
     {
         LeafJoin lj(ctx, RP, SP, plan);
-        n = lj.total();
+        out = alloc_output(ctx, R.cols, S.cols, lj.bound());
+        while ((n = lj.emit(out, 0)) == LeafJoin::kRetry) {
+            free_output(ctx, out);
+            out = alloc_output(ctx, R.cols, S.cols, lj.bound());
+        }
         big_build_ms = lj.big_build_ms();
-        out = alloc_output(ctx, R.cols, S.cols, n);
-        lj.emit(out, 0);
     }
     HIPCHK(hipEventRecord(ej.b, ctx->stream));
     HIPCHK(hipEventRecord(e2e.b, ctx->stream));
@@ -906,12 +960,13 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
         double bb = 0;
         {
             LeafJoin lj(ctx, RP, SP, plan);
-            bb = lj.big_build_ms();
-            const uint64_t add = lj.total();
-            if (n + add > out.cap)
-                grow_output(ctx, out, n, std::max(n + add, 2 * out.cap));
-            lj.emit(out, n);
+            uint64_t add;
+            do {
+                if (n + lj.bound() > out.cap)
+                    grow_output(ctx, out, n, std::max(n + lj.bound(), 2 * out.cap));
+            } while ((add = lj.emit(out, n)) == LeafJoin::kRetry);
             n += add;
+            bb = lj.big_build_ms();
         }
         HIPCHK(hipEventRecord(ej.b, ctx->stream));
         launch_fill_u32(ctx->stream, nonempty, 1, 0);

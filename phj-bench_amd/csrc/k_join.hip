@@ -97,25 +97,13 @@ __device__ __forceinline__ bool locate_item(const LeafJoinArgs &a, uint32_t w, I
     return true;
 }
 
-// Probe pass over one (leaf, probe chunk) work item: build the leaf's table of
-// 16-byte cells in LDS (64-bit CAS on the key, exchange on the chain head),
-// look up every probe row, and record per probe row the matched build row
-// (heads, 16 bits with a "chain continues" flag) and the item's output rows.
-// Chains are exported as next16 so the emit pass needs no table.
-template <int THREADS, int U>
-__global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ uint32_t tmp[THREADS / 64 + 1];
-    __shared__ uint32_t side_head;
-    __shared__ uint32_t blk_chain;
-    Item it;
-    const uint32_t w = xcd_share(blockIdx.x, a.work_first[a.n_leaves]);
-    if (!locate_item(a, w, it))
-        return;
-    const uint32_t cap = table_cap(it.nr, a.load_pct);
-    LdsCell *cells = reinterpret_cast<LdsCell *>(smem);
-    uint16_t *next = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * sizeof(LdsCell));
-
+// Builds the leaf's open-addressing table of 16-byte cells in LDS (64-bit CAS
+// on the key, exchange on the chain head; src/HashTable.h:84-109) and the
+// leaf-local chain successors next[] (src/BlockStore.h:86-89). Ends with a
+// barrier. Returns true in every thread when some key was inserted twice.
+template <int THREADS>
+__device__ __forceinline__ void build_leaf_table(const LeafJoinArgs &a, const Item &it, uint32_t cap, LdsCell *cells,
+                                                 uint16_t *next, uint32_t &side_head, uint32_t &dup) {
     for (uint32_t i = threadIdx.x; i < cap; i += THREADS) {
         LdsCell c;
         c.key = kEmptyKey;
@@ -125,10 +113,11 @@ __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
     }
     if (threadIdx.x == 0) {
         side_head = kInvalidRow;
-        blk_chain = 0;
+        dup = 0;
     }
     __syncthreads();
 
+    bool seen = false;
     for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
         const uint64_t k = a.r_keys[it.r0 + i];
         uint32_t prev;
@@ -147,9 +136,50 @@ __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
                 pos = (pos + 1 == cap) ? 0 : pos + 1;
             }
         }
+        seen |= prev != kInvalidRow;
         next[i] = prev == kInvalidRow ? kNoNext : static_cast<uint16_t>(prev);
     }
+    if (seen)
+        dup = 1u;
     __syncthreads();
+}
+
+// Head of key k's chain (leaf-local build row) or kInvalidRow.
+__device__ __forceinline__ uint32_t lookup_leaf(const LdsCell *cells, uint32_t cap, uint32_t side_head, uint64_t k) {
+    if (k == kEmptyKey)
+        return side_head;
+    uint32_t pos = home_slot(int_hash64(k), cap);
+    while (true) {
+        const LdsCell c = cells[pos];
+        if (c.key == k)
+            return c.row_no;
+        if (c.key == kEmptyKey)
+            return kInvalidRow;
+        pos = (pos + 1 == cap) ? 0 : pos + 1;
+    }
+}
+
+// Probe pass over one (leaf, probe chunk) work item: build the leaf's table,
+// look up every probe row, and record per probe row the matched build row
+// (heads, 16 bits with a "chain continues" flag) and the item's output rows.
+// Chains are exported as next16 so the emit pass needs no table.
+template <int THREADS, int U>
+__global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t tmp[THREADS / 64 + 1];
+    __shared__ uint32_t side_head;
+    __shared__ uint32_t dup;
+    __shared__ uint32_t blk_chain;
+    Item it;
+    const uint32_t w = xcd_share(blockIdx.x, a.work_first[a.n_leaves]);
+    if (!locate_item(a, w, it))
+        return;
+    const uint32_t cap = table_cap(it.nr, a.load_pct);
+    LdsCell *cells = reinterpret_cast<LdsCell *>(smem);
+    uint16_t *next = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * sizeof(LdsCell));
+    if (threadIdx.x == 0)
+        blk_chain = 0;
+    build_leaf_table<THREADS>(a, it, cap, cells, next, side_head, dup);
     // Export the chains (first work item of the leaf only): successor plus
     // whether the successor continues.
     if (it.item == 0) {
@@ -173,23 +203,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
             const uint32_t j = base + u * THREADS + threadIdx.x;
             if (j >= it.s1)
                 continue;
-            const uint64_t k = key[u];
-            uint32_t hd = kInvalidRow;
-            if (k == kEmptyKey) {
-                hd = side_head;
-            } else {
-                uint32_t pos = home_slot(int_hash64(k), cap);
-                while (true) {
-                    const LdsCell c = cells[pos];
-                    if (c.key == k) {
-                        hd = c.row_no;
-                        break;
-                    }
-                    if (c.key == kEmptyKey)
-                        break;
-                    pos = (pos + 1 == cap) ? 0 : pos + 1;
-                }
-            }
+            const uint32_t hd = lookup_leaf(cells, cap, side_head, key[u]);
             uint16_t h16 = kNoNext;
             if (hd != kInvalidRow) {
                 h16 = static_cast<uint16_t>(hd | (next[hd] != kNoNext ? kMore : 0));
@@ -403,6 +417,144 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             }
         }
         round_base += L.total;
+    }
+}
+
+// Cell index of key k in the leaf table, kSideSlot for the empty-key
+// sentinel's side chain, or kInvalidRow.
+constexpr uint32_t kSideSlot = 0xFFFEu;
+constexpr int kFusedRowsPerThread = 32;
+__device__ __forceinline__ uint32_t lookup_slot(const LdsCell *cells, uint32_t cap, uint32_t side_head, uint64_t k) {
+    if (k == kEmptyKey)
+        return side_head == kInvalidRow ? kInvalidRow : kSideSlot;
+    uint32_t pos = home_slot(int_hash64(k), cap);
+    while (true) {
+        const uint64_t ck = cells[pos].key;
+        if (ck == k)
+            return pos;
+        if (ck == kEmptyKey)
+            return kInvalidRow;
+        pos = (pos + 1 == cap) ? 0 : pos + 1;
+    }
+}
+
+// Fused leaf join for build sides without repeated keys (every probe row has
+// at most one match). One workgroup per (leaf, probe chunk of up to
+// THREADS*K rows): the chunk's probe keys are loaded first, the leaf's LDS
+// table is built meanwhile, every row is looked up (the matched cell index
+// is kept in registers), the item's output rows are reserved with one atomic
+// add on the sink cursor, and rounds of THREADS*U rows are written slot-major
+// (the emit pass's single-match layout). The output key is the matched
+// cell's key. Rows at or beyond the sink capacity are counted, not written.
+// A leaf with a repeated key raises *chain_flag and writes nothing: the
+// caller then joins with the two-phase path.
+template <int THREADS, int K, int U>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_leaf_join(LeafJoinArgs a) {
+    static_assert(K % U == 0 && K % 2 == 0, "rounds of U slots");
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t tmp[THREADS / 64 + 1];
+    __shared__ uint32_t wave_cnt[2][U][THREADS / 64];
+    __shared__ uint32_t side_head;
+    __shared__ uint32_t dup;
+    __shared__ unsigned long long item_base;
+    Item it;
+    const uint32_t w = xcd_share(blockIdx.x, a.work_first[a.n_leaves]);
+    if (!locate_item(a, w, it))
+        return;
+    const uint32_t cap = table_cap(it.nr, a.load_pct);
+    LdsCell *cells = reinterpret_cast<LdsCell *>(smem);
+    uint16_t *next = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * sizeof(LdsCell));
+
+    // The first KP probe keys are loaded before the table is built.
+    constexpr int KP = K / 2;
+    uint64_t key[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const uint32_t j = it.s0 + k * THREADS + threadIdx.x;
+        key[k] = j < it.s1 ? a.s_keys[j] : 0;
+    }
+    build_leaf_table<THREADS>(a, it, cap, cells, next, side_head, dup);
+    if (dup) {
+        if (threadIdx.x == 0)
+            atomicOr(a.chain_flag, 1u);
+        return;
+    }
+    // slot code per row, two per word: 0 = no match, else cell index + 1
+    uint32_t code[K / 2];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t j = it.s0 + k * THREADS + threadIdx.x;
+        if (k >= KP && k % KP == 0) {
+            // the next KP keys, loaded once the previous ones are consumed
+#pragma unroll
+            for (int q = 0; q < KP; ++q) {
+                const uint32_t jq = j + q * THREADS;
+                key[q] = jq < it.s1 ? a.s_keys[jq] : 0;
+            }
+        }
+        const uint32_t sl = j < it.s1 ? lookup_slot(cells, cap, side_head, key[k % KP]) : kInvalidRow;
+        const uint32_t c = sl == kInvalidRow ? 0u : sl + 1;
+        cnt += c != 0;
+        if (k % 2 == 0)
+            code[k / 2] = c;
+        else
+            code[k / 2] |= c << 16;
+    }
+    uint32_t total;
+    block_excl_scan<THREADS>(cnt, tmp, &total);
+    if (total == 0)
+        return;
+    if (threadIdx.x == 0)
+        item_base = atomicAdd(a.out.cursor, static_cast<unsigned long long>(total));
+    __syncthreads();
+    uint64_t rb = item_base;
+    const uint64_t out_cap = a.out.cap;
+#pragma unroll
+    for (int r = 0; r < K / U; ++r) {
+        uint16_t h[U];
+        uint32_t none[U];
+        uint32_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = r * U + u;
+            c[u] = (code[k / 2] >> (16 * (k % 2))) & 0xFFFFu;
+            h[u] = c[u] ? 0 : kNoNext;
+            none[u] = 0;
+        }
+        RoundLayout<THREADS, U> L;
+        round_layout<THREADS, U>(h, none, wave_cnt[r & 1], tmp, L);
+        if (L.total == 0)
+            continue;
+        uint32_t ridx[U], sidx[U];
+        uint64_t opos[U], okey[U];
+        bool on[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            opos[u] = rb + L.single_pos[u];
+            on[u] = c[u] != 0 && opos[u] < out_cap;
+            sidx[u] = it.s0 + (r * U + u) * THREADS + threadIdx.x;
+            uint32_t row = 0;
+            okey[u] = 0;
+            if (on[u]) {
+                if (c[u] - 1 == kSideSlot) {
+                    okey[u] = kEmptyKey;
+                    row = side_head;
+                } else {
+                    const LdsCell cell = cells[c[u] - 1];
+                    okey[u] = cell.key;
+                    row = cell.row_no;
+                }
+            }
+            ridx[u] = it.r0 + row;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u])
+                a.out.keys[opos[u]] = okey[u];
+        gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
+        gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
+        rb += L.total;
     }
 }
 
@@ -683,6 +835,18 @@ void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_
         launch_probe<512, 16>(s, a, max_work, lds);
     else
         launch_probe<256, 16>(s, a, max_work, lds);
+}
+
+uint32_t leaf_join_fused_chunk() { return 512 * kFusedRowsPerThread; }
+
+void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm) {
+    if (max_work == 0)
+        return;
+    const size_t lds = leaf_join_lds_bytes(a.lds_max_rows, a.load_pct);
+    TimedScope ts(tm, "leaf_join", s);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 4>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(lds));
+    k_leaf_join<512, kFusedRowsPerThread, 4><<<grid8(max_work), 512, lds, s>>>(a);
 }
 
 void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm) {

@@ -103,6 +103,12 @@ uint64_t leaf_max_work(uint64_t probe_rows, uint32_t n_leaves, uint32_t chunk_ro
 // Probe pass: LDS tables, heads/next16, item_count. Emit pass: the rows.
 void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_t threads, KernelTimer *tm);
 void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm);
+// Fused probe + emit for build sides without repeated keys: one output
+// reservation per work item on out.cursor (rows at or beyond out.cap are
+// counted only); a leaf with a repeated key sets *chain_flag and writes
+// nothing. Work items must hold at most leaf_join_fused_chunk() probe rows.
+uint32_t leaf_join_fused_chunk();
+void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm);
 size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct);
 uint32_t leaf_join_lds_max_rows(uint32_t lds_bytes, uint32_t load_pct);
 

@@ -228,6 +228,38 @@ def test_phj_pure_tuning_paths(ctx, flags):
     res.release()
 
 
+def unique_with_sentinels(rng, n_build, n_probe):
+    """Distinct build keys including 0 and 2^64-1, probe keys half hits."""
+    keys = np.unique(rng.integers(0, 2**64, n_build, dtype=np.uint64, endpoint=False))
+    keys = np.unique(np.concatenate([keys, np.array([0, 2**64 - 1], dtype=np.uint64)]))
+    rng.shuffle(keys)
+    b = random_relation(rng, keys.size, [G.U32, G.U128])
+    b = O.Relation(keys, b.cols, b.types)
+    hit = rng.random(n_probe) < 0.5
+    pk = np.where(hit, keys[rng.integers(0, keys.size, n_probe)], rng.integers(0, 2**64, n_probe, dtype=np.uint64))
+    pk[:4] = [2**64 - 1, 0, 2**64 - 1, 5]
+    p = random_relation(rng, n_probe, [G.U16, G.U64])
+    return b, O.Relation(pk.astype(np.uint64), p.cols, p.types)
+
+
+@pytest.mark.parametrize("pass_bits", [[4], [2, 2], [8]])
+@pytest.mark.parametrize("flags", [0, 2])
+def test_fused_and_two_phase_leaf_join(ctx, pass_bits, flags):
+    """Unique build keys (the fused leaf join) against the forced two-phase
+    join, with leaves whose probe side spans several work items and the
+    empty-key sentinel on both sides."""
+    rng = np.random.default_rng(21)
+    ob, op = unique_with_sentinels(rng, 4000, 300000)
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(pass_bits), tuning=dict(flags=flags))
+    assert res.out_rows == want.shape[0]
+    assert np.array_equal(sorted_rows_of(res.to_host()), want)
+    res.release()
+    res = ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig(pass_bits), bep_budget_mib=1, tuning=dict(flags=flags))
+    assert res.checksum() == O.checksum_rows(want)
+    res.release()
+
+
 def test_join_rejects_bad_radix(ctx):
     ob, op = data()["uniform_half"]
     with pytest.raises(ValueError):
