@@ -4,9 +4,15 @@ radix 2 passes x 256 partitions, load factor 0.5, on one MI355X.
 
 One step = one full runPHJ (build shuffle, probe shuffle, per-leaf build and
 probe, output materialised in HBM) over inputs already resident in HBM.
+The line's "batched" object measures 4096 small joins (|R_i| = 2^U[10,20],
+|S_i| = 8|R_i|) as one phj_join_batch call against one runPHJ call per join;
+"skew" times PHJ-PURE on a Zipf-skewed join (2^22 keys with repeated build
+rows vs 2^27 probes) and checks CHJ and PHJ agree with it; "serving" runs
+open-loop Poisson requests through the serving loop (phj-bench_amd/serve.py)
+at several offered rates (p50/p99 latency).
 Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--scale-log2 0] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--scale-log2 0] [--no-cpu-baseline] [--batch-joins 4096]
 """
 import argparse
 import json
@@ -55,6 +61,141 @@ def cpu_baseline(scale_log2, threads):
                       f"distribution and schema, pass_bits {{8,8}}, best of 2 after 1 warm-up"}
 
 
+def batched_bench(ctx, n_joins, seed=42):
+    """4096 independent uniform joins (|R_i| = floor(2^U[10,20]) distinct keys,
+    |S_i| = 8 |R_i|, 75% match, u64 payload each side), back to back in HBM:
+    one phj_join_batch call against one runPHJ call per join. Per-join
+    checksums of the two must agree."""
+    specs = G.batched_join_specs(seed=seed, n_joins=n_joins)
+    build, probe, boff, poff = ctx.generate_batch(specs)
+    pairs = [(build.slice(int(boff[i]), int(boff[i + 1])), probe.slice(int(poff[i]), int(poff[i + 1])))
+             for i in range(n_joins)]
+    cfgs = [G.RadixConfig.for_build_rows(b.rows) for b, _ in pairs]
+    R, S = build.rows, probe.rows
+    log(f"batch: {n_joins} joins, {R} build + {S} probe rows")
+    # warm-up of both paths on the first joins
+    for r in ctx.join_batch(pairs[:64], cfgs[:64]):
+        r.release()
+    for (b, p), cfg in zip(pairs[:16], cfgs[:16]):
+        ctx.runPHJ(b, p, cfg).release()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    res = ctx.join_batch(pairs, cfgs)
+    ctx.synchronize()
+    t_batch = time.perf_counter() - t0
+    got = [r.checksum() for r in res]
+    out_rows = sum(r.out_rows for r in res)
+    for r in res:
+        r.release()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plain = [ctx.runPHJ(b, p, cfg) for (b, p), cfg in zip(pairs, cfgs)]
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t0
+    want = [r.checksum() for r in plain]
+    for r in plain:
+        r.release()
+    build.release()
+    probe.release()
+    parity = got == want
+    log(f"batch: batched {t_batch * 1e3:.1f} ms, plain {t_plain * 1e3:.1f} ms, per-join parity {parity}")
+    return {"joins": n_joins, "build_rows": R, "probe_rows": S, "out_rows": out_rows,
+            "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3, "speedup": t_plain / t_batch,
+            "batched_tuples_per_s": (R + S) / t_batch, "plain_tuples_per_s": (R + S) / t_plain,
+            "parity_per_join": parity,
+            "workload": f"{n_joins} joins, |R_i| = floor(2^U[10,20]), |S_i| = 8|R_i|, 75% match, u64 payloads"}
+
+
+def skew_bench(reps=2, scale_log2=0):
+    """2^22 distinct build keys, each repeated k ~ Zipf(1.2) on [1,64] times and
+    shuffled (u8+u16+u64 payload); 2^27 probe rows, 90% drawing a key by
+    Zipf(0.8) rank, 10% missing (u64 payload). PHJ-PURE timed; CHJ and PHJ
+    (best-effort, 32 MiB per workgroup x 512 workgroups) must give the same
+    checksum."""
+    ctx = G.Context(0)
+    spec = G.gen_spec(seed=42, build_rows=1 << (22 - scale_log2), probe_rows=1 << (27 - scale_log2),
+                      distribution=G.DIST_ZIPF, match_rate=0.9,
+                      build_types=(G.U8, G.U16, G.U64), probe_types=(G.U64,), zipf_max_dup=64, zipf_build_s=1.2,
+                      zipf_probe_s=0.8)
+    build, probe = ctx.generate(spec)
+    log(f"skew: generated {build.rows} x {probe.rows} rows")
+    cfg = G.RadixConfig([8, 8])
+    ctx.runPHJ(build, probe, cfg).release()
+    log("skew: warm-up PHJ-PURE done")
+    best, res = None, None
+    for _ in range(reps):
+        if res is not None:
+            res.release()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        res = ctx.runPHJ(build, probe, cfg)
+        ms = (time.perf_counter() - t0) * 1e3
+        best = ms if best is None else min(best, ms)
+    want = res.checksum()
+    out_rows = res.out_rows
+    res.release()
+    sums = {}
+    log(f"skew: PHJ-PURE best {best:.2f} ms")
+    r = ctx.runCHJ(build, probe)
+    sums["chj"] = r.checksum() == want
+    log(f"skew: CHJ e2e {r.e2e_wall_ms:.2f} ms")
+    r.release()
+    r = ctx.runPhjBep(build, probe, cfg, bep_budget_mib=32 * 512)
+    sums["phj_bep"] = r.checksum() == want
+    bep = {"evictions": r.bep_evictions, "refinements": r.bep_refinements, "peak_bytes": r.bep_peak_bytes,
+           "e2e_ms": r.e2e_wall_ms}
+    r.release()
+    R, S = build.rows, probe.rows
+    build.release()
+    probe.release()
+    ctx.close()
+    log(f"skew: {R} x {S} rows, {out_rows} out, PHJ-PURE {best:.2f} ms, schemes agree {sums}")
+    return {"build_rows": R, "probe_rows": S, "out_rows": out_rows, "phj_pure_ms": best,
+            "tuples_per_s": (R + S) / (best / 1e3), "schemes_agree": all(sums.values()), "agree": sums,
+            "bep": bep, "workload": "2^22 keys x Zipf(1.2) dups on [1,64] (u8+u16+u64) vs 2^27 probes, 90% "
+                                    "Zipf(0.8)-ranked hits (u64), pass_bits {8,8}"}
+
+
+def serving_bench(n_requests, rates, workers=4, budget_gb=200, scale_log2=0):
+    """Open-loop Poisson requests (|R| = 2^U[12,24], |S| = U[4,16] |R|, match
+    U[0.1,1], 1-3 payload columns of u32/u64/u128 per side; 30% probe one of 64
+    resident build sides) through the serving loop at each offered rate. Every
+    20th request's checksum is compared with the same request run alone."""
+    import serve
+
+    ctx = G.Context(0)
+    kinds, residents, n_new = serve.request_pool(ctx, seed=42, n_new=32, n_resident=64, n_resident_probes=16,
+                                                 hi_log2=24 - scale_log2)
+    log(f"serve: {len(kinds)} request kinds, {len(residents)} resident build sides ready")
+    alone = [serve.run_request(ctx, k, want_checksum=True) for k in kinds]
+    log("serve: every request kind run alone")
+    srv = serve.JoinServer(workers=workers, budget_bytes=budget_gb << 30)
+    points, parity = [], True
+    for rate in rates:
+        sched = serve.poisson_schedule(rate, n_requests, len(kinds), seed=rate, n_new=n_new, resident_share=0.3)
+        outs = srv.run(kinds, sched, verify_every=20)
+        parity = parity and all(o.checksum is None or (o.out_rows, o.checksum) == alone[o.kind] for o in outs)
+        sm = serve.summarize(outs, kinds)
+        sm["offered_per_s"] = rate
+        points.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in sm.items()})
+        log(f"serve {rate}/s: p50 {sm['p50_ms']:.2f} ms p99 {sm['p99_ms']:.2f} ms, "
+            f"{sm['completed_per_s']:.0f} req/s, {sm['tuples_per_s']:.3g} tuples/s")
+    srv.close()
+    for k in kinds:
+        for rel in (k.build, k.probe):
+            if rel is not None:
+                rel.release()
+    for r in residents:
+        r.release()
+    ctx.close()
+    sat = max(points, key=lambda q: q["completed_per_s"])
+    return {"workers": workers, "budget_gb": budget_gb, "requests_per_point": n_requests, "points": points,
+            "saturation_req_per_s": sat["completed_per_s"], "saturation_tuples_per_s": sat["tuples_per_s"],
+            "parity_alone_vs_mix": parity,
+            "workload": "Poisson arrivals; |R| = 2^U[12,24], |S| = U[4,16]|R|, match U[0.1,1], 1-3 payload columns "
+                        "of u32/u64/u128 per side, 30% of requests probe one of 64 resident build sides"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +207,11 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain device copy of this size (HBM counter calibration under a profiler)")
     ap.add_argument("--tune", default="", help="phj_tuning overrides, e.g. flags=12,join_threads=1024")
+    ap.add_argument("--batch-joins", type=int, default=4096, help="joins of the batched-join measurement (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests per offered rate of the serving "
+                                                                     "measurement (0: skip)")
+    ap.add_argument("--serve-rates", default="250,500,1000,2000,5000,10000,20000")
+    ap.add_argument("--no-skew", action="store_true", help="skip the skewed-join measurement")
     args = ap.parse_args()
     tuning = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(",") if kv)}
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -156,6 +302,15 @@ def main():
                     "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": traffic, "kernel": dom,
                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
 
+    build.release()
+    probe.release()
+    batched = batched_bench(ctx, args.batch_joins) if args.batch_joins > 0 else None
+    skew = skew_bench(scale_log2=args.scale_log2) if not args.no_skew else None
+    serving = None
+    if args.serve_requests > 0:
+        serving = serving_bench(args.serve_requests, [int(r) for r in args.serve_rates.split(",") if r],
+                                scale_log2=args.scale_log2)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -191,12 +346,15 @@ def main():
         "parity_checksum": parity,
         "phase_ms_last": dict(zip(("build_shuffle", "probe_shuffle", "build", "probe"), phases[-1])),
         "kernel_ms_per_step": {k: round(v[0] / n, 4) for k, v in sorted(ktimes.items())},
+        "batched": batched,
+        "skew": skew,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    build.release()
-    probe.release()
     ctx.close()
-    if not parity:
+    if (not parity or (batched is not None and not batched["parity_per_join"])
+            or (serving is not None and not serving["parity_alone_vs_mix"])
+            or (skew is not None and not skew["schemes_agree"])):
         sys.exit(1)
 
 

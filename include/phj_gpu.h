@@ -237,6 +237,30 @@ phj_status phj_run_phj_pure(phj_ctx *ctx, const phj_relation *build, const phj_r
                             uint32_t n_passes, uint32_t threads, phj_join_result *result);
 phj_status phj_run_phj(phj_ctx *ctx, const phj_relation *build, const phj_relation *probe, const uint8_t *pass_bits,
                        uint32_t n_passes, uint32_t threads, uint64_t bep_budget_mib, phj_join_result *result);
+/* Resident build sides: a build relation radix-partitioned once and kept in
+ * HBM, probed any number of times (PHJ-PURE's probe half, src/PHJ.cpp:19-150
+ * after the build shuffle). The reference re-shuffles the build stream on
+ * every call (README.md:146-151); this handle is what a serving loop keeps
+ * between requests. phj_probe_registered may run from any context on the same
+ * device, concurrently with other probes of the same handle; release through
+ * the registering context once no probe is in flight. */
+typedef struct phj_resident phj_resident;
+phj_status phj_build_register(phj_ctx *ctx, const phj_relation *build, const uint8_t *pass_bits, uint32_t n_passes,
+                              const phj_tuning *tuning, phj_resident **out);
+phj_status phj_probe_registered(phj_ctx *ctx, const phj_resident *build, const phj_relation *probe,
+                                const phj_tuning *tuning, phj_join_result *result);
+phj_status phj_build_release(phj_ctx *ctx, phj_resident *build);
+/* Device bytes held by a resident build side. */
+phj_status phj_build_bytes(const phj_resident *build, uint64_t *bytes);
+
+/* Batched PHJ-PURE joins: no counterpart in the reference, which joins one
+ * pair per call (runPHJ, src/PHJ.h:31); each results[i] equals what
+ * phj_join(descs[i]) returns (same rows; phase times are those of the group
+ * the join ran in). Joins whose relations lie back to back in device memory
+ * (same schema) are joined together in groups; the results of a group share
+ * one output allocation, freed with the last phj_result_release. Every desc
+ * must be PHJ_SCHEME_PHJ_PURE with a valid radix configuration. */
+phj_status phj_join_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *descs, phj_join_result *results);
 phj_status phj_result_release(phj_ctx *ctx, phj_join_result *result);
 /* Copy output columns to host arrays sized out_rows * width. */
 phj_status phj_result_copy_to_host(phj_ctx *ctx, const phj_join_result *result, uint64_t *keys, void *const *left,
@@ -269,6 +293,11 @@ typedef struct phj_gen_spec {
 
 /* Allocate and fill device relations; release with phj_relation_release. */
 phj_status phj_gen_relations(phj_ctx *ctx, const phj_gen_spec *spec, phj_relation *build, phj_relation *probe);
+/* n joins generated back to back into one build and one probe relation
+ * (same schema): join i's rows are [build_off[i], build_off[i+1]) and
+ * [probe_off[i], probe_off[i+1]) (offset arrays of n + 1 entries). */
+phj_status phj_gen_batch(phj_ctx *ctx, uint32_t n, const phj_gen_spec *specs, phj_relation *build, phj_relation *probe,
+                         uint64_t *build_off, uint64_t *probe_off);
 phj_status phj_relation_release(phj_ctx *ctx, phj_relation *rel);
 /* Checksum of the exact join output implied by the generator (PHJ_DIST_UNIFORM
  * and PHJ_DIST_REFERENCE), computed without joining. */

@@ -204,6 +204,8 @@ struct phj_ctx {
     EventTimer timer;
     bool timing = false;
     std::unordered_map<uint64_t, std::vector<void *>> owned; // handle -> buffers
+    // handle -> buffers shared by the results of one batch group (freed with the last)
+    std::unordered_map<uint64_t, std::shared_ptr<std::vector<void *>>> shared;
     uint64_t next_handle = 1;
 
     KernelTimer *tm() { return timing ? &timer : nullptr; }
@@ -387,8 +389,8 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
         rs.seg_tile_first = scr.get<uint32_t>(2 * (static_cast<uint64_t>(cur_nseg) + 1));
         rs.hist = scr.get<uint32_t>(max_tiles * B);
         rs.block_sums = scr.get<uint32_t>(radix_max_chunks(rows, cur_nseg) * B);
-        radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, shift, bits, nseg, rs, flags,
-                   ctx->tm());
+        radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, uniform_bits(shift, bits), nseg, rs,
+                   flags, ctx->tm());
         // previous pass output is dead once this pass has consumed it
         for (void *b : prev_bufs)
             ctx->arena.free(b);
@@ -789,6 +791,45 @@ template <class F> uint64_t join_with_capacity(phj_ctx *ctx, const ColSet &l, co
 }
 
 // ---- PHJ-PURE: src/PHJ.cpp:19-150 ----
+// Probe half of PHJ-PURE: partitions S like the build side (same passes)
+// and joins leaf by leaf against the partitioned build side RP. `e2e` was
+// started by the caller; `ebs` brackets the build shuffle (null: resident).
+void phj_pure_probe(phj_ctx *ctx, const Partitioned &RP, const ColSet &r_cols, const DevRel &S,
+                    const std::vector<uint32_t> &passes, const phj_tuning &tun, EventPair &e2e, EventPair *ebs,
+                    std::vector<void *> &keep, phj_join_result *res) {
+    const LeafJoinPlan plan = plan_of(tun);
+    EventPair eps, ej;
+    HIPCHK(hipEventRecord(eps.a, ctx->stream));
+    const Partitioned SP = radix_partition(ctx, S, nullptr, 1, 0, passes, keep, "probe", radix_flags(tun));
+    HIPCHK(hipEventRecord(eps.b, ctx->stream));
+    HIPCHK(hipEventRecord(ej.a, ctx->stream));
+    double big_build_ms = 0;
+    OutBuffers out;
+    uint64_t n = 0;
+    {
+        LeafJoin lj(ctx, RP, SP, plan);
+        out = alloc_output(ctx, r_cols, S.cols, lj.bound());
+        while ((n = lj.emit(out, 0)) == LeafJoin::kRetry) {
+            free_output(ctx, out);
+            out = alloc_output(ctx, r_cols, S.cols, lj.bound());
+        }
+        big_build_ms = lj.big_build_ms();
+    }
+    HIPCHK(hipEventRecord(ej.b, ctx->stream));
+    HIPCHK(hipEventRecord(e2e.b, ctx->stream));
+    ctx->sync();
+    for (void *k : keep)
+        ctx->arena.free(k);
+    set_phase(res->build_shuffle, ebs ? ebs->ms() : 0.0, RP.rows);
+    set_phase(res->probe_shuffle, eps.ms(), S.rows);
+    // Leaf build and probe run fused in one kernel; the fused time is
+    // reported as probe, and build carries only separately built HBM tables.
+    set_phase(res->build, big_build_ms, RP.rows);
+    set_phase(res->probe, ej.ms() - big_build_ms, S.rows);
+    res->e2e_wall_ms = e2e.ms();
+    ResultBuilder{ctx, res, RP.rows, S.rows}.finish(out, n);
+}
+
 void run_phj_pure(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, const uint8_t *pb, uint32_t np,
                   const phj_tuning &tun, phj_join_result *res) {
     validate_rel(&b);
@@ -800,43 +841,14 @@ void run_phj_pure(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, co
     const DevRel R = stage_relation(ctx, scr, b);
     const DevRel S = stage_relation(ctx, scr, p);
     ctx->arena.reset_peak();
-    const LeafJoinPlan plan = plan_of(tun);
 
-    EventPair e2e, ebs, eps, ej;
+    EventPair e2e, ebs;
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
     HIPCHK(hipEventRecord(ebs.a, ctx->stream));
     std::vector<void *> keep;
     const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
-    HIPCHK(hipEventRecord(eps.a, ctx->stream));
-    const Partitioned SP = radix_partition(ctx, S, nullptr, 1, 0, passes, keep, "probe", radix_flags(tun));
-    HIPCHK(hipEventRecord(eps.b, ctx->stream));
-    HIPCHK(hipEventRecord(ej.a, ctx->stream));
-    double big_build_ms = 0;
-    OutBuffers out;
-    uint64_t n = 0;
-    {
-        LeafJoin lj(ctx, RP, SP, plan);
-        out = alloc_output(ctx, R.cols, S.cols, lj.bound());
-        while ((n = lj.emit(out, 0)) == LeafJoin::kRetry) {
-            free_output(ctx, out);
-            out = alloc_output(ctx, R.cols, S.cols, lj.bound());
-        }
-        big_build_ms = lj.big_build_ms();
-    }
-    HIPCHK(hipEventRecord(ej.b, ctx->stream));
-    HIPCHK(hipEventRecord(e2e.b, ctx->stream));
-    ctx->sync();
-    for (void *k : keep)
-        ctx->arena.free(k);
-    set_phase(res->build_shuffle, ebs.ms(), R.rows);
-    set_phase(res->probe_shuffle, eps.ms(), S.rows);
-    // Leaf build and probe run fused in one kernel; the fused time is
-    // reported as probe, and build carries only separately built HBM tables.
-    set_phase(res->build, big_build_ms, R.rows);
-    set_phase(res->probe, ej.ms() - big_build_ms, S.rows);
-    res->e2e_wall_ms = e2e.ms();
-    ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, n);
+    phj_pure_probe(ctx, RP, R.cols, S, passes, tun, e2e, &ebs, keep, res);
 }
 
 // ---- CHJ: src/CHJ.cpp:16-119 with 256 lock-free sub-tables ----
@@ -897,6 +909,51 @@ void run_chj(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, phj_joi
     set_phase(res->probe, ep.ms(), S.rows);
     res->e2e_wall_ms = e2e.ms();
     ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, n);
+}
+
+} // namespace
+
+// A build side partitioned once and kept in HBM (device-resident BlockStore
+// equivalent): probes against it skip the build shuffle.
+struct phj_resident {
+    phj_ctx *owner = nullptr;
+    Partitioned RP;
+    ColSet r_cols{}; // column types (pointers unused)
+    std::vector<uint32_t> passes;
+    std::vector<void *> bufs;
+    uint64_t bytes = 0;
+};
+
+namespace {
+
+void register_build(phj_ctx *ctx, const phj_relation &b, const uint8_t *pb, uint32_t np, const phj_tuning &tun,
+                    phj_resident *r) {
+    validate_rel(&b);
+    const uint32_t total = validate_pass_bits(pb, np);
+    r->owner = ctx;
+    r->passes = device_passes(total);
+    Scratch scr(ctx);
+    const DevRel R = stage_relation(ctx, scr, b);
+    r->RP = radix_partition(ctx, R, nullptr, 1, 0, r->passes, r->bufs, "build", radix_flags(tun));
+    r->r_cols = R.cols;
+    uint64_t w = 8;
+    for (uint32_t c = 0; c < R.cols.n; ++c)
+        w += type_size(R.cols.types[c]);
+    r->bytes = R.rows * w + (r->RP.parts + 1) * 4;
+    ctx->sync();
+}
+
+void probe_resident(phj_ctx *ctx, const phj_resident &r, const phj_relation &p, const phj_tuning &tun,
+                    phj_join_result *res) {
+    validate_rel(&p);
+    std::memset(res, 0, sizeof(*res));
+    Scratch scr(ctx);
+    const DevRel S = stage_relation(ctx, scr, p);
+    ctx->arena.reset_peak();
+    EventPair e2e;
+    HIPCHK(hipEventRecord(e2e.a, ctx->stream));
+    std::vector<void *> keep;
+    phj_pure_probe(ctx, r.RP, r.r_cols, S, r.passes, tun, e2e, nullptr, keep, res);
 }
 
 __global__ void k_count_nonempty(const uint32_t *off, uint32_t n, uint32_t *out) {
@@ -998,6 +1055,334 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     res->bep_peak_bytes = peak_bytes;
     res->bep_build_skip_retries = 0;
     ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, n);
+}
+
+// ---- Batched PHJ-PURE joins ----
+// A batch is cut into groups of consecutive joins whose relations lie back to
+// back in memory (so a group is one segmented relation per side, with no
+// copy) and whose input bytes fit kBatchGroupBytes. Each join picks its own
+// leaf bits for leaves of about kBatchLeafRows build rows (its output does not
+// depend on them); a group runs as many device passes as its largest join
+// needs, every pass splitting each join's segments by that join's bits, then
+// one fused leaf join over all leaves of all joins writes each join's rows to
+// its own region of a shared output (bounded by its probe rows, as no build
+// key repeats). A join whose leaves repeat a key or outgrow LDS is rerun alone
+// on the plain path.
+constexpr uint64_t kBatchLeafRows = 2048;
+constexpr uint64_t kBatchGroupBytes = 8ull << 30;
+
+uint32_t batch_bits(uint64_t build_rows) {
+    uint32_t b = 0;
+    while (b < 16 && (build_rows >> b) > kBatchLeafRows)
+        ++b;
+    return b;
+}
+
+uint64_t rel_bytes(const phj_relation &r) {
+    uint64_t w = 8;
+    for (uint32_t c = 0; c < r.n_cols; ++c)
+        w += type_size(r.cols[c].type);
+    return r.rows * w;
+}
+
+bool same_schema(const phj_relation &a, const phj_relation &b) {
+    if (a.n_cols != b.n_cols || a.on_device != b.on_device)
+        return false;
+    for (uint32_t c = 0; c < a.n_cols; ++c)
+        if (a.cols[c].type != b.cols[c].type)
+            return false;
+    return true;
+}
+
+// b's rows directly follow a's in every column.
+bool follows(const phj_relation &a, const phj_relation &b) {
+    if (b.keys != a.keys + a.rows)
+        return false;
+    for (uint32_t c = 0; c < a.n_cols; ++c) {
+        const uint64_t w = type_size(a.cols[c].type);
+        if (static_cast<const uint8_t *>(b.cols[c].data) != static_cast<const uint8_t *>(a.cols[c].data) + a.rows * w)
+            return false;
+    }
+    return true;
+}
+
+// Radix partition of a segmented relation (segment j = join j) where join j
+// splits by bits[p][j] in pass p (0 keeps the segment whole). Returns the
+// leaves; leaf_join[l] is the join owning leaf l (a join's leaves are
+// consecutive).
+Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t *seg_off, uint32_t n_joins,
+                                  const std::vector<std::vector<uint32_t>> &bits, std::vector<void *> &keep,
+                                  uint32_t flags, std::vector<uint32_t> &leaf_join) {
+    Scratch scr(ctx);
+    const uint64_t rows = in.rows;
+    std::vector<uint32_t> join_of(n_joins);
+    for (uint32_t j = 0; j < n_joins; ++j)
+        join_of[j] = j;
+    std::vector<uint32_t> consumed(n_joins, 0);
+    const uint64_t *cur_keys = in.keys;
+    ColSet cur_cols = in.cols;
+    const uint32_t *cur_seg = seg_off;
+    uint32_t cur_nseg = n_joins;
+    std::vector<void *> prev_bufs;
+    Partitioned out;
+    out.rows = rows;
+    for (size_t p = 0; p < bits.size(); ++p) {
+        std::vector<uint16_t> sb(cur_nseg);
+        std::vector<uint32_t> first(static_cast<size_t>(cur_nseg) + 1);
+        std::vector<uint32_t> next_join;
+        uint32_t maxb = 0;
+        uint64_t run = 0;
+        for (uint32_t sg = 0; sg < cur_nseg; ++sg) {
+            const uint32_t j = join_of[sg];
+            const uint32_t b = bits[p][j];
+            sb[sg] = static_cast<uint16_t>((b ? 64 - consumed[j] - b : 0u) | (b << 8));
+            first[sg] = static_cast<uint32_t>(run);
+            run += 1ull << b;
+            maxb = std::max(maxb, b);
+            next_join.insert(next_join.end(), static_cast<size_t>(1) << b, j);
+        }
+        if (run > 0xFFFFFFF0ull)
+            throw PhjError(PHJ_ERR_UNSUPPORTED, "too many partitions in one batch group");
+        first[cur_nseg] = static_cast<uint32_t>(run);
+        for (uint32_t j = 0; j < n_joins; ++j)
+            consumed[j] += bits[p][j];
+        uint16_t *dsb = scr.get<uint16_t>(sb.size());
+        uint32_t *dfirst = scr.get<uint32_t>(first.size());
+        HIPCHK(hipMemcpyAsync(dsb, sb.data(), sb.size() * 2, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(dfirst, first.data(), first.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        const uint64_t B = 1ull << maxb;
+        std::vector<void *> bufs;
+        uint64_t *nkeys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
+        bufs.push_back(nkeys);
+        OutColSet ncols = alloc_cols(ctx, cur_cols, rows, bufs);
+        uint32_t *nseg = ctx->alloc<uint32_t>(run + 1);
+        bufs.push_back(nseg);
+        RadixPassScratch rs;
+        rs.seg_tile_first = scr.get<uint32_t>(2 * (static_cast<uint64_t>(cur_nseg) + 1));
+        rs.hist = scr.get<uint32_t>(radix_max_tiles(rows, cur_nseg) * B);
+        rs.block_sums = scr.get<uint32_t>(radix_max_chunks(rows, cur_nseg) * B);
+        radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, PassBits{dsb, dfirst, 0, maxb},
+                   nseg, rs, flags, ctx->tm());
+        for (void *b : prev_bufs)
+            ctx->arena.free(b);
+        prev_bufs = bufs;
+        cur_keys = nkeys;
+        cur_cols = as_in(ncols);
+        cur_seg = nseg;
+        cur_nseg = static_cast<uint32_t>(run);
+        join_of.swap(next_join);
+        out.keys = nkeys;
+        out.cols = ncols;
+        out.off = nseg;
+        out.parts = run;
+    }
+    for (void *b : prev_bufs)
+        keep.push_back(b);
+    leaf_join.swap(join_of);
+    return out;
+}
+
+// One group of a batch: m joins whose relations are back to back.
+void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_result *res) {
+    Scratch scr(ctx);
+    ctx->arena.reset_peak();
+    const phj_tuning tun = d[0].tuning;
+    DevRel R, S;
+    R.keys = d[0].build.keys;
+    S.keys = d[0].probe.keys;
+    R.cols.n = d[0].build.n_cols;
+    S.cols.n = d[0].probe.n_cols;
+    for (uint32_t c = 0; c < R.cols.n; ++c) {
+        R.cols.types[c] = d[0].build.cols[c].type;
+        R.cols.ptr[c] = d[0].build.cols[c].data;
+    }
+    for (uint32_t c = 0; c < S.cols.n; ++c) {
+        S.cols.types[c] = d[0].probe.cols[c].type;
+        S.cols.ptr[c] = d[0].probe.cols[c].data;
+    }
+    std::vector<uint32_t> r_off(m + 1, 0), s_off(m + 1, 0);
+    std::vector<uint64_t> base(m), end(m);
+    std::vector<uint32_t> jbits(m);
+    uint32_t n_passes = 0;
+    for (uint32_t j = 0; j < m; ++j) {
+        r_off[j + 1] = r_off[j] + static_cast<uint32_t>(d[j].build.rows);
+        s_off[j + 1] = s_off[j] + static_cast<uint32_t>(d[j].probe.rows);
+        base[j] = s_off[j];
+        end[j] = s_off[j + 1];
+        jbits[j] = batch_bits(d[j].build.rows);
+        n_passes = std::max(n_passes, (jbits[j] + 7) / 8);
+    }
+    R.rows = r_off[m];
+    S.rows = s_off[m];
+    std::vector<std::vector<uint32_t>> bits(n_passes, std::vector<uint32_t>(m, 0));
+    for (uint32_t j = 0; j < m; ++j) {
+        uint32_t left = jbits[j];
+        for (uint32_t p = 0; p < n_passes; ++p) {
+            bits[p][j] = std::min(left, 8u);
+            left -= bits[p][j];
+        }
+    }
+    uint32_t *dr_off = scr.get<uint32_t>(m + 1);
+    uint32_t *ds_off = scr.get<uint32_t>(m + 1);
+    HIPCHK(hipMemcpyAsync(dr_off, r_off.data(), (m + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ds_off, s_off.data(), (m + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
+
+    EventPair e2e, ebs, eps, ej;
+    HIPCHK(hipEventRecord(e2e.a, ctx->stream));
+    HIPCHK(hipEventRecord(ebs.a, ctx->stream));
+    std::vector<void *> keep;
+    std::vector<uint32_t> leaf_join;
+    LeafJoinArgs a{};
+    a.r_cols = R.cols;
+    a.s_cols = S.cols;
+    if (n_passes > 0) {
+        const Partitioned RP = radix_partition_joins(ctx, R, dr_off, m, bits, keep, radix_flags(tun), leaf_join);
+        HIPCHK(hipEventRecord(ebs.b, ctx->stream));
+        HIPCHK(hipEventRecord(eps.a, ctx->stream));
+        std::vector<uint32_t> lj2;
+        const Partitioned SP = radix_partition_joins(ctx, S, ds_off, m, bits, keep, radix_flags(tun), lj2);
+        HIPCHK(hipEventRecord(eps.b, ctx->stream));
+        a.r_keys = RP.keys;
+        a.r_cols = as_in(RP.cols);
+        a.r_off = RP.off;
+        a.s_keys = SP.keys;
+        a.s_cols = as_in(SP.cols);
+        a.s_off = SP.off;
+    } else {
+        HIPCHK(hipEventRecord(ebs.b, ctx->stream));
+        HIPCHK(hipEventRecord(eps.a, ctx->stream));
+        HIPCHK(hipEventRecord(eps.b, ctx->stream));
+        leaf_join.resize(m);
+        for (uint32_t j = 0; j < m; ++j)
+            leaf_join[j] = j;
+        a.r_keys = R.keys;
+        a.r_off = dr_off;
+        a.s_keys = S.keys;
+        a.s_off = ds_off;
+    }
+    const uint32_t n_leaves = static_cast<uint32_t>(leaf_join.size());
+    HIPCHK(hipEventRecord(ej.a, ctx->stream));
+    OutBuffers out = alloc_output(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1));
+    unsigned long long *cur = scr.get<unsigned long long>(m);
+    uint64_t *dend = scr.get<uint64_t>(m);
+    uint32_t *dflag = scr.get<uint32_t>(m);
+    uint32_t *dleaf = scr.get<uint32_t>(n_leaves);
+    HIPCHK(hipMemcpyAsync(cur, base.data(), m * 8ull, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dend, end.data(), m * 8ull, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dleaf, leaf_join.data(), n_leaves * 4ull, hipMemcpyHostToDevice, ctx->stream));
+    launch_fill_u32(ctx->stream, dflag, m, 0);
+    // Leaf sizes decide the LDS table size; joins with a larger leaf rerun alone.
+    std::vector<uint32_t> h_roff(n_leaves + 1);
+    HIPCHK(hipMemcpyAsync(h_roff.data(), a.r_off, (n_leaves + 1) * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    std::vector<uint32_t> join_max(m, 0);
+    uint32_t max_leaf = 0;
+    for (uint32_t l = 0; l < n_leaves; ++l) {
+        const uint32_t sz = h_roff[l + 1] - h_roff[l];
+        join_max[leaf_join[l]] = std::max(join_max[leaf_join[l]], sz);
+        max_leaf = std::max(max_leaf, sz);
+    }
+    const LeafJoinPlan plan = plan_of(tun);
+    uint32_t lds_cap = leaf_join_lds_max_rows(78u * 1024u, plan.load_pct);
+    if (max_leaf > lds_cap)
+        lds_cap = leaf_join_lds_max_rows(150u * 1024u, plan.load_pct);
+    const uint32_t lds_max = std::max<uint32_t>(std::min(max_leaf, lds_cap), 1);
+    a.n_leaves = n_leaves;
+    a.lds_max_rows = lds_max;
+    a.load_pct = plan.load_pct;
+    a.chunk_rows = leaf_join_fused_chunk();
+    a.leaf_join = dleaf;
+    a.join_cursor = cur;
+    a.join_end = dend;
+    a.join_flag = dflag;
+    a.out = sink_of(out, nullptr);
+    {
+        uint32_t *work_first = scr.get<uint32_t>(n_leaves + 1);
+        uint32_t *scan_tmp = scr.get<uint32_t>(radix_scan_blocks(n_leaves + 1) + 1);
+        const uint64_t max_work = leaf_max_work(S.rows, n_leaves, a.chunk_rows);
+        uint32_t *item_leaf = scr.get<uint32_t>(max_work);
+        a.work_first = work_first;
+        a.item_leaf = item_leaf;
+        leaf_work_plan(ctx->stream, a, work_first, scan_tmp, item_leaf, ctx->tm());
+        leaf_join_fused(ctx->stream, a, max_work, ctx->tm());
+    }
+    HIPCHK(hipEventRecord(ej.b, ctx->stream));
+    HIPCHK(hipEventRecord(e2e.b, ctx->stream));
+    std::vector<uint64_t> h_cur(m);
+    std::vector<uint32_t> h_flag(m);
+    HIPCHK(hipMemcpyAsync(h_cur.data(), cur, m * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h_flag.data(), dflag, m * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    for (void *k : keep)
+        ctx->arena.free(k);
+    const double t_bs = ebs.ms(), t_ps = eps.ms(), t_j = ej.ms(), t_e2e = e2e.ms();
+    const uint64_t peak = ctx->arena.peak();
+    auto bufs = std::make_shared<std::vector<void *>>(out.bufs);
+    out.bufs.clear();
+    for (uint32_t j = 0; j < m; ++j) {
+        phj_join_result *r = &res[j];
+        if (h_flag[j] != 0 || join_max[j] > lds_max) {
+            run_phj_pure(ctx, d[j].build, d[j].probe, d[j].pass_bits, d[j].n_passes, d[j].tuning, r);
+            continue;
+        }
+        std::memset(r, 0, sizeof(*r));
+        OutBuffers view;
+        view.keys = out.keys + base[j];
+        view.left = out.left;
+        view.right = out.right;
+        for (uint32_t c = 0; c < out.left.n; ++c)
+            view.left.ptr[c] = static_cast<uint8_t *>(out.left.ptr[c]) + base[j] * type_size(out.left.types[c]);
+        for (uint32_t c = 0; c < out.right.n; ++c)
+            view.right.ptr[c] = static_cast<uint8_t *>(out.right.ptr[c]) + base[j] * type_size(out.right.types[c]);
+        fill_result_cols(r, view, h_cur[j] - base[j]);
+        // Members of a group report the group's phase times.
+        set_phase(r->build_shuffle, t_bs, R.rows);
+        set_phase(r->probe_shuffle, t_ps, S.rows);
+        set_phase(r->build, 0.0, R.rows);
+        set_phase(r->probe, t_j, S.rows);
+        r->e2e_wall_ms = t_e2e;
+        r->peak_mem_bytes = peak;
+        r->handle = ctx->next_handle++;
+        ctx->shared[r->handle] = bufs;
+    }
+    if (bufs.use_count() == 1)
+        for (void *b : *bufs)
+            ctx->arena.free(b);
+}
+
+void run_phj_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *d, phj_join_result *res) {
+    if (n > 0 && (d == nullptr || res == nullptr))
+        invalid("null descriptors/results");
+    bool batchable = true;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (d[i].scheme != PHJ_SCHEME_PHJ_PURE)
+            invalid("batched joins run PHJ-PURE");
+        validate_rel(&d[i].build);
+        validate_rel(&d[i].probe);
+        validate_pass_bits(d[i].pass_bits, d[i].n_passes);
+        std::memset(&res[i], 0, sizeof(res[i]));
+        batchable = batchable && d[i].build.on_device && same_schema(d[0].build, d[i].build) &&
+                    same_schema(d[0].probe, d[i].probe) && !(d[i].tuning.flags & PHJ_TUNE_TWO_PHASE_JOIN);
+    }
+    uint32_t i = 0;
+    while (i < n) {
+        if (!batchable) {
+            run_phj_pure(ctx, d[i].build, d[i].probe, d[i].pass_bits, d[i].n_passes, d[i].tuning, &res[i]);
+            ++i;
+            continue;
+        }
+        uint64_t bytes = rel_bytes(d[i].build) + rel_bytes(d[i].probe);
+        uint32_t e = i + 1;
+        while (e < n && follows(d[e - 1].build, d[e].build) && follows(d[e - 1].probe, d[e].probe)) {
+            const uint64_t add = rel_bytes(d[e].build) + rel_bytes(d[e].probe);
+            if (bytes + add > kBatchGroupBytes)
+                break;
+            bytes += add;
+            ++e;
+        }
+        run_batch_group(ctx, d + i, e - i, res + i);
+        i = e;
+    }
 }
 
 __global__ void k_u64_to_u32(const uint64_t *in, uint32_t *out, uint64_t n) {
@@ -1425,6 +1810,60 @@ phj_status phj_join(phj_ctx *ctx, const phj_join_desc *d, phj_join_result *resul
     });
 }
 
+phj_status phj_build_register(phj_ctx *ctx, const phj_relation *build, const uint8_t *pass_bits, uint32_t n_passes,
+                              const phj_tuning *tuning, phj_resident **out) {
+    return guard(ctx, [&] {
+        if (build == nullptr || out == nullptr)
+            invalid("null argument");
+        *out = nullptr;
+        const phj_tuning t = tuning ? *tuning : phj_tuning{};
+        auto r = std::make_unique<phj_resident>();
+        try {
+            register_build(ctx, *build, pass_bits, n_passes, t, r.get());
+        } catch (...) {
+            for (void *b : r->bufs)
+                ctx->arena.free(b);
+            throw;
+        }
+        *out = r.release();
+    });
+}
+
+phj_status phj_probe_registered(phj_ctx *ctx, const phj_resident *build, const phj_relation *probe,
+                                const phj_tuning *tuning, phj_join_result *result) {
+    return guard(ctx, [&] {
+        if (build == nullptr || probe == nullptr || result == nullptr)
+            invalid("null argument");
+        if (build->owner->device != ctx->device)
+            invalid("resident build side lives on another device");
+        const phj_tuning t = tuning ? *tuning : phj_tuning{};
+        probe_resident(ctx, *build, *probe, t, result);
+    });
+}
+
+phj_status phj_build_release(phj_ctx *ctx, phj_resident *build) {
+    return guard(ctx, [&] {
+        if (build == nullptr)
+            return;
+        if (build->owner != ctx)
+            invalid("release a resident build side through the context that registered it");
+        for (void *b : build->bufs)
+            ctx->arena.free(b);
+        delete build;
+    });
+}
+
+phj_status phj_build_bytes(const phj_resident *build, uint64_t *bytes) {
+    if (build == nullptr || bytes == nullptr)
+        return PHJ_ERR_INVALID_ARGUMENT;
+    *bytes = build->bytes;
+    return PHJ_OK;
+}
+
+phj_status phj_join_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *descs, phj_join_result *results) {
+    return guard(ctx, [&] { run_phj_batch(ctx, n, descs, results); });
+}
+
 phj_status phj_run_chj(phj_ctx *ctx, const phj_relation *build, const phj_relation *probe, uint32_t,
                        phj_join_result *result) {
     return guard(ctx, [&] {
@@ -1463,6 +1902,14 @@ phj_status phj_result_release(phj_ctx *ctx, phj_join_result *result) {
             for (void *b : it->second)
                 ctx->arena.free(b);
             ctx->owned.erase(it);
+        }
+        auto sh = ctx->shared.find(result->handle);
+        if (sh != ctx->shared.end()) {
+            const std::shared_ptr<std::vector<void *>> bufs = sh->second;
+            ctx->shared.erase(sh);
+            if (bufs.use_count() == 1)
+                for (void *b : *bufs)
+                    ctx->arena.free(b);
         }
         std::memset(result, 0, sizeof(*result));
     });
@@ -1522,6 +1969,61 @@ phj_status phj_gen_relations(phj_ctx *ctx, const phj_gen_spec *spec, phj_relatio
         }
         gen_side(ctx, g, 0, g.build_total, spec->build_n_cols, bt, build);
         gen_side(ctx, g, 1, spec->probe_rows, spec->probe_n_cols, pt, probe);
+        ctx->sync();
+    });
+}
+
+phj_status phj_gen_batch(phj_ctx *ctx, uint32_t n, const phj_gen_spec *specs, phj_relation *build, phj_relation *probe,
+                         uint64_t *build_off, uint64_t *probe_off) {
+    return guard(ctx, [&] {
+        if (n == 0 || specs == nullptr || build == nullptr || probe == nullptr || build_off == nullptr ||
+            probe_off == nullptr)
+            invalid("null argument");
+        Scratch scr(ctx);
+        std::vector<GenParams> g;
+        g.reserve(n);
+        build_off[0] = probe_off[0] = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            g.push_back(make_gen_params(ctx, specs[i], scr));
+            if (specs[i].build_n_cols != specs[0].build_n_cols || specs[i].probe_n_cols != specs[0].probe_n_cols ||
+                std::memcmp(specs[i].build_types, specs[0].build_types, PHJ_MAX_COLS) != 0 ||
+                std::memcmp(specs[i].probe_types, specs[0].probe_types, PHJ_MAX_COLS) != 0)
+                invalid("the joins of a generated batch share one schema");
+            build_off[i + 1] = build_off[i] + g[i].build_total;
+            probe_off[i + 1] = probe_off[i] + specs[i].probe_rows;
+        }
+        if (build_off[n] > 0xFFFFFFFEull * 64 || probe_off[n] > 0xFFFFFFFEull * 64)
+            invalid("batch too large");
+        uint32_t bt[PHJ_MAX_COLS], pt[PHJ_MAX_COLS];
+        for (uint32_t c = 0; c < PHJ_MAX_COLS; ++c) {
+            bt[c] = specs[0].build_types[c];
+            pt[c] = specs[0].probe_types[c];
+        }
+        auto fill = [&](int side, const uint64_t *off, uint32_t ncols, const uint32_t *types, phj_relation *rel) {
+            std::memset(rel, 0, sizeof(*rel));
+            const uint64_t rows = off[n];
+            rel->rows = rows;
+            rel->on_device = 1;
+            rel->n_cols = ncols;
+            uint64_t *keys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
+            rel->keys = keys;
+            OutColSet oc{};
+            oc.n = ncols;
+            for (uint32_t c = 0; c < ncols; ++c) {
+                oc.types[c] = types[c];
+                oc.ptr[c] = ctx->arena.alloc(std::max<uint64_t>(rows * type_size(types[c]), 1));
+                rel->cols[c].type = types[c];
+                rel->cols[c].data = oc.ptr[c];
+            }
+            for (uint32_t i = 0; i < n; ++i) {
+                OutColSet part = oc;
+                for (uint32_t c = 0; c < ncols; ++c)
+                    part.ptr[c] = static_cast<uint8_t *>(oc.ptr[c]) + off[i] * type_size(types[c]);
+                gen_fill(ctx->stream, g[i], side, off[i + 1] - off[i], keys + off[i], part);
+            }
+        };
+        fill(0, build_off, specs[0].build_n_cols, bt, build);
+        fill(1, probe_off, specs[0].probe_n_cols, pt, probe);
         ctx->sync();
     });
 }

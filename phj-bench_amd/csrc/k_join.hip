@@ -34,8 +34,14 @@ __host__ __device__ __forceinline__ uint32_t table_cap(uint32_t rows, uint32_t l
     return static_cast<uint32_t>(cap);
 }
 
-__device__ __forceinline__ uint32_t home_slot(uint64_t h, uint32_t cap) {
-    return static_cast<uint32_t>((static_cast<uint64_t>(static_cast<uint32_t>(h)) * cap) >> 32);
+// Home cell of key k in a leaf's LDS table of `cap` cells. The leaf's keys
+// already share the top bits of intHash64, so the table mixes the key itself:
+// the 32-bit fold of k times a Fibonacci constant, scaled to [0, cap). The
+// cell layout of a leaf table is internal (only the rows a probe finds are
+// observable), so it need not follow JoinHashTable's home rule.
+__device__ __forceinline__ uint32_t home_slot(uint64_t k, uint32_t cap) {
+    const uint32_t x = (static_cast<uint32_t>(k) ^ static_cast<uint32_t>(k >> 32)) * 0x9E3779B1u;
+    return __umulhi(x, cap);
 }
 
 // Appends the columns of one output row at `pos`.
@@ -92,18 +98,31 @@ __device__ __forceinline__ bool locate_item(const LeafJoinArgs &a, uint32_t w, I
     it.item = w - a.work_first[it.leaf];
     it.r0 = a.r_off[it.leaf];
     it.nr = a.r_off[it.leaf + 1] - it.r0;
-    it.s0 = a.s_off[it.leaf] + it.item * a.chunk_rows;
-    it.s1 = min(a.s_off[it.leaf + 1], it.s0 + a.chunk_rows);
+    // the leaf's probe rows split evenly over its items (each <= chunk_rows)
+    const uint32_t sl0 = a.s_off[it.leaf];
+    const uint64_t ns = a.s_off[it.leaf + 1] - sl0;
+    const uint32_t items = a.work_first[it.leaf + 1] - a.work_first[it.leaf];
+    it.s0 = sl0 + static_cast<uint32_t>(ns * it.item / items);
+    it.s1 = sl0 + static_cast<uint32_t>(ns * (it.item + 1) / items);
     return true;
 }
 
 // Builds the leaf's open-addressing table of 16-byte cells in LDS (64-bit CAS
 // on the key, exchange on the chain head; src/HashTable.h:84-109) and the
 // leaf-local chain successors next[] (src/BlockStore.h:86-89). Ends with a
-// barrier. Returns true in every thread when some key was inserted twice.
+// barrier; `dup` (LDS) is then nonzero when some key was inserted twice.
 template <int THREADS>
 __device__ __forceinline__ void build_leaf_table(const LeafJoinArgs &a, const Item &it, uint32_t cap, LdsCell *cells,
                                                  uint16_t *next, uint32_t &side_head, uint32_t &dup) {
+    // Build keys are loaded KB per thread at a time, all loads in flight
+    // together; the first batch is issued before the table is cleared.
+    constexpr int KB = 8;
+    uint64_t kk[KB];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+        const uint32_t i = q * THREADS + threadIdx.x;
+        kk[q] = i < it.nr ? a.r_keys[it.r0 + i] : 0;
+    }
     for (uint32_t i = threadIdx.x; i < cap; i += THREADS) {
         LdsCell c;
         c.key = kEmptyKey;
@@ -118,26 +137,39 @@ __device__ __forceinline__ void build_leaf_table(const LeafJoinArgs &a, const It
     __syncthreads();
 
     bool seen = false;
-    for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
-        const uint64_t k = a.r_keys[it.r0 + i];
-        uint32_t prev;
-        if (k == kEmptyKey) {
-            prev = atomicExch(&side_head, i);
-        } else {
-            uint32_t pos = home_slot(int_hash64(k), cap);
-            while (true) {
-                const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long *>(&cells[pos].key),
-                                                         static_cast<unsigned long long>(kEmptyKey),
-                                                         static_cast<unsigned long long>(k));
-                if (old == kEmptyKey || old == k) {
-                    prev = atomicExch(&cells[pos].row_no, i);
-                    break;
-                }
-                pos = (pos + 1 == cap) ? 0 : pos + 1;
+    for (uint32_t base = 0; base < it.nr; base += KB * THREADS) {
+        if (base != 0) {
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const uint32_t i = base + q * THREADS + threadIdx.x;
+                kk[q] = i < it.nr ? a.r_keys[it.r0 + i] : 0;
             }
         }
-        seen |= prev != kInvalidRow;
-        next[i] = prev == kInvalidRow ? kNoNext : static_cast<uint16_t>(prev);
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const uint32_t i = base + q * THREADS + threadIdx.x;
+            if (i >= it.nr)
+                continue;
+            const uint64_t k = kk[q];
+            uint32_t prev;
+            if (k == kEmptyKey) {
+                prev = atomicExch(&side_head, i);
+            } else {
+                uint32_t pos = home_slot(k, cap);
+                while (true) {
+                    const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long *>(&cells[pos].key),
+                                                             static_cast<unsigned long long>(kEmptyKey),
+                                                             static_cast<unsigned long long>(k));
+                    if (old == kEmptyKey || old == k) {
+                        prev = atomicExch(&cells[pos].row_no, i);
+                        break;
+                    }
+                    pos = (pos + 1 == cap) ? 0 : pos + 1;
+                }
+            }
+            seen |= prev != kInvalidRow;
+            next[i] = prev == kInvalidRow ? kNoNext : static_cast<uint16_t>(prev);
+        }
     }
     if (seen)
         dup = 1u;
@@ -148,7 +180,7 @@ __device__ __forceinline__ void build_leaf_table(const LeafJoinArgs &a, const It
 __device__ __forceinline__ uint32_t lookup_leaf(const LdsCell *cells, uint32_t cap, uint32_t side_head, uint64_t k) {
     if (k == kEmptyKey)
         return side_head;
-    uint32_t pos = home_slot(int_hash64(k), cap);
+    uint32_t pos = home_slot(k, cap);
     while (true) {
         const LdsCell c = cells[pos];
         if (c.key == k)
@@ -424,10 +456,11 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
 // sentinel's side chain, or kInvalidRow.
 constexpr uint32_t kSideSlot = 0xFFFEu;
 constexpr int kFusedRowsPerThread = 32;
+constexpr uint32_t kFusedChunkRows = 512u * kFusedRowsPerThread;
 __device__ __forceinline__ uint32_t lookup_slot(const LdsCell *cells, uint32_t cap, uint32_t side_head, uint64_t k) {
     if (k == kEmptyKey)
         return side_head == kInvalidRow ? kInvalidRow : kSideSlot;
-    uint32_t pos = home_slot(int_hash64(k), cap);
+    uint32_t pos = home_slot(k, cap);
     while (true) {
         const uint64_t ck = cells[pos].key;
         if (ck == k)
@@ -438,25 +471,94 @@ __device__ __forceinline__ uint32_t lookup_slot(const LdsCell *cells, uint32_t c
     }
 }
 
+// Element at byte offset `off` of a uniform base: the 32-bit offset keeps the
+// address arithmetic in the scalar base + vector offset form.
+template <class T> __device__ __forceinline__ T ld_at(const void *base, uint32_t off) {
+    return *reinterpret_cast<const T *>(static_cast<const char *>(base) + off);
+}
+template <class T> __device__ __forceinline__ void st_at(void *base, uint32_t off, T v) {
+    *reinterpret_cast<T *>(static_cast<char *>(base) + off) = v;
+}
+
+// One column of one round: rows idx[u] of src (element indices relative to
+// src) to rows pos[u] of dst (relative to dst); loads first, then stores.
+template <int U, class T>
+__device__ __forceinline__ void move_col(const void *src, void *dst, const uint32_t (&idx)[U],
+                                         const uint32_t (&pos)[U], const bool (&on)[U]) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        v[u] = T{};
+        if (on[u])
+            v[u] = ld_at<T>(src, idx[u] * static_cast<uint32_t>(sizeof(T)));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (on[u])
+            st_at<T>(dst, pos[u] * static_cast<uint32_t>(sizeof(T)), v[u]);
+}
+
+template <int U>
+__device__ __forceinline__ void move_col128(const void *src, void *dst, const uint32_t (&idx)[U],
+                                            const uint32_t (&pos)[U], const bool (&on)[U]) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        uint64_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v[u] = 0;
+            if (on[u])
+                v[u] = ld_at<uint64_t>(src, idx[u] * 16u + 8u * half);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u])
+                st_at<uint64_t>(dst, pos[u] * 16u + 8u * half, v[u]);
+    }
+}
+
+// Every column of a set: source rows from `src_row0`, output rows from
+// `dst_row0` (both uniform), per-lane indices relative to them.
+template <int U>
+__device__ __forceinline__ void move_set(const ColSet &in, uint64_t src_row0, const OutColSet &out, uint64_t dst_row0,
+                                         const uint32_t (&idx)[U], const uint32_t (&pos)[U], const bool (&on)[U]) {
+    for (uint32_t c = 0; c < in.n; ++c) {
+        const uint32_t t = col_type(in, c);
+        const uint64_t sz = type_size(t);
+        const void *src = static_cast<const char *>(col_ptr(in, c)) + src_row0 * sz;
+        void *dst = static_cast<char *>(col_ptr(out, c)) + dst_row0 * sz;
+        switch (t) {
+        case 0: move_col<U, uint8_t>(src, dst, idx, pos, on); break;
+        case 1: move_col<U, uint16_t>(src, dst, idx, pos, on); break;
+        case 2: move_col<U, uint32_t>(src, dst, idx, pos, on); break;
+        case 3: move_col<U, uint64_t>(src, dst, idx, pos, on); break;
+        default: move_col128<U>(src, dst, idx, pos, on); break;
+        }
+    }
+}
+
 // Fused leaf join for build sides without repeated keys (every probe row has
-// at most one match). One workgroup per (leaf, probe chunk of up to
-// THREADS*K rows): the chunk's probe keys are loaded first, the leaf's LDS
-// table is built meanwhile, every row is looked up (the matched cell index
-// is kept in registers), the item's output rows are reserved with one atomic
-// add on the sink cursor, and rounds of THREADS*U rows are written slot-major
-// (the emit pass's single-match layout). The output key is the matched
-// cell's key. Rows at or beyond the sink capacity are counted, not written.
-// A leaf with a repeated key raises *chain_flag and writes nothing: the
-// caller then joins with the two-phase path.
+// at most one match). One workgroup per (leaf, probe chunk of at most
+// THREADS*K rows) builds the leaf's LDS table, looks every probe row up (the
+// matched cell index is kept in registers), reserves the chunk's output rows
+// with one atomic add on the sink cursor and lays them out slot-major (slot k
+// of every lane, wave by wave, so each column store of a slot is one
+// contiguous run), then writes them in rounds of U slots. The output key is
+// the matched cell's key. Rows at or beyond the sink capacity are counted,
+// not written. A leaf with a repeated key raises *chain_flag and writes
+// nothing: the caller then joins with the two-phase path.
 template <int THREADS, int K, int U>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_leaf_join(LeafJoinArgs a) {
-    static_assert(K % U == 0 && K % 2 == 0, "rounds of U slots");
+    static_assert(K % U == 0 && K % 2 == 0 && K <= 64, "K slots per lane in rounds of U");
+    constexpr int W = THREADS / 64;
+    constexpr int LB = 8; // keys looked up in lock step
+    static_assert(K % LB == 0, "lookup batches");
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ uint32_t tmp[THREADS / 64 + 1];
-    __shared__ uint32_t wave_cnt[2][U][THREADS / 64];
+    __shared__ uint32_t slot_off[K][W];
     __shared__ uint32_t side_head;
     __shared__ uint32_t dup;
-    __shared__ unsigned long long item_base;
+    __shared__ uint32_t sr_total;
+    __shared__ unsigned long long sr_base;
     Item it;
     const uint32_t w = xcd_share(blockIdx.x, a.work_first[a.n_leaves]);
     if (!locate_item(a, w, it))
@@ -464,97 +566,159 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     const uint32_t cap = table_cap(it.nr, a.load_pct);
     LdsCell *cells = reinterpret_cast<LdsCell *>(smem);
     uint16_t *next = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * sizeof(LdsCell));
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64;
 
-    // The first KP probe keys are loaded before the table is built.
-    constexpr int KP = K / 2;
-    uint64_t key[KP];
-#pragma unroll
-    for (int k = 0; k < KP; ++k) {
-        const uint32_t j = it.s0 + k * THREADS + threadIdx.x;
-        key[k] = j < it.s1 ? a.s_keys[j] : 0;
+    unsigned long long *cursor = a.out.cursor;
+    uint64_t out_cap = a.out.cap;
+    uint32_t *flag = a.chain_flag;
+    if (a.leaf_join) {
+        const uint32_t j = a.leaf_join[it.leaf];
+        cursor = a.join_cursor + j;
+        out_cap = a.join_end[j];
+        flag = a.join_flag + j;
     }
     build_leaf_table<THREADS>(a, it, cap, cells, next, side_head, dup);
     if (dup) {
         if (threadIdx.x == 0)
-            atomicOr(a.chain_flag, 1u);
+            atomicOr(flag, 1u);
         return;
     }
-    // slot code per row, two per word: 0 = no match, else cell index + 1
-    uint32_t code[K / 2];
-    uint32_t cnt = 0;
+    {
+        const uint32_t sb = it.s0; // one super-round: items hold at most THREADS*K probe rows
+        // slot code per row, two per word: 0 = no match, else cell index + 1
+        uint32_t code[K / 2];
+        // Lookups run LB keys at a time in lock step (one probe step of every
+        // pending key per iteration, all reads in flight together); the next
+        // LB keys load meanwhile.
+        uint64_t key[LB], nkey[LB];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const uint32_t j = it.s0 + k * THREADS + threadIdx.x;
-        if (k >= KP && k % KP == 0) {
-            // the next KP keys, loaded once the previous ones are consumed
-#pragma unroll
-            for (int q = 0; q < KP; ++q) {
-                const uint32_t jq = j + q * THREADS;
-                key[q] = jq < it.s1 ? a.s_keys[jq] : 0;
-            }
+        for (int q = 0; q < LB; ++q) {
+            const uint32_t j = sb + q * THREADS + threadIdx.x;
+            key[q] = j < it.s1 ? a.s_keys[j] : 0;
         }
-        const uint32_t sl = j < it.s1 ? lookup_slot(cells, cap, side_head, key[k % KP]) : kInvalidRow;
-        const uint32_t c = sl == kInvalidRow ? 0u : sl + 1;
-        cnt += c != 0;
-        if (k % 2 == 0)
-            code[k / 2] = c;
-        else
-            code[k / 2] |= c << 16;
-    }
-    uint32_t total;
-    block_excl_scan<THREADS>(cnt, tmp, &total);
-    if (total == 0)
-        return;
-    if (threadIdx.x == 0)
-        item_base = atomicAdd(a.out.cursor, static_cast<unsigned long long>(total));
-    __syncthreads();
-    uint64_t rb = item_base;
-    const uint64_t out_cap = a.out.cap;
 #pragma unroll
-    for (int r = 0; r < K / U; ++r) {
-        uint16_t h[U];
-        uint32_t none[U];
-        uint32_t c[U];
+        for (int g = 0; g < K / LB; ++g) {
+            if (g + 1 < K / LB) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = r * U + u;
-            c[u] = (code[k / 2] >> (16 * (k % 2))) & 0xFFFFu;
-            h[u] = c[u] ? 0 : kNoNext;
-            none[u] = 0;
-        }
-        RoundLayout<THREADS, U> L;
-        round_layout<THREADS, U>(h, none, wave_cnt[r & 1], tmp, L);
-        if (L.total == 0)
-            continue;
-        uint32_t ridx[U], sidx[U];
-        uint64_t opos[U], okey[U];
-        bool on[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            opos[u] = rb + L.single_pos[u];
-            on[u] = c[u] != 0 && opos[u] < out_cap;
-            sidx[u] = it.s0 + (r * U + u) * THREADS + threadIdx.x;
-            uint32_t row = 0;
-            okey[u] = 0;
-            if (on[u]) {
-                if (c[u] - 1 == kSideSlot) {
-                    okey[u] = kEmptyKey;
-                    row = side_head;
-                } else {
-                    const LdsCell cell = cells[c[u] - 1];
-                    okey[u] = cell.key;
-                    row = cell.row_no;
+                for (int q = 0; q < LB; ++q) {
+                    const uint32_t j = sb + ((g + 1) * LB + q) * THREADS + threadIdx.x;
+                    nkey[q] = j < it.s1 ? a.s_keys[j] : 0;
                 }
             }
-            ridx[u] = it.r0 + row;
+            uint32_t pos[LB], res[LB];
+            uint32_t pend = 0;
+#pragma unroll
+            for (int q = 0; q < LB; ++q) {
+                const uint32_t j = sb + (g * LB + q) * THREADS + threadIdx.x;
+                pos[q] = home_slot(key[q], cap);
+                res[q] = kInvalidRow;
+                if (j < it.s1) {
+                    if (key[q] != kEmptyKey)
+                        pend |= 1u << q;
+                    else if (side_head != kInvalidRow)
+                        res[q] = kSideSlot;
+                }
+            }
+            while (pend) {
+                uint64_t ck[LB];
+#pragma unroll
+                for (int q = 0; q < LB; ++q)
+                    ck[q] = cells[pos[q]].key;
+#pragma unroll
+                for (int q = 0; q < LB; ++q) {
+                    const bool p = (pend >> q) & 1u;
+                    const bool hit = ck[q] == key[q];
+                    const bool stop = hit || ck[q] == kEmptyKey;
+                    res[q] = (p && hit) ? pos[q] : res[q];
+                    pend &= ~((p && stop ? 1u : 0u) << q);
+                    pos[q] = (p && !stop) ? (pos[q] + 1 == cap ? 0u : pos[q] + 1) : pos[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < LB; ++q) {
+                const int k = g * LB + q;
+                const uint32_t c = res[q] == kInvalidRow ? 0u : res[q] + 1;
+                if (k % 2 == 0)
+                    code[k / 2] = c;
+                else
+                    code[k / 2] |= c << 16;
+            }
+            if (g + 1 < K / LB) {
+#pragma unroll
+                for (int q = 0; q < LB; ++q)
+                    key[q] = nkey[q];
+            }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (on[u])
-                a.out.keys[opos[u]] = okey[u];
-        gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
-        gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
-        rb += L.total;
+        for (int k = 0; k < K; ++k) {
+            const uint64_t m = __ballot(((code[k / 2] >> (16 * (k % 2))) & 0xFFFFu) != 0);
+            if (lane == 0)
+                slot_off[k][wave] = static_cast<uint32_t>(__popcll(m));
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // lane k: exclusive offsets of slot k's wave runs, slots in order
+            uint32_t pre[W];
+            uint32_t tot = 0;
+#pragma unroll
+            for (int ww = 0; ww < W; ++ww) {
+                pre[ww] = tot;
+                tot += lane < static_cast<uint32_t>(K) ? slot_off[lane < K ? lane : 0][ww] : 0u;
+            }
+            const uint32_t incl = wave_incl_scan(tot);
+            if (lane < static_cast<uint32_t>(K)) {
+#pragma unroll
+                for (int ww = 0; ww < W; ++ww)
+                    slot_off[lane][ww] = incl - tot + pre[ww];
+            }
+            if (lane == 63) {
+                sr_total = incl;
+                if (incl)
+                    sr_base = atomicAdd(cursor, static_cast<unsigned long long>(incl));
+            }
+        }
+        __syncthreads();
+        if (sr_total != 0) {
+            const uint64_t rb = sr_base;
+#pragma unroll
+            for (int r = 0; r < K / U; ++r) {
+                if (sb + r * U * THREADS >= it.s1)
+                    continue; // uniform: rounds past the chunk's end
+                uint32_t ridx[U], sidx[U], opos[U];
+                uint64_t okey[U];
+                bool on[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int k = r * U + u;
+                    const uint32_t c = (code[k / 2] >> (16 * (k % 2))) & 0xFFFFu;
+                    const uint64_t m = __ballot(c != 0);
+                    opos[u] = slot_off[k][wave] + mbcnt(m);
+                    on[u] = c != 0 && rb + opos[u] < out_cap;
+                    sidx[u] = k * THREADS + threadIdx.x;
+                    uint32_t row = 0;
+                    okey[u] = 0;
+                    if (on[u]) {
+                        if (c - 1 == kSideSlot) {
+                            okey[u] = kEmptyKey;
+                            row = side_head;
+                        } else {
+                            const LdsCell cell = cells[c - 1];
+                            okey[u] = cell.key;
+                            row = cell.row_no;
+                        }
+                    }
+                    ridx[u] = row;
+                }
+                uint64_t *okeys = a.out.keys + rb;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (on[u])
+                        st_at<uint64_t>(okeys, opos[u] * 8u, okey[u]);
+                move_set<U>(a.r_cols, it.r0, a.out.left, rb, ridx, opos, on);
+                move_set<U>(a.s_cols, sb, a.out.right, rb, sidx, opos, on);
+            }
+        }
     }
 }
 
@@ -837,16 +1001,16 @@ void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_
         launch_probe<256, 16>(s, a, max_work, lds);
 }
 
-uint32_t leaf_join_fused_chunk() { return 512 * kFusedRowsPerThread; }
+uint32_t leaf_join_fused_chunk() { return kFusedChunkRows; }
 
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm) {
     if (max_work == 0)
         return;
     const size_t lds = leaf_join_lds_bytes(a.lds_max_rows, a.load_pct);
     TimedScope ts(tm, "leaf_join", s);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 4>), hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2>), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(lds));
-    k_leaf_join<512, kFusedRowsPerThread, 4><<<grid8(max_work), 512, lds, s>>>(a);
+    k_leaf_join<512, kFusedRowsPerThread, 2><<<grid8(max_work), 512, lds, s>>>(a);
 }
 
 void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm) {

@@ -59,6 +59,28 @@ __device__ __forceinline__ bool locate_tile(uint32_t t, const uint32_t *seg_off,
 __device__ __forceinline__ uint32_t n_tiles_of(uint32_t rows) { return (rows + kTile - 1) / kTile; }
 __device__ __forceinline__ uint32_t n_chunks_of(uint32_t tiles) { return (tiles + kChunkTiles - 1) / kChunkTiles; }
 
+// Bit window of segment s: the pass's uniform (shift, bits), or per segment
+// from sb[s] = shift | bits << 8 (segments of batched joins split by their own
+// bit counts). Output segment d of s is out_first[s] + d (uniform: s * B).
+__device__ __forceinline__ void seg_window(const PassBits &p, uint32_t s, uint32_t &shift, uint32_t &bits) {
+    if (p.sb) {
+        const uint32_t v = p.sb[s];
+        shift = v & 0xFFu;
+        bits = v >> 8;
+    } else {
+        shift = p.shift;
+        bits = p.bits;
+    }
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t h, uint32_t shift, uint32_t bits) {
+    return bits ? static_cast<uint32_t>(h >> shift) & ((1u << bits) - 1u) : 0u;
+}
+
+__device__ __forceinline__ uint32_t out_first_of(const PassBits &p, uint32_t s) {
+    return p.out_first ? p.out_first[s] : s << p.bits;
+}
+
 // Tiles and chunks per segment, exclusively scanned (single workgroup).
 __global__ __launch_bounds__(1024) void k_seg_tiles(const uint32_t *__restrict__ seg_off, uint32_t n_seg,
                                                     uint32_t *__restrict__ seg_first) {
@@ -97,14 +119,15 @@ __global__ void k_seg_single(uint32_t *seg_off, uint32_t rows) {
 // Per-tile digit counts, hist[tile * B + d].
 __global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ seg_off,
                                                        const uint32_t *__restrict__ seg_first, uint32_t n_seg,
-                                                       uint32_t shift, uint32_t bits, uint32_t *__restrict__ hist) {
+                                                       PassBits pb, uint32_t *__restrict__ hist) {
     constexpr int W = kPartThreads / 64;
     __shared__ uint32_t sub[W][kMaxDigits];
     TileLoc L;
     if (!locate_tile(xcd_share(blockIdx.x, seg_first[n_seg]), seg_off, seg_first, n_seg, L))
         return;
-    const uint32_t B = 1u << bits;
-    const uint32_t mask = B - 1;
+    const uint32_t B = 1u << pb.bits;
+    uint32_t shift, bits;
+    seg_window(pb, L.seg, shift, bits);
     for (uint32_t i = threadIdx.x; i < W * kMaxDigits; i += kPartThreads)
         (&sub[0][0])[i] = 0;
     __syncthreads();
@@ -114,7 +137,7 @@ __global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restric
         const uint32_t idx = r * kPartThreads + threadIdx.x;
         if (idx < L.len) {
             const uint64_t h = int_hash64(keys[L.row0 + idx]);
-            atomicAdd(&sub[wave][static_cast<uint32_t>(h >> shift) & mask], 1u);
+            atomicAdd(&sub[wave][digit_of(h, shift, bits)], 1u);
         }
     }
     __syncthreads();
@@ -157,12 +180,15 @@ __global__ __launch_bounds__(kMaxDigits) void k_colscan_chunk(const uint32_t *__
 // and write the output segment offsets out_off[s * B + d].
 __global__ __launch_bounds__(kMaxDigits) void k_colscan_seg(const uint32_t *__restrict__ seg_off,
                                                             const uint32_t *__restrict__ seg_first, uint32_t n_seg,
-                                                            uint32_t bits, uint32_t *__restrict__ chunk_tot,
+                                                            PassBits pb, uint32_t *__restrict__ chunk_tot,
                                                             uint32_t *__restrict__ out_off) {
     __shared__ uint32_t tmp[kMaxDigits / 64 + 1];
     const uint32_t *chunk_first = seg_first + n_seg + 1;
     const uint32_t s = blockIdx.x;
-    const uint32_t B = 1u << bits;
+    const uint32_t B = 1u << pb.bits;
+    uint32_t shift_s, bits_s;
+    seg_window(pb, s, shift_s, bits_s);
+    const uint32_t Bs = 1u << bits_s; // digits of this segment (the rest count zero)
     const uint32_t d = threadIdx.x;
     const uint32_t c0 = chunk_first[s], c1 = chunk_first[s + 1];
     uint32_t run = 0;
@@ -178,10 +204,11 @@ __global__ __launch_bounds__(kMaxDigits) void k_colscan_seg(const uint32_t *__re
     if (d < B) {
         for (uint32_t c = c0; c < c1; ++c)
             chunk_tot[c * B + d] += base;
-        out_off[s * B + d] = base;
     }
+    if (d < Bs)
+        out_off[out_first_of(pb, s) + d] = base;
     if (s == n_seg - 1 && d == 0)
-        out_off[n_seg * B] = seg_off[n_seg];
+        out_off[out_first_of(pb, n_seg)] = seg_off[n_seg];
 }
 
 // ---- generic device-wide exclusive scan of uint32 (three kernels) ----
@@ -334,15 +361,16 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
                                                           uint64_t *__restrict__ keys_out, OutColSet cout,
                                                           const uint32_t *__restrict__ seg_off,
                                                           const uint32_t *__restrict__ seg_first, uint32_t n_seg,
-                                                          uint32_t shift, uint32_t bits,
+                                                          PassBits pb,
                                                           const uint32_t *__restrict__ hist,
                                                           const uint32_t *__restrict__ chunk_base) {
     __shared__ ScatterShared sh;
     TileLoc L;
     if (!locate_tile(xcd_share(blockIdx.x, seg_first[n_seg]), seg_off, seg_first, n_seg, L))
         return;
-    const uint32_t B = 1u << bits;
-    const uint32_t mask = B - 1;
+    const uint32_t B = 1u << pb.bits;
+    uint32_t shift, bits;
+    seg_window(pb, L.seg, shift, bits);
     for (uint32_t i = threadIdx.x; i < B; i += kPartThreads)
         sh.cursor[i] = 0;
     __syncthreads();
@@ -356,7 +384,7 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
         slot[r] = kNoSlot;
         if (idx < L.len) {
             key[r] = ld<NTL>(keys_in + L.row0 + idx);
-            const uint32_t d = static_cast<uint32_t>(int_hash64(key[r]) >> shift) & mask;
+            const uint32_t d = digit_of(int_hash64(key[r]), shift, bits);
             slot[r] = (atomicAdd(&sh.cursor[d], 1u) << 8) | d;
         }
     }
@@ -446,8 +474,9 @@ void launch_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out, uint64_
 }
 
 void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
-                const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, uint32_t shift, uint32_t bits,
+                const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, const PassBits &pb,
                 uint32_t *out_seg_off, const RadixPassScratch &scr, uint32_t flags, KernelTimer *tm) {
+    const uint32_t bits = pb.bits;
     const uint64_t max_tiles = radix_max_tiles(rows, n_seg);
     const uint64_t max_chunks = radix_max_chunks(rows, n_seg);
     {
@@ -456,22 +485,22 @@ void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint6
     }
     if (rows > 0) {
         TimedScope ts(tm, "radix_hist", s);
-        k_hist<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, seg_off, scr.seg_tile_first, n_seg, shift, bits, scr.hist);
+        k_hist<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, seg_off, scr.seg_tile_first, n_seg, pb, scr.hist);
     }
     {
         TimedScope ts(tm, "radix_scan", s);
         if (rows > 0)
             k_colscan_chunk<<<static_cast<uint32_t>(max_chunks), kMaxDigits, 0, s>>>(scr.seg_tile_first, n_seg, bits, scr.hist,
                                                                                       scr.block_sums);
-        k_colscan_seg<<<n_seg, kMaxDigits, 0, s>>>(seg_off, scr.seg_tile_first, n_seg, bits, scr.block_sums, out_seg_off);
+        k_colscan_seg<<<n_seg, kMaxDigits, 0, s>>>(seg_off, scr.seg_tile_first, n_seg, pb, scr.block_sums, out_seg_off);
     }
     if (rows > 0) {
         TimedScope ts(tm, "radix_scatter", s);
         const bool ntl = flags & kRadixNtLoads, nts = flags & kRadixNtStores;
         auto k = ntl ? (nts ? k_scatter<true, true> : k_scatter<true, false>)
                      : (nts ? k_scatter<false, true> : k_scatter<false, false>);
-        k<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg, shift,
-                                                     bits, scr.hist, scr.block_sums);
+        k<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg, pb,
+                                                     scr.hist, scr.block_sums);
     }
 }
 

@@ -53,8 +53,19 @@ uint64_t radix_scan_blocks(uint64_t elems);
 // Scatter variants (phj_tuning.flags bits share these values).
 constexpr uint32_t kRadixNtLoads = 0x4;  // input columns read with nontemporal loads
 constexpr uint32_t kRadixNtStores = 0x8; // output columns written with nontemporal stores
+// Bit window of a pass. Uniform: every segment uses (shift, bits) and input
+// segment s yields output segments s * 2^bits + d. Per segment (sb != null):
+// sb[s] = shift_s | bits_s << 8 with bits_s <= bits, and output segment d of
+// s is out_first[s] + d for d < 2^bits_s (out_first has n_seg + 1 entries).
+struct PassBits {
+    const uint16_t *sb;
+    const uint32_t *out_first;
+    uint32_t shift;
+    uint32_t bits;
+};
+inline PassBits uniform_bits(uint32_t shift, uint32_t bits) { return PassBits{nullptr, nullptr, shift, bits}; }
 void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
-                const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, uint32_t shift, uint32_t bits,
+                const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, const PassBits &pb,
                 uint32_t *out_seg_off, const RadixPassScratch &scr, uint32_t flags, KernelTimer *tm);
 
 // Exclusive scan of n uint32 (device count: n = *count_ptr * mul when count_ptr != nullptr).
@@ -94,6 +105,13 @@ struct LeafJoinArgs {
     uint32_t *chain_flag;       // probe pass sets it when any probe row has several matches
     OutSink out;                // emit pass: rows land at out_base + item_off[item] + ...
     uint64_t out_base;
+    // Batched joins (fused path): leaf -> join, and per join an output cursor,
+    // the end of its output region (capacity) and a repeated-key flag. When
+    // leaf_join is null the fused join uses out.cursor, out.cap, chain_flag.
+    const uint32_t *leaf_join;
+    unsigned long long *join_cursor;
+    const uint64_t *join_end;
+    uint32_t *join_flag;
 };
 // Work items per leaf for the LDS path (0 for empty or oversize leaves),
 // scanned, and the leaf of each item.
@@ -106,7 +124,7 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
 // Fused probe + emit for build sides without repeated keys: one output
 // reservation per work item on out.cursor (rows at or beyond out.cap are
 // counted only); a leaf with a repeated key sets *chain_flag and writes
-// nothing. Work items must hold at most leaf_join_fused_chunk() probe rows.
+// nothing. Work items are probe chunks of leaf_join_fused_chunk() rows.
 uint32_t leaf_join_fused_chunk();
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm);
 size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct);

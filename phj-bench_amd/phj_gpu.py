@@ -172,6 +172,13 @@ _SIGNATURES = {
     "phj_hashtable_build_find": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int,
                                   C.c_void_p, C.c_void_p], C.c_int),
     "phj_join": ([C.c_void_p, C.POINTER(_JoinDesc), C.POINTER(_JoinResult)], C.c_int),
+    "phj_build_register": ([C.c_void_p, C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.POINTER(_Tuning),
+                            C.POINTER(C.c_void_p)], C.c_int),
+    "phj_probe_registered": ([C.c_void_p, C.c_void_p, C.POINTER(_Relation), C.POINTER(_Tuning), C.POINTER(_JoinResult)],
+                             C.c_int),
+    "phj_build_release": ([C.c_void_p, C.c_void_p], C.c_int),
+    "phj_build_bytes": ([C.c_void_p, C.POINTER(C.c_uint64)], C.c_int),
+    "phj_join_batch": ([C.c_void_p, C.c_uint32, C.POINTER(_JoinDesc), C.POINTER(_JoinResult)], C.c_int),
     "phj_run_chj": ([C.c_void_p, C.POINTER(_Relation), C.POINTER(_Relation), C.c_uint32, C.POINTER(_JoinResult)], C.c_int),
     "phj_run_phj_pure": ([C.c_void_p, C.POINTER(_Relation), C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.c_uint32,
                           C.POINTER(_JoinResult)], C.c_int),
@@ -181,6 +188,8 @@ _SIGNATURES = {
     "phj_result_copy_to_host": ([C.c_void_p, C.POINTER(_JoinResult), C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "phj_result_checksum": ([C.c_void_p, C.POINTER(_JoinResult), C.POINTER(_Checksum)], C.c_int),
     "phj_gen_relations": ([C.c_void_p, C.POINTER(_GenSpec), C.POINTER(_Relation), C.POINTER(_Relation)], C.c_int),
+    "phj_gen_batch": ([C.c_void_p, C.c_uint32, C.POINTER(_GenSpec), C.POINTER(_Relation), C.POINTER(_Relation),
+                       C.c_void_p, C.c_void_p], C.c_int),
     "phj_relation_release": ([C.c_void_p, C.POINTER(_Relation)], C.c_int),
     "phj_gen_expected_checksum": ([C.c_void_p, C.POINTER(_GenSpec), C.POINTER(_Checksum)], C.c_int),
     "phj_set_kernel_timing": ([C.c_void_p, C.c_int], C.c_int),
@@ -255,6 +264,14 @@ class RadixConfig:
         base, extra = divmod(total, passes)
         return RadixConfig([base + (1 if p < extra else 0) for p in range(passes)])
 
+    @staticmethod
+    def for_build_rows(rows, leaf_rows=2048):
+        """Leaves of at most leaf_rows build rows (at least 1 bit), passes of at most 8 bits."""
+        total = 1
+        while total < 16 and (rows >> total) > leaf_rows:
+            total += 1
+        return RadixConfig.from_partitions(1 << total, (total + 7) // 8)
+
 
 @dataclass
 class PhaseTiming:
@@ -323,7 +340,21 @@ class DeviceRelation:
             cols.append(a)
         return HostRelation(keys, cols, self.types)
 
+    def slice(self, a, b) -> "DeviceRelation":
+        """Rows [a, b) as a view (no ownership; the parent must outlive it)."""
+        v = _Relation()
+        C.memmove(C.byref(v), C.byref(self._rel), C.sizeof(_Relation))
+        v.rows = b - a
+        v.keys = self._rel.keys + 8 * a
+        for i, t in enumerate(self.types):
+            v.cols[i].data = self._rel.cols[i].data + TYPE_SIZE[t] * a
+        rel = DeviceRelation(self.ctx, v)
+        rel._view = True
+        return rel
+
     def release(self):
+        if getattr(self, "_view", False):
+            return
         if self._rel.keys:
             _raise(self.ctx._p, load_library().phj_relation_release(self.ctx._p, C.byref(self._rel)))
 
@@ -397,6 +428,43 @@ class JoinResult:
             _raise(self.ctx._p, load_library().phj_result_release(self.ctx._p, C.byref(self._res)))
 
 
+def _tuning(d: dict | None) -> _Tuning:
+    t = _Tuning()
+    for k, v in (d or {}).items():
+        setattr(t, k, v)
+    return t
+
+
+class ResidentBuild:
+    """A build side partitioned once and kept in HBM (phj_build_register); every
+    probe() is runPHJ's probe half against it. Probes may come from any Context
+    on the device; release() goes through the registering one."""
+
+    def __init__(self, ctx, ptr, rows):
+        self.ctx = ctx
+        self._h = ptr
+        self.rows = rows
+
+    @property
+    def nbytes(self):
+        n = C.c_uint64()
+        _raise(self.ctx._p, load_library().phj_build_bytes(self._h, C.byref(n)))
+        return int(n.value)
+
+    def probe(self, probe, tuning: dict | None = None, ctx=None) -> JoinResult:
+        c = ctx or self.ctx
+        res = _JoinResult()
+        p = probe._c()
+        t = _tuning(tuning)
+        _raise(c._p, load_library().phj_probe_registered(c._p, self._h, C.byref(p), C.byref(t), C.byref(res)))
+        return JoinResult(c, res)
+
+    def release(self):
+        if self._h:
+            _raise(self.ctx._p, load_library().phj_build_release(self.ctx._p, self._h))
+            self._h = None
+
+
 class Partitioned:
     """Radix shuffle output: contiguous columns grouped by partition; offsets[p]..offsets[p+1]."""
 
@@ -452,6 +520,30 @@ def gen_spec(seed=42, build_rows=1 << 20, probe_rows=1 << 24, distribution=DIST_
     s.zipf_build_s_milli = int(round(zipf_build_s * 1000))
     s.zipf_probe_s_milli = int(round(zipf_probe_s * 1000))
     return s
+
+
+def _mix64(z):
+    z = np.uint64(z)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return int(z ^ (z >> np.uint64(31)))
+
+
+def batched_join_specs(seed=42, n_joins=4096, lo_log2=10, hi_log2=20, probe_factor=8, match_rate=0.75,
+                       build_types=(U64,), probe_types=(U64,)):
+    """A batch of independent uniform joins: join i has floor(2^U[lo, hi]) distinct
+    build keys (U drawn from a counter hash of (seed, i)), probe_factor times as
+    many probe rows, match_rate of them matching (misses come from the disjoint
+    range [2^63, 2^64)), seeds derived from (seed, i)."""
+    specs = []
+    for i in range(n_joins):
+        u = (_mix64((seed * 0x9E3779B97F4A7C15 + i + 1) & (2**64 - 1)) >> 11) / float(1 << 53)
+        rows = int(2.0 ** (lo_log2 + (hi_log2 - lo_log2) * u))
+        js = _mix64((seed ^ ((i + 1) * 0xD1B54A32D192ED03)) & (2**64 - 1))
+        specs.append(gen_spec(seed=js, build_rows=rows, probe_rows=probe_factor * rows, distribution=DIST_UNIFORM,
+                              match_rate=match_rate, build_types=build_types, probe_types=probe_types))
+    return specs
 
 
 def _pass_array(pass_bits):
@@ -546,6 +638,41 @@ class Context:
         _raise(self._p, load_library().phj_join(self._p, C.byref(d), C.byref(res)))
         return JoinResult(self, res)
 
+    def register_build(self, build, cfg: RadixConfig | None = None, tuning: dict | None = None) -> ResidentBuild:
+        cfg = cfg or RadixConfig.for_build_rows(build.rows)
+        pb, n = _pass_array(cfg.pass_bits)
+        r = build._c()
+        t = _tuning(tuning)
+        h = C.c_void_p()
+        _raise(self._p, load_library().phj_build_register(self._p, C.byref(r), pb, n, C.byref(t), C.byref(h)))
+        return ResidentBuild(self, h.value, build.rows)
+
+    def join_batch(self, pairs, cfgs=None, tuning: dict | None = None) -> list:
+        """Batched PHJ-PURE joins (phj_join_batch): one JoinResult per (build, probe)
+        pair, each equal to runPHJ on that pair."""
+        n = len(pairs)
+        descs = (_JoinDesc * max(n, 1))()
+        for i, (b, p) in enumerate(pairs):
+            d = descs[i]
+            d.scheme = SCHEME_PHJ_PURE
+            d.threads = 1
+            d.build = b._c()
+            d.probe = p._c()
+            cfg = cfgs[i] if cfgs is not None else RadixConfig.for_build_rows(b.rows)
+            d.n_passes = len(cfg.pass_bits)
+            for k, bits in enumerate(cfg.pass_bits):
+                d.pass_bits[k] = int(bits)
+            for k, v in (tuning or {}).items():
+                setattr(d.tuning, k, v)
+        res = (_JoinResult * max(n, 1))()
+        _raise(self._p, load_library().phj_join_batch(self._p, n, descs, res))
+        out = []
+        for i in range(n):
+            r = _JoinResult()
+            C.memmove(C.byref(r), C.byref(res[i]), C.sizeof(_JoinResult))
+            out.append(JoinResult(self, r))
+        return out
+
     def runCHJ(self, build, probe, threads=1, **kw) -> JoinResult:
         return self.join(SCHEME_CHJ, build, probe, None, threads, **kw)
 
@@ -560,6 +687,18 @@ class Context:
         b, p = _Relation(), _Relation()
         _raise(self._p, load_library().phj_gen_relations(self._p, C.byref(spec), C.byref(b), C.byref(p)))
         return DeviceRelation(self, b), DeviceRelation(self, p)
+
+    def generate_batch(self, specs):
+        """Joins generated back to back: (build, probe, build_off, probe_off) with
+        join i's rows at [off[i], off[i+1]) of each relation."""
+        n = len(specs)
+        arr = (_GenSpec * n)(*specs)
+        b, p = _Relation(), _Relation()
+        boff = np.zeros(n + 1, dtype=np.uint64)
+        poff = np.zeros(n + 1, dtype=np.uint64)
+        _raise(self._p, load_library().phj_gen_batch(self._p, n, arr, C.byref(b), C.byref(p), boff.ctypes.data,
+                                                     poff.ctypes.data))
+        return DeviceRelation(self, b), DeviceRelation(self, p), boff, poff
 
     def expected_checksum(self, spec: _GenSpec):
         cs = _Checksum()
