@@ -76,6 +76,18 @@ __global__ void k_leaf_work_count(const uint32_t *__restrict__ r_off, const uint
     }
 }
 
+// Per leaf: chain-export slots of the two-phase join, one region of nr rows per
+// work item of the leaf (each item builds its own table, so chain orders of
+// two items of one leaf need not agree).
+__global__ void k_leaf_chain_slots(const uint32_t *__restrict__ r_off, const uint32_t *__restrict__ work_first,
+                                   uint32_t n_leaves, uint32_t *__restrict__ chain_first) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n_leaves)
+        chain_first[g] = (work_first[g + 1] - work_first[g]) * (r_off[g + 1] - r_off[g]);
+    else if (g == n_leaves)
+        chain_first[g] = 0;
+}
+
 constexpr uint16_t kNoNext = 0xFFFFu;   // end of a chain / probe row without a match
 constexpr uint16_t kMore = 0x8000u;     // the referenced row has a successor in its chain
 constexpr uint16_t kRowMask = 0x7FFFu;  // leaf-local build row (leaves hold < 32768 rows)
@@ -105,6 +117,11 @@ __device__ __forceinline__ bool locate_item(const LeafJoinArgs &a, uint32_t w, I
     it.s0 = sl0 + static_cast<uint32_t>(ns * it.item / items);
     it.s1 = sl0 + static_cast<uint32_t>(ns * (it.item + 1) / items);
     return true;
+}
+
+// First chain-export slot of a work item (two-phase join).
+__device__ __forceinline__ uint64_t item_chain_base(const LeafJoinArgs &a, const Item &it) {
+    return static_cast<uint64_t>(a.chain_first[it.leaf]) + static_cast<uint64_t>(it.item) * it.nr;
 }
 
 // Builds the leaf's open-addressing table of 16-byte cells in LDS (64-bit CAS
@@ -212,13 +229,12 @@ __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
     if (threadIdx.x == 0)
         blk_chain = 0;
     build_leaf_table<THREADS>(a, it, cap, cells, next, side_head, dup);
-    // Export the chains (first work item of the leaf only): successor plus
-    // whether the successor continues.
-    if (it.item == 0) {
-        for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
-            const uint16_t nx = next[i];
-            a.next16[it.r0 + i] = nx == kNoNext ? kNoNext : static_cast<uint16_t>(nx | (next[nx] != kNoNext ? kMore : 0));
-        }
+    // Export this item's chains (successor plus whether the successor
+    // continues) to the item's own region: the heads below refer to them.
+    uint16_t *chains = a.next16 + item_chain_base(a, it);
+    for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
+        const uint16_t nx = next[i];
+        chains[i] = nx == kNoNext ? kNoNext : static_cast<uint16_t>(nx | (next[nx] != kNoNext ? kMore : 0));
     }
 
     uint32_t cnt = 0;
@@ -380,6 +396,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     if (!locate_item(a, w, it))
         return;
     uint64_t round_base = a.out_base + a.item_off[w];
+    const uint16_t *chains = a.next16 + item_chain_base(a, it);
     uint32_t parity = 0;
     for (uint32_t base = it.s0; base < it.s1; base += THREADS * U, parity ^= 1) {
         uint16_t h[U];
@@ -394,7 +411,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             chainlen[u] = 0;
             if (h[u] != kNoNext && (h[u] & kMore)) {
                 uint32_t c = 1;
-                for (uint16_t x = a.next16[it.r0 + (h[u] & kRowMask)];; x = a.next16[it.r0 + (x & kRowMask)]) {
+                for (uint16_t x = chains[h[u] & kRowMask];; x = chains[x & kRowMask]) {
                     ++c;
                     if (!(x & kMore))
                         break;
@@ -444,7 +461,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                     emit_row(a.out, pos++, k, a.r_cols, it.r0 + (x & kRowMask), a.s_cols, j);
                     if (!(x & kMore))
                         break;
-                    x = a.next16[it.r0 + (x & kRowMask)];
+                    x = chains[x & kRowMask];
                 }
             }
         }
@@ -1002,6 +1019,11 @@ void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_
 }
 
 uint32_t leaf_join_fused_chunk() { return kFusedChunkRows; }
+
+void leaf_chain_slots(hipStream_t s, const LeafJoinArgs &a, uint32_t *chain_first, uint32_t *scan_tmp) {
+    k_leaf_chain_slots<<<(a.n_leaves + 1 + 255) / 256, 256, 0, s>>>(a.r_off, a.work_first, a.n_leaves, chain_first);
+    scan_u32(s, chain_first, a.n_leaves + 1, nullptr, 1, scan_tmp, nullptr, "join_plan");
+}
 
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm) {
     if (max_work == 0)

@@ -260,6 +260,22 @@ def test_fused_and_two_phase_leaf_join(ctx, pass_bits, flags):
     res.release()
 
 
+@pytest.mark.parametrize("chunk", [700, 5000])
+def test_chains_across_work_items(ctx, chunk):
+    """Repeated build keys in leaves of thousands of rows (tables built by many
+    wavefronts, so chain order varies between builds) whose probe side spans
+    many work items: each item must expand chains from its own table."""
+    spec = O.make_spec(seed=12, build_rows=6000, probe_rows=120000, distribution=2, match_rate=0.9,
+                       build_types=(G.U16, G.U64), probe_types=(G.U32,))
+    ob, op = O.gen(spec)
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    for bits in ([4], [2, 3]):  # leaves of about 3,800 and 1,900 build rows, in LDS
+        res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(bits), tuning=dict(probe_chunk_rows=chunk))
+        assert res.out_rows == want.shape[0]
+        assert np.array_equal(sorted_rows_of(res.to_host()), want)
+        res.release()
+
+
 def test_join_rejects_bad_radix(ctx):
     ob, op = data()["uniform_half"]
     with pytest.raises(ValueError):
