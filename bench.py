@@ -170,6 +170,9 @@ def serving_bench(n_requests, rates, workers=4, budget_gb=200, scale_log2=0):
     alone = [serve.run_request(ctx, k, want_checksum=True) for k in kinds]
     log("serve: every request kind run alone")
     srv = serve.JoinServer(workers=workers, budget_bytes=budget_gb << 30)
+    # warm-up: every worker's memory pool grows to its working size
+    srv.run(kinds, serve.poisson_schedule(200, 4 * len(kinds), len(kinds), seed=1))
+    log("serve: warm-up done")
     points, parity = [], True
     for rate in rates:
         sched = serve.poisson_schedule(rate, n_requests, len(kinds), seed=rate, n_new=n_new, resident_share=0.3)

@@ -44,37 +44,44 @@ struct PhjError : std::runtime_error {
 
 void invalid(const std::string &m) { throw PhjError(PHJ_ERR_INVALID_ARGUMENT, m); }
 
-// ---------------------------------------------------------------- device arena
-// Caching allocator: freed blocks are kept per rounded size and reused by
-// later allocations on the same stream (stream order makes reuse safe).
+// ---------------------------------------------------------------- device memory
+// Best-fit pool of one context. Memory comes from hipMalloc in segments (at
+// least kSegmentBytes); freed blocks coalesce with free neighbours of their
+// segment and are reused by later allocations of any size. Every allocation
+// and every kernel of a context runs on the context's one stream, so a block
+// freed there may be reused at once: stream order puts the new user's work
+// after the old user's. Blocks never move to another stream. Segments go back
+// to the device only when an allocation fails, or when more than 60% of the
+// device's memory sits free in this context.
 class Arena {
 public:
     ~Arena() { release_cached(); }
 
+    void bind(hipStream_t s) {
+        stream_ = s;
+        size_t free_mem = 0, total = 0;
+        if (hipMemGetInfo(&free_mem, &total) == hipSuccess)
+            cache_limit_ = total / 10 * 6;
+        else
+            (void)hipGetLastError();
+    }
+
     void *alloc(uint64_t bytes) {
-        const uint64_t sz = round(bytes);
-        auto it = free_.find(sz);
-        void *p = nullptr;
-        if (it != free_.end()) {
-            p = it->second;
-            free_.erase(it);
-            cached_ -= sz;
-        } else {
-            hipError_t e = hipMalloc(&p, sz);
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
-                release_cached();
-                e = hipMalloc(&p, sz);
-                if (e != hipSuccess) {
-                    (void)hipGetLastError();
-                    throw PhjError(PHJ_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(sz) + " bytes failed");
-                }
-            }
+        const uint64_t sz = (std::max<uint64_t>(bytes, 1) + kAlign - 1) & ~(kAlign - 1);
+        Block *b = take(sz);
+        if (b == nullptr)
+            b = grow(sz);
+        if (b == nullptr) {
+            release_cached();
+            b = grow(sz);
         }
-        live_[p] = sz;
-        current_ += sz;
+        if (b == nullptr)
+            throw PhjError(PHJ_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(sz) + " bytes failed");
+        live_[b->ptr] = b;
+        free_bytes_ -= b->size;
+        current_ += b->size;
         peak_ = std::max(peak_, current_);
-        return p;
+        return b->ptr;
     }
 
     void free(void *p) {
@@ -83,22 +90,37 @@ public:
         auto it = live_.find(p);
         if (it == live_.end())
             throw PhjError(PHJ_ERR_INVALID_ARGUMENT, "pointer not allocated by this context");
-        current_ -= it->second;
-        cached_ += it->second;
-        free_.emplace(it->second, p);
+        Block *b = it->second;
         live_.erase(it);
+        current_ -= b->size;
+        free_bytes_ += b->size;
+        b->free = true;
+        b = coalesce(b);
+        free_.emplace(b->size, b);
+        if (free_bytes_ > cache_limit_)
+            release_cached();
     }
 
     bool owns(const void *p) const { return live_.count(const_cast<void *>(p)) != 0; }
 
+    // Returns segments with no live block to the device (after the stream's
+    // queued work, which may still use them, has finished).
     void release_cached() {
-        if (free_.empty())
-            return;
-        (void)hipDeviceSynchronize();
-        for (auto &kv : free_)
-            (void)hipFree(kv.second);
-        free_.clear();
-        cached_ = 0;
+        if (stream_ != nullptr)
+            (void)hipStreamSynchronize(stream_);
+        for (auto it = free_.begin(); it != free_.end();) {
+            Block *b = it->second;
+            auto seg = segments_.find(b->ptr);
+            if (seg != segments_.end() && b->prev == nullptr && b->next == nullptr && b->size == seg->second) {
+                (void)hipFree(b->ptr);
+                free_bytes_ -= b->size;
+                segments_.erase(seg);
+                delete b;
+                it = free_.erase(it);
+            } else {
+                ++it;
+            }
+        }
     }
 
     uint64_t current() const { return current_; }
@@ -106,19 +128,87 @@ public:
     void reset_peak() { peak_ = current_; }
 
 private:
-    static uint64_t round(uint64_t b) {
-        if (b == 0)
-            b = 1;
-        if (b >= (2ull << 20))
-            return (b + (2ull << 20) - 1) & ~((2ull << 20) - 1);
-        uint64_t p = 256;
-        while (p < b)
-            p <<= 1;
-        return p;
+    static constexpr uint64_t kAlign = 256;
+    static constexpr uint64_t kSegmentBytes = 1ull << 30;
+    static constexpr uint64_t kMinSplit = 1ull << 20;
+
+    struct Block {
+        char *ptr;
+        uint64_t size;
+        bool free;
+        Block *prev, *next; // neighbours in the segment, by address
+    };
+
+    void erase_free(Block *b) {
+        auto range = free_.equal_range(b->size);
+        for (auto it = range.first; it != range.second; ++it)
+            if (it->second == b) {
+                free_.erase(it);
+                return;
+            }
     }
-    std::multimap<uint64_t, void *> free_;
-    std::unordered_map<void *, uint64_t> live_;
-    uint64_t current_ = 0, peak_ = 0, cached_ = 0;
+
+    Block *coalesce(Block *b) {
+        if (b->next && b->next->free) {
+            Block *n = b->next;
+            erase_free(n);
+            b->size += n->size;
+            b->next = n->next;
+            if (n->next)
+                n->next->prev = b;
+            delete n;
+        }
+        if (b->prev && b->prev->free) {
+            Block *p = b->prev;
+            erase_free(p);
+            p->size += b->size;
+            p->next = b->next;
+            if (b->next)
+                b->next->prev = p;
+            delete b;
+            b = p;
+        }
+        return b;
+    }
+
+    // Smallest free block of at least sz, split when the rest is worth keeping.
+    Block *take(uint64_t sz) {
+        auto it = free_.lower_bound(sz);
+        if (it == free_.end())
+            return nullptr;
+        Block *b = it->second;
+        free_.erase(it);
+        if (b->size - sz >= kMinSplit) {
+            Block *r = new Block{b->ptr + sz, b->size - sz, true, b, b->next};
+            if (b->next)
+                b->next->prev = r;
+            b->next = r;
+            b->size = sz;
+            free_.emplace(r->size, r);
+        }
+        b->free = false;
+        return b;
+    }
+
+    Block *grow(uint64_t sz) {
+        const uint64_t seg = std::max(sz, kSegmentBytes);
+        char *p = nullptr;
+        if (hipMalloc(reinterpret_cast<void **>(&p), seg) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        segments_[p] = seg;
+        free_bytes_ += seg;
+        free_.emplace(seg, new Block{p, seg, true, nullptr, nullptr});
+        return take(sz);
+    }
+
+    hipStream_t stream_ = nullptr;
+    std::multimap<uint64_t, Block *> free_;
+    std::unordered_map<void *, Block *> live_;
+    std::unordered_map<char *, uint64_t> segments_;
+    uint64_t current_ = 0, peak_ = 0, free_bytes_ = 0;
+    uint64_t cache_limit_ = 64ull << 30;
 };
 
 // ---------------------------------------------------------------- kernel timer
@@ -220,7 +310,7 @@ struct phj_ctx {
 
 namespace {
 
-// Buffers freed when the scope ends (returned to the arena cache).
+// Buffers freed when the scope ends (back to the context's pool).
 struct Scratch {
     phj_ctx *ctx;
     std::vector<void *> bufs;
@@ -1581,6 +1671,7 @@ phj_status phj_ctx_create(int device, phj_ctx **out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
         return PHJ_ERR_HIP;
+    ctx->arena.bind(ctx->stream);
     *out = ctx.release();
     return PHJ_OK;
 }

@@ -5,18 +5,17 @@ is either new (partitioned by the request) or one of the device-resident build
 sides registered up front (phj_build_register), whose probe half alone runs
 (phj_probe_registered). Admission is by HBM footprint: a request holds a
 reservation (its output bound and partition buffers) from admission until it
-completes, and arrivals wait in FIFO order while the budget is full. Admitted
-requests are taken by W workers, one HIP stream (Context) each, so up to W
-joins are in flight on the GPU at once; a worker takes the next admitted
-request as soon as its current one finishes.
+completes; arrivals wait in FIFO order while the budget is full. W workers,
+one HIP stream (Context) each, admit and run requests, so up to W joins are in
+flight on the GPU at once; a worker admits the next request as soon as its
+current one finishes.
 
 Latency is completion minus arrival. The reference has no serving mode; its
 unit of work, runPHJ (src/PHJ.h:31), is what every request runs.
 """
-import queue
 import threading
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -119,7 +118,7 @@ def run_request(ctx, kind: RequestInput, want_checksum=False):
 
 
 class JoinServer:
-    """W workers (one Context each) behind an admission queue bounded by HBM bytes."""
+    """W workers (one Context each) behind a FIFO admission queue bounded by HBM bytes."""
 
     def __init__(self, workers=4, budget_bytes=200 << 30, device=0):
         self.ctxs = [G.Context(device) for _ in range(workers)]
@@ -130,25 +129,42 @@ class JoinServer:
             c.close()
 
     def run(self, kinds, schedule, verify_every=0):
-        """schedule: list of (arrival_seconds, kind_index). Returns Outcomes."""
-        ready = queue.Queue()
-        lock = threading.Condition()
-        state = {"reserved": 0}
+        """schedule: list of (arrival_seconds, kind_index). Returns Outcomes.
+
+        Arrivals join a FIFO queue. A worker admits the head of the queue when
+        its footprint fits the budget beside the requests in flight (or nothing
+        is in flight), runs it, and releases the footprint when it completes."""
+        cond = threading.Condition()
+        pending = []  # arrived, not yet admitted (FIFO)
+        state = {"inflight": 0, "arrived": 0, "done": False}
         outcomes = [Outcome(i, k, t) for i, (t, k) in enumerate(schedule)]
+        errors = []
         t0 = time.perf_counter()
 
         def worker(ctx):
             while True:
-                i = ready.get()
-                if i is None:
-                    return
+                with cond:
+                    while True:
+                        if pending:
+                            need = kinds[outcomes[pending[0]].kind].footprint
+                            if state["inflight"] == 0 or state["inflight"] + need <= self.budget:
+                                i = pending.pop(0)
+                                state["inflight"] += need
+                                break
+                        elif state["done"]:
+                            return
+                        cond.wait()
                 o = outcomes[i]
                 kind = kinds[o.kind]
-                o.out_rows, o.checksum = run_request(ctx, kind, verify_every and i % verify_every == 0)
+                o.admitted = time.perf_counter() - t0
+                try:
+                    o.out_rows, o.checksum = run_request(ctx, kind, bool(verify_every) and i % verify_every == 0)
+                except Exception as e:  # reported by run(); the footprint is still released
+                    errors.append((i, repr(e)))
                 o.done = time.perf_counter() - t0
-                with lock:
-                    state["reserved"] -= kind.footprint
-                    lock.notify_all()
+                with cond:
+                    state["inflight"] -= kind.footprint
+                    cond.notify_all()
 
         threads = [threading.Thread(target=worker, args=(c,), daemon=True) for c in self.ctxs]
         for th in threads:
@@ -157,17 +173,16 @@ class JoinServer:
             delay = o.arrival - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
-            need = kinds[o.kind].footprint
-            with lock:
-                while state["reserved"] > 0 and state["reserved"] + need > self.budget:
-                    lock.wait()
-                state["reserved"] += need
-            o.admitted = time.perf_counter() - t0
-            ready.put(i)
-        for _ in threads:
-            ready.put(None)
+            with cond:
+                pending.append(i)
+                cond.notify_all()
+        with cond:
+            state["done"] = True
+            cond.notify_all()
         for th in threads:
             th.join()
+        if errors:
+            raise RuntimeError(f"{len(errors)} requests failed, first: request {errors[0][0]}: {errors[0][1]}")
         return outcomes
 
 
