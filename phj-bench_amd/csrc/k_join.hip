@@ -472,8 +472,9 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
 // Cell index of key k in the leaf table, kSideSlot for the empty-key
 // sentinel's side chain, or kInvalidRow.
 constexpr uint32_t kSideSlot = 0xFFFEu;
-constexpr int kFusedRowsPerThread = 32;
-constexpr uint32_t kFusedChunkRows = 512u * kFusedRowsPerThread;
+constexpr int kFusedRowsPerThread = 16;
+constexpr int kFusedSuperRounds = 1;
+constexpr uint32_t kFusedChunkRows = kFusedSuperRounds * 512u * kFusedRowsPerThread;
 __device__ __forceinline__ uint32_t lookup_slot(const LdsCell *cells, uint32_t cap, uint32_t side_head, uint64_t k) {
     if (k == kEmptyKey)
         return side_head == kInvalidRow ? kInvalidRow : kSideSlot;
@@ -554,21 +555,92 @@ __device__ __forceinline__ void move_set(const ColSet &in, uint64_t src_row0, co
     }
 }
 
+// Leaf table of the fused join: keys[cap] (u64, kEmptyKey = free) and the
+// leaf-local row of each key, rows[cap] (u16); no chains, since the fused
+// join only runs without repeated keys. About 10 bytes per slot, so three
+// 512-thread workgroups share a CU's LDS for leaves of ~2,500 rows.
+__host__ __device__ __forceinline__ size_t fused_lds_bytes(uint32_t max_rows, uint32_t load_pct) {
+    const size_t cap = table_cap(max_rows, load_pct);
+    return cap * 8 + ((cap * 2 + 15) & ~size_t{15});
+}
+
+// Builds the fused table (64-bit CAS on the key slot, then the row). Sets
+// `dup` (LDS) when a key occurs twice. Ends with a barrier.
+template <int THREADS>
+__device__ __forceinline__ void build_fused_table(const LeafJoinArgs &a, const Item &it, uint32_t cap, uint64_t *tkeys,
+                                                  uint16_t *trows, uint32_t &side_head, uint32_t &dup) {
+    constexpr int KB = 8;
+    uint64_t kk[KB];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+        const uint32_t i = q * THREADS + threadIdx.x;
+        kk[q] = i < it.nr ? a.r_keys[it.r0 + i] : 0;
+    }
+    for (uint32_t i = threadIdx.x; i < cap; i += THREADS)
+        tkeys[i] = kEmptyKey;
+    if (threadIdx.x == 0) {
+        side_head = kInvalidRow;
+        dup = 0;
+    }
+    __syncthreads();
+    bool seen = false;
+    for (uint32_t base = 0; base < it.nr; base += KB * THREADS) {
+        if (base != 0) {
+#pragma unroll
+            for (int q = 0; q < KB; ++q) {
+                const uint32_t i = base + q * THREADS + threadIdx.x;
+                kk[q] = i < it.nr ? a.r_keys[it.r0 + i] : 0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KB; ++q) {
+            const uint32_t i = base + q * THREADS + threadIdx.x;
+            if (i >= it.nr)
+                continue;
+            const uint64_t k = kk[q];
+            if (k == kEmptyKey) {
+                seen |= atomicExch(&side_head, i) != kInvalidRow;
+                continue;
+            }
+            uint32_t pos = home_slot(k, cap);
+            while (true) {
+                const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long *>(&tkeys[pos]),
+                                                         static_cast<unsigned long long>(kEmptyKey),
+                                                         static_cast<unsigned long long>(k));
+                if (old == kEmptyKey) {
+                    trows[pos] = static_cast<uint16_t>(i);
+                    break;
+                }
+                if (old == k) {
+                    seen = true;
+                    break;
+                }
+                pos = (pos + 1 == cap) ? 0 : pos + 1;
+            }
+        }
+    }
+    if (seen)
+        dup = 1u;
+    __syncthreads();
+}
+
 // Fused leaf join for build sides without repeated keys (every probe row has
-// at most one match). One workgroup per (leaf, probe chunk of at most
-// THREADS*K rows) builds the leaf's LDS table, looks every probe row up (the
-// matched cell index is kept in registers), reserves the chunk's output rows
-// with one atomic add on the sink cursor and lays them out slot-major (slot k
-// of every lane, wave by wave, so each column store of a slot is one
-// contiguous run), then writes them in rounds of U slots. The output key is
-// the matched cell's key. Rows at or beyond the sink capacity are counted,
-// not written. A leaf with a repeated key raises *chain_flag and writes
-// nothing: the caller then joins with the two-phase path.
+// at most one match). One workgroup per (leaf, probe chunk) builds the leaf's
+// compact LDS table, then walks the chunk in super-rounds of THREADS*K rows:
+// every row is looked up (LB keys advance in lock step, one probe step per
+// iteration, reads in flight together; the matched slot is kept in
+// registers), the super-round's output rows are reserved with one atomic add
+// on the sink cursor and laid out slot-major (slot k of every lane, wave by
+// wave, so each column store of a slot is one contiguous run), then written
+// in rounds of U slots. The output key is the matched slot's key. Rows at or
+// beyond the sink capacity are counted, not written. A leaf with a repeated
+// key raises the flag and writes nothing: the caller then joins with the
+// two-phase path.
 template <int THREADS, int K, int U>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_leaf_join(LeafJoinArgs a) {
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_leaf_join(LeafJoinArgs a) {
     static_assert(K % U == 0 && K % 2 == 0 && K <= 64, "K slots per lane in rounds of U");
     constexpr int W = THREADS / 64;
-    constexpr int LB = 8; // keys looked up in lock step
+    constexpr int LB = 4; // keys looked up in lock step
     static_assert(K % LB == 0, "lookup batches");
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t slot_off[K][W];
@@ -581,8 +653,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     if (!locate_item(a, w, it))
         return;
     const uint32_t cap = table_cap(it.nr, a.load_pct);
-    LdsCell *cells = reinterpret_cast<LdsCell *>(smem);
-    uint16_t *next = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * sizeof(LdsCell));
+    uint64_t *tkeys = reinterpret_cast<uint64_t *>(smem);
+    uint16_t *trows = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * 8);
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / 64;
 
@@ -595,19 +667,19 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         out_cap = a.join_end[j];
         flag = a.join_flag + j;
     }
-    build_leaf_table<THREADS>(a, it, cap, cells, next, side_head, dup);
+    build_fused_table<THREADS>(a, it, cap, tkeys, trows, side_head, dup);
     if (dup) {
         if (threadIdx.x == 0)
             atomicOr(flag, 1u);
         return;
     }
-    {
-        const uint32_t sb = it.s0; // one super-round: items hold at most THREADS*K probe rows
-        // slot code per row, two per word: 0 = no match, else cell index + 1
+#pragma unroll
+    for (int sr = 0; sr < kFusedSuperRounds; ++sr) {
+        const uint32_t sb = it.s0 + sr * THREADS * K;
+        if (sb >= it.s1)
+            break; // uniform
+        // slot code per row, two per word: 0 = no match, else slot index + 1
         uint32_t code[K / 2];
-        // Lookups run LB keys at a time in lock step (one probe step of every
-        // pending key per iteration, all reads in flight together); the next
-        // LB keys load meanwhile.
         uint64_t key[LB], nkey[LB];
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
@@ -641,7 +713,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                 uint64_t ck[LB];
 #pragma unroll
                 for (int q = 0; q < LB; ++q)
-                    ck[q] = cells[pos[q]].key;
+                    ck[q] = tkeys[pos[q]];
 #pragma unroll
                 for (int q = 0; q < LB; ++q) {
                     const bool p = (pend >> q) & 1u;
@@ -720,9 +792,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                             okey[u] = kEmptyKey;
                             row = side_head;
                         } else {
-                            const LdsCell cell = cells[c - 1];
-                            okey[u] = cell.key;
-                            row = cell.row_no;
+                            okey[u] = tkeys[c - 1];
+                            row = trows[c - 1];
                         }
                     }
                     ridx[u] = row;
@@ -736,6 +807,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
                 move_set<U>(a.s_cols, sb, a.out.right, rb, sidx, opos, on);
             }
         }
+        __syncthreads(); // slot_off and sr_* are rewritten by the next super-round
     }
 }
 
@@ -1028,7 +1100,7 @@ void leaf_chain_slots(hipStream_t s, const LeafJoinArgs &a, uint32_t *chain_firs
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm) {
     if (max_work == 0)
         return;
-    const size_t lds = leaf_join_lds_bytes(a.lds_max_rows, a.load_pct);
+    const size_t lds = fused_lds_bytes(a.lds_max_rows, a.load_pct);
     TimedScope ts(tm, "leaf_join", s);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2>), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(lds));
