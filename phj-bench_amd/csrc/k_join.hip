@@ -138,7 +138,7 @@ __device__ __forceinline__ void build_leaf_table(const LeafJoinArgs &a, const It
 #pragma unroll
     for (int q = 0; q < KB; ++q) {
         const uint32_t i = q * THREADS + threadIdx.x;
-        kk[q] = i < it.nr ? a.r_keys[it.r0 + i] : 0;
+        kk[q] = a.r_keys[it.r0 + min(i, it.nr - 1)]; // rows past the leaf are skipped below
     }
     for (uint32_t i = threadIdx.x; i < cap; i += THREADS) {
         LdsCell c;
@@ -159,7 +159,7 @@ __device__ __forceinline__ void build_leaf_table(const LeafJoinArgs &a, const It
 #pragma unroll
             for (int q = 0; q < KB; ++q) {
                 const uint32_t i = base + q * THREADS + threadIdx.x;
-                kk[q] = i < it.nr ? a.r_keys[it.r0 + i] : 0;
+                kk[q] = a.r_keys[it.r0 + min(i, it.nr - 1)];
             }
         }
 #pragma unroll
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t j = base + u * THREADS + threadIdx.x;
-            key[u] = j < it.s1 ? a.s_keys[j] : 0;
+            key[u] = a.s_keys[min(j, it.s1 - 1)];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -288,11 +288,10 @@ template <class T> __device__ __forceinline__ T *optr(const OutColSet &s, uint32
 template <int U, class T>
 __device__ __forceinline__ void gather_col(const T *__restrict__ src, T *__restrict__ dst, const uint32_t (&idx)[U],
                                            const uint64_t (&pos)[U], const bool (&on)[U]) {
-    T v[U] = {};
+    T v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-        if (on[u])
-            v[u] = src[idx[u]];
+        v[u] = src[on[u] ? idx[u] : 0u];
 #pragma unroll
     for (int u = 0; u < U; ++u)
         if (on[u])
@@ -307,11 +306,8 @@ __device__ __forceinline__ void gather_col128(const uint64_t *__restrict__ src, 
     for (int half = 0; half < 2; ++half) {
         uint64_t v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            v[u] = 0;
-            if (on[u])
-                v[u] = src[2 * static_cast<uint64_t>(idx[u]) + half];
-        }
+        for (int u = 0; u < U; ++u)
+            v[u] = src[2 * static_cast<uint64_t>(on[u] ? idx[u] : 0u) + half];
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (on[u])
@@ -404,7 +400,8 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t j = base + u * THREADS + threadIdx.x;
-            h[u] = j < it.s1 ? a.heads[j] : kNoNext;
+            const uint16_t hv = a.heads[min(j, it.s1 - 1)];
+            h[u] = j < it.s1 ? hv : kNoNext;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -437,11 +434,8 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             {
                 uint64_t kv[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    kv[u] = 0;
-                    if (on[u])
-                        kv[u] = a.s_keys[sidx[u]];
-                }
+                for (int u = 0; u < U; ++u)
+                    kv[u] = a.s_keys[on[u] ? sidx[u] : 0u];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                     if (on[u])
@@ -503,13 +497,12 @@ template <class T> __device__ __forceinline__ void st_at(void *base, uint32_t of
 template <int U, class T>
 __device__ __forceinline__ void move_col(const void *src, void *dst, const uint32_t (&idx)[U],
                                          const uint32_t (&pos)[U], const bool (&on)[U]) {
+    // Unconditional loads (row 0 stands in for lanes without a row): a load
+    // under a branch makes the compiler wait for it before the branch joins.
     T v[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        v[u] = T{};
-        if (on[u])
-            v[u] = ld_at<T>(src, idx[u] * static_cast<uint32_t>(sizeof(T)));
-    }
+    for (int u = 0; u < U; ++u)
+        v[u] = ld_at<T>(src, (on[u] ? idx[u] : 0u) * static_cast<uint32_t>(sizeof(T)));
 #pragma unroll
     for (int u = 0; u < U; ++u)
         if (on[u])
@@ -523,11 +516,8 @@ __device__ __forceinline__ void move_col128(const void *src, void *dst, const ui
     for (int half = 0; half < 2; ++half) {
         uint64_t v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            v[u] = 0;
-            if (on[u])
-                v[u] = ld_at<uint64_t>(src, idx[u] * 16u + 8u * half);
-        }
+        for (int u = 0; u < U; ++u)
+            v[u] = ld_at<uint64_t>(src, (on[u] ? idx[u] : 0u) * 16u + 8u * half);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (on[u])
@@ -574,7 +564,7 @@ __device__ __forceinline__ void build_fused_table(const LeafJoinArgs &a, const I
 #pragma unroll
     for (int q = 0; q < KB; ++q) {
         const uint32_t i = q * THREADS + threadIdx.x;
-        kk[q] = i < it.nr ? a.r_keys[it.r0 + i] : 0;
+        kk[q] = a.r_keys[it.r0 + min(i, it.nr - 1)]; // rows past the leaf are skipped below
     }
     for (uint32_t i = threadIdx.x; i < cap; i += THREADS)
         tkeys[i] = kEmptyKey;
@@ -589,7 +579,7 @@ __device__ __forceinline__ void build_fused_table(const LeafJoinArgs &a, const I
 #pragma unroll
             for (int q = 0; q < KB; ++q) {
                 const uint32_t i = base + q * THREADS + threadIdx.x;
-                kk[q] = i < it.nr ? a.r_keys[it.r0 + i] : 0;
+                kk[q] = a.r_keys[it.r0 + min(i, it.nr - 1)];
             }
         }
 #pragma unroll
@@ -684,7 +674,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
 #pragma unroll
         for (int q = 0; q < LB; ++q) {
             const uint32_t j = sb + q * THREADS + threadIdx.x;
-            key[q] = j < it.s1 ? a.s_keys[j] : 0;
+            key[q] = a.s_keys[min(j, it.s1 - 1)]; // rows past the chunk are masked below
         }
 #pragma unroll
         for (int g = 0; g < K / LB; ++g) {
@@ -692,7 +682,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
 #pragma unroll
                 for (int q = 0; q < LB; ++q) {
                     const uint32_t j = sb + ((g + 1) * LB + q) * THREADS + threadIdx.x;
-                    nkey[q] = j < it.s1 ? a.s_keys[j] : 0;
+                    nkey[q] = a.s_keys[min(j, it.s1 - 1)];
                 }
             }
             uint32_t pos[LB], res[LB];

@@ -132,13 +132,19 @@ __global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restric
         (&sub[0][0])[i] = 0;
     __syncthreads();
     const uint32_t wave = threadIdx.x / 64;
+    // all loads first: the LDS atomics below would otherwise order each load
+    // after the previous row's atomic
+    uint64_t k[kPartRpt];
 #pragma unroll
     for (int r = 0; r < kPartRpt; ++r) {
         const uint32_t idx = r * kPartThreads + threadIdx.x;
-        if (idx < L.len) {
-            const uint64_t h = int_hash64(keys[L.row0 + idx]);
-            atomicAdd(&sub[wave][digit_of(h, shift, bits)], 1u);
-        }
+        k[r] = __builtin_nontemporal_load(keys + L.row0 + min(idx, L.len - 1)); // past the tile: masked below
+    }
+#pragma unroll
+    for (int r = 0; r < kPartRpt; ++r) {
+        const uint32_t idx = r * kPartThreads + threadIdx.x;
+        if (idx < L.len)
+            atomicAdd(&sub[wave][digit_of(int_hash64(k[r]), shift, bits)], 1u);
     }
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < B; d += kPartThreads) {
@@ -350,8 +356,7 @@ __device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__res
 #pragma unroll
     for (int r = 0; r < kPartRpt; ++r) {
         const uint32_t idx = r * kPartThreads + threadIdx.x;
-        if (idx < L.len)
-            v[r] = ld<NTL>(in + L.row0 + idx);
+        v[r] = ld<NTL>(in + L.row0 + min(idx, L.len - 1)); // rows past the tile carry no slot
     }
     write_staged<NTS, T>(sh, v, out, L, slot);
 }
@@ -381,9 +386,13 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
 #pragma unroll
     for (int r = 0; r < kPartRpt; ++r) {
         const uint32_t idx = r * kPartThreads + threadIdx.x;
+        key[r] = ld<NTL>(keys_in + L.row0 + min(idx, L.len - 1)); // past the tile: masked below
+    }
+#pragma unroll
+    for (int r = 0; r < kPartRpt; ++r) {
+        const uint32_t idx = r * kPartThreads + threadIdx.x;
         slot[r] = kNoSlot;
         if (idx < L.len) {
-            key[r] = ld<NTL>(keys_in + L.row0 + idx);
             const uint32_t d = digit_of(int_hash64(key[r]), shift, bits);
             slot[r] = (atomicAdd(&sh.cursor[d], 1u) << 8) | d;
         }
