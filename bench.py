@@ -307,7 +307,12 @@ def main():
 
     build.release()
     probe.release()
-    batched = batched_bench(ctx, args.batch_joins) if args.batch_joins > 0 else None
+    ctx.close()  # each workload below runs in a context of its own, on a device with nothing cached
+    batched = None
+    if args.batch_joins > 0:
+        bctx = G.Context(0)
+        batched = batched_bench(bctx, args.batch_joins)
+        bctx.close()
     skew = skew_bench(scale_log2=args.scale_log2) if not args.no_skew else None
     serving = None
     if args.serve_requests > 0:
@@ -354,7 +359,6 @@ def main():
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    ctx.close()
     if (not parity or (batched is not None and not batched["parity_per_join"])
             or (serving is not None and not serving["parity_alone_vs_mix"])
             or (skew is not None and not skew["schemes_agree"])):
