@@ -64,8 +64,8 @@ def cpu_baseline(scale_log2, threads):
 def batched_bench(ctx, n_joins, seed=42):
     """4096 independent uniform joins (|R_i| = floor(2^U[10,20]) distinct keys,
     |S_i| = 8 |R_i|, 75% match, u64 payload each side), back to back in HBM:
-    one phj_join_batch call against one runPHJ call per join. Per-join
-    checksums of the two must agree."""
+    one phj_join_batch call (median of 3) against one runPHJ call per join.
+    Per-join checksums of the two must agree."""
     specs = G.batched_join_specs(seed=seed, n_joins=n_joins)
     build, probe, boff, poff = ctx.generate_batch(specs)
     pairs = [(build.slice(int(boff[i]), int(boff[i + 1])), probe.slice(int(poff[i]), int(poff[i + 1])))
@@ -73,25 +73,36 @@ def batched_bench(ctx, n_joins, seed=42):
     cfgs = [G.RadixConfig.for_build_rows(b.rows) for b, _ in pairs]
     R, S = build.rows, probe.rows
     log(f"batch: {n_joins} joins, {R} build + {S} probe rows")
-    # warm-up of both paths on the first joins
-    for r in ctx.join_batch(pairs[:64], cfgs[:64]):
-        r.release()
-    for (b, p), cfg in zip(pairs[:16], cfgs[:16]):
-        ctx.runPHJ(b, p, cfg).release()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    res = ctx.join_batch(pairs, cfgs)
-    ctx.synchronize()
-    t_batch = time.perf_counter() - t0
-    got = [r.checksum() for r in res]
-    out_rows = sum(r.out_rows for r in res)
-    for r in res:
-        r.release()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    plain = [ctx.runPHJ(b, p, cfg) for (b, p), cfg in zip(pairs, cfgs)]
-    ctx.synchronize()
-    t_plain = time.perf_counter() - t0
+
+    def run_batched():
+        return ctx.join_batch(pairs, cfgs)
+
+    def run_plain():
+        return [ctx.runPHJ(b, p, cfg) for (b, p), cfg in zip(pairs, cfgs)]
+
+    def timed(run):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        res = run()
+        ctx.synchronize()
+        return time.perf_counter() - t0, res
+
+    # One untimed run of the whole workload per path first: the context's
+    # device pool then holds the memory both paths need (fresh device memory
+    # is cleared by the driver when first mapped, which is not join work).
+    for run in (run_batched, run_plain):
+        for r in run():
+            r.release()
+    t_batches = []
+    for _ in range(3):
+        t, res = timed(run_batched)
+        t_batches.append(t)
+        got = [r.checksum() for r in res]
+        out_rows = sum(r.out_rows for r in res)
+        for r in res:
+            r.release()
+    t_batch = sorted(t_batches)[1]
+    t_plain, plain = timed(run_plain)
     want = [r.checksum() for r in plain]
     for r in plain:
         r.release()
