@@ -27,6 +27,11 @@ namespace {
 constexpr int kPartThreads = 512;
 constexpr int kPartRpt = 8;
 constexpr int kTile = kPartThreads * kPartRpt; // rows per tile
+// The scatter covers the same tile with twice the threads and half the rows
+// each: at 64 VGPRs two of its workgroups share a CU (eight waves per SIMD),
+// where 512 threads x 8 rows needed 146 VGPRs and ran one workgroup per CU.
+constexpr int kScatThreads = 1024;
+constexpr int kScatRpt = kTile / kScatThreads;
 constexpr int kMaxDigits = 256;                // at most 8 bits per device pass
 constexpr int kChunkTiles = 256;               // tiles per column-scan chunk
 constexpr int kScanThreads = 512;
@@ -319,7 +324,7 @@ struct ScatterShared {
     uint32_t cursor[kMaxDigits];
     uint32_t lstart[kMaxDigits];
     uint32_t gbase[kMaxDigits];
-    uint32_t scan_tmp[kPartThreads / 64 + 1];
+    uint32_t scan_tmp[kScatThreads / 64 + 1];
 };
 
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
@@ -327,8 +332,8 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 // Values already in registers are staged in LDS at their digit-ordered slot,
 // then leave in slot order so each digit's rows form one contiguous run.
 template <bool NTS, class T>
-__device__ __forceinline__ void write_staged(ScatterShared &sh, const T (&v)[kPartRpt], T *__restrict__ out,
-                                             const TileLoc &L, const uint32_t (&slot)[kPartRpt]) {
+__device__ __forceinline__ void write_staged(ScatterShared &sh, const T (&v)[kScatRpt], T *__restrict__ out,
+                                             const TileLoc &L, const uint32_t (&slot)[kScatRpt]) {
     constexpr uint32_t kPerRound = sizeof(T) <= 8 ? kTile : kTile * 8 / sizeof(T);
     constexpr uint32_t kRounds = kTile / kPerRound;
     T *stg = reinterpret_cast<T *>(sh.stage);
@@ -336,12 +341,12 @@ __device__ __forceinline__ void write_staged(ScatterShared &sh, const T (&v)[kPa
     for (uint32_t round = 0; round < kRounds; ++round) {
         const uint32_t lo = round * kPerRound;
 #pragma unroll
-        for (int r = 0; r < kPartRpt; ++r)
+        for (int r = 0; r < kScatRpt; ++r)
             if (slot[r] - lo < kPerRound) // kNoSlot never qualifies
                 stg[slot[r] - lo] = v[r];
         __syncthreads();
         const uint32_t hi = min(L.len, lo + kPerRound);
-        for (uint32_t k = lo + threadIdx.x; k < hi; k += kPartThreads) {
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += kScatThreads) {
             const uint32_t d = sh.slot_digit[k];
             st<NTS>(out + sh.gbase[d] + (k - sh.lstart[d]), stg[k - lo]);
         }
@@ -351,18 +356,19 @@ __device__ __forceinline__ void write_staged(ScatterShared &sh, const T (&v)[kPa
 
 template <bool NTL, bool NTS, class T>
 __device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__restrict__ in, T *__restrict__ out,
-                                               const TileLoc &L, const uint32_t (&slot)[kPartRpt]) {
-    T v[kPartRpt];
+                                               const TileLoc &L, const uint32_t (&slot)[kScatRpt]) {
+    T v[kScatRpt];
 #pragma unroll
-    for (int r = 0; r < kPartRpt; ++r) {
-        const uint32_t idx = r * kPartThreads + threadIdx.x;
+    for (int r = 0; r < kScatRpt; ++r) {
+        const uint32_t idx = r * kScatThreads + threadIdx.x;
         v[r] = ld<NTL>(in + L.row0 + min(idx, L.len - 1)); // rows past the tile carry no slot
     }
     write_staged<NTS, T>(sh, v, out, L, slot);
 }
 
 template <bool NTL, bool NTS>
-__global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
+__global__ __launch_bounds__(kScatThreads) __attribute__((amdgpu_waves_per_eu(8)))
+void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
                                                           uint64_t *__restrict__ keys_out, OutColSet cout,
                                                           const uint32_t *__restrict__ seg_off,
                                                           const uint32_t *__restrict__ seg_first, uint32_t n_seg,
@@ -376,21 +382,21 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
     const uint32_t B = 1u << pb.bits;
     uint32_t shift, bits;
     seg_window(pb, L.seg, shift, bits);
-    for (uint32_t i = threadIdx.x; i < B; i += kPartThreads)
+    for (uint32_t i = threadIdx.x; i < B; i += kScatThreads)
         sh.cursor[i] = 0;
     __syncthreads();
 
     // Rank each row inside its digit with one LDS atomic (rank << 8 | digit).
-    uint64_t key[kPartRpt];
-    uint32_t slot[kPartRpt];
+    uint64_t key[kScatRpt];
+    uint32_t slot[kScatRpt];
 #pragma unroll
-    for (int r = 0; r < kPartRpt; ++r) {
-        const uint32_t idx = r * kPartThreads + threadIdx.x;
+    for (int r = 0; r < kScatRpt; ++r) {
+        const uint32_t idx = r * kScatThreads + threadIdx.x;
         key[r] = ld<NTL>(keys_in + L.row0 + min(idx, L.len - 1)); // past the tile: masked below
     }
 #pragma unroll
-    for (int r = 0; r < kPartRpt; ++r) {
-        const uint32_t idx = r * kPartThreads + threadIdx.x;
+    for (int r = 0; r < kScatRpt; ++r) {
+        const uint32_t idx = r * kScatThreads + threadIdx.x;
         slot[r] = kNoSlot;
         if (idx < L.len) {
             const uint32_t d = digit_of(int_hash64(key[r]), shift, bits);
@@ -401,7 +407,7 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
     {
         const uint32_t c = threadIdx.x < B ? sh.cursor[threadIdx.x] : 0u;
         uint32_t total;
-        const uint32_t off = block_excl_scan<kPartThreads>(c, sh.scan_tmp, &total);
+        const uint32_t off = block_excl_scan<kScatThreads>(c, sh.scan_tmp, &total);
         if (threadIdx.x < B) {
             const uint32_t chunk = seg_first[n_seg + 1 + L.seg] + L.local / kChunkTiles;
             sh.lstart[threadIdx.x] = off;
@@ -410,7 +416,7 @@ __global__ __launch_bounds__(kPartThreads) void k_scatter(const uint64_t *__rest
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kPartRpt; ++r) {
+    for (int r = 0; r < kScatRpt; ++r) {
         if (slot[r] != kNoSlot) {
             const uint32_t d = slot[r] & 0xFFu;
             const uint32_t sl = sh.lstart[d] + (slot[r] >> 8);
@@ -508,7 +514,7 @@ void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint6
         const bool ntl = flags & kRadixNtLoads, nts = flags & kRadixNtStores;
         auto k = ntl ? (nts ? k_scatter<true, true> : k_scatter<true, false>)
                      : (nts ? k_scatter<false, true> : k_scatter<false, false>);
-        k<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg, pb,
+        k<<<grid8(max_tiles), kScatThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg, pb,
                                                      scr.hist, scr.block_sums);
     }
 }
