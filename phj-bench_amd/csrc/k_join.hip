@@ -337,6 +337,8 @@ template <int THREADS, int U>
 struct RoundLayout {
     uint32_t single_pos[U]; // round-relative position of this thread's slot u (single match)
     uint32_t chain_pos;     // round-relative position of this thread's first multi-match row
+    uint32_t chain_start;   // round-relative position of the first multi-match row
+    uint32_t chain_total;   // multi-match rows of the round
     uint32_t total;
 };
 
@@ -374,25 +376,49 @@ __device__ __forceinline__ void round_layout(const uint16_t (&h)[U], const uint3
         running += tot;
     }
     L.chain_pos = running + chain_off;
+    L.chain_start = running;
+    L.chain_total = chain_total;
     L.total = running + chain_total;
+}
+
+// Multi-match rows are written kEmitBatch at a time: (round-relative probe
+// row, leaf-local build row) pairs are laid out in LDS in output order, then
+// the workgroup writes them to consecutive output positions.
+constexpr uint32_t kEmitBatch = 4096;
+constexpr int kEmitVec = 4;
+
+__host__ __device__ constexpr size_t emit_chain_lds(uint32_t lds_max_rows) {
+    return (static_cast<size_t>(lds_max_rows) * 2 + 15) & ~size_t{15};
 }
 
 // Emit pass: a streaming kernel over the probe rows of each work item using
 // the recorded heads; no table. Output row = probe key | build columns of the
 // matched row | probe columns (src/JoinOps.h:77-125), at the item's offset.
-// CHAINS == false writes the single-match rows column by column (every load of
-// a column issued before its stores); CHAINS == true derives the same layout
-// and writes only the multi-match rows, walking their chains.
+// The item's chains are first copied to LDS (dynamic shared memory), so chain
+// walks are LDS reads. CHAINS == false writes the single-match rows column by
+// column (every load of a column issued before its stores); CHAINS == true
+// derives the same layout and writes only the multi-match rows: each thread
+// walks its chains into an LDS batch of (probe row, build row) pairs, then
+// the workgroup writes the batch's rows at consecutive output positions.
 template <int THREADS, int U, bool CHAINS>
 __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t tmp[THREADS / 64 + 1];
     __shared__ uint32_t wave_cnt[2][U][THREADS / 64];
     Item it;
     const uint32_t w = xcd_share(blockIdx.x, a.work_first[a.n_leaves]);
     if (!locate_item(a, w, it))
         return;
+    uint16_t *chains = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *hl = reinterpret_cast<uint16_t *>(smem + emit_chain_lds(a.lds_max_rows)); // [U][THREADS]
+    uint32_t *ent = reinterpret_cast<uint32_t *>(smem + emit_chain_lds(a.lds_max_rows) + U * THREADS * 2);
+    {
+        const uint16_t *src = a.next16 + item_chain_base(a, it);
+        for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS)
+            chains[i] = src[i];
+        __syncthreads();
+    }
     uint64_t round_base = a.out_base + a.item_off[w];
-    const uint16_t *chains = a.next16 + item_chain_base(a, it);
     uint32_t parity = 0;
     for (uint32_t base = it.s0; base < it.s1; base += THREADS * U, parity ^= 1) {
         uint16_t h[U];
@@ -443,20 +469,60 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             }
             gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
             gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
-        } else {
-            uint64_t pos = round_base + L.chain_pos;
-            for (int u = 0; u < U; ++u) {
-                if (chainlen[u] == 0)
-                    continue;
-                const uint32_t j = base + u * THREADS + threadIdx.x;
-                const uint64_t k = a.s_keys[j];
-                uint16_t x = h[u];
-                while (true) {
-                    emit_row(a.out, pos++, k, a.r_cols, it.r0 + (x & kRowMask), a.s_cols, j);
-                    if (!(x & kMore))
-                        break;
-                    x = chains[x & kRowMask];
+        } else if (L.chain_total > 0) {
+            // this thread's heads of multi-match rows, indexable at run time
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                hl[u * THREADS + threadIdx.x] = chainlen[u] ? h[u] : kNoNext;
+            uint32_t cu = 0; // slot being walked (U: done)
+            while (cu < U && hl[cu * THREADS + threadIdx.x] == kNoNext)
+                ++cu;
+            uint16_t cx = cu < U ? hl[cu * THREADS + threadIdx.x] : kNoNext;
+            uint32_t cpos = L.chain_pos - L.chain_start; // next entry of this thread in the round's chain rows
+            for (uint32_t b0 = 0; b0 < L.chain_total; b0 += kEmitBatch) {
+                const uint32_t b1 = min(L.chain_total, b0 + kEmitBatch);
+                while (cu < U && cpos < b1) {
+                    ent[cpos - b0] = ((cu * THREADS + threadIdx.x) << 16) | (cx & kRowMask);
+                    ++cpos;
+                    if (cx & kMore) {
+                        cx = chains[cx & kRowMask];
+                    } else {
+                        ++cu;
+                        while (cu < U && hl[cu * THREADS + threadIdx.x] == kNoNext)
+                            ++cu;
+                        cx = cu < U ? hl[cu * THREADS + threadIdx.x] : kNoNext;
+                    }
                 }
+                __syncthreads();
+                // kEmitVec rows per thread per step, column by column (every
+                // load of a column issued before its stores)
+                const uint64_t pos0 = round_base + L.chain_start + b0;
+                const uint32_t n = b1 - b0;
+                for (uint32_t k0 = 0; k0 < n; k0 += THREADS * kEmitVec) {
+                    uint32_t ridx[kEmitVec], sidx[kEmitVec];
+                    uint64_t opos[kEmitVec];
+                    bool on[kEmitVec];
+#pragma unroll
+                    for (int v = 0; v < kEmitVec; ++v) {
+                        const uint32_t k = k0 + v * THREADS + threadIdx.x;
+                        on[v] = k < n;
+                        const uint32_t e = ent[on[v] ? k : 0u];
+                        sidx[v] = base + (e >> 16);
+                        ridx[v] = it.r0 + (e & 0xFFFFu);
+                        opos[v] = pos0 + k;
+                    }
+                    uint64_t kv[kEmitVec];
+#pragma unroll
+                    for (int v = 0; v < kEmitVec; ++v)
+                        kv[v] = a.s_keys[sidx[v]];
+#pragma unroll
+                    for (int v = 0; v < kEmitVec; ++v)
+                        if (on[v])
+                            a.out.keys[opos[v]] = kv[v];
+                    gather_set<kEmitVec>(a.r_cols, a.out.left, ridx, opos, on);
+                    gather_set<kEmitVec>(a.s_cols, a.out.right, sidx, opos, on);
+                }
+                __syncthreads();
             }
         }
         round_base += L.total;
@@ -1100,13 +1166,19 @@ void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, Ke
 void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm) {
     if (max_work == 0)
         return;
+    const size_t lds_chains = emit_chain_lds(a.lds_max_rows);
     {
         TimedScope ts(tm, "leaf_emit", s);
-        k_leaf_emit<256, 8, false><<<grid8(max_work), 256, 0, s>>>(a);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<256, 8, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_chains));
+        k_leaf_emit<256, 8, false><<<grid8(max_work), 256, lds_chains, s>>>(a);
     }
     if (chains) {
         TimedScope ts(tm, "leaf_emit_chains", s);
-        k_leaf_emit<256, 8, true><<<grid8(max_work), 256, 0, s>>>(a);
+        const size_t lds = lds_chains + 8 * 256 * 2 + kEmitBatch * 4;
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<256, 8, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        k_leaf_emit<256, 8, true><<<grid8(max_work), 256, lds, s>>>(a);
     }
 }
 

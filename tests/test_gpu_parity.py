@@ -276,6 +276,29 @@ def test_chains_across_work_items(ctx, chunk):
         res.release()
 
 
+@pytest.mark.parametrize("bits", [[1], [2], [3]])
+def test_long_chains_in_emit_batches(ctx, bits):
+    """300 keys repeated 40 times each, every probe row matching: a round of
+    2,048 probe rows yields ~80,000 multi-match output rows, far more than one
+    LDS batch of the cooperative chain emission, and chains of 40 rows run
+    across batch boundaries. Leaves of ~6,000 build rows ([1]) go to HBM
+    tables; ~3,000 and ~1,500 ([2], [3]) stay in LDS."""
+    rng = np.random.default_rng(77)
+    distinct = rng.permutation(np.unique(rng.integers(1, 2**63, 400, dtype=np.uint64))[:300])
+    bkeys = rng.permutation(np.repeat(distinct, 40))
+    bcol = np.arange(bkeys.size, dtype=np.uint32)
+    pkeys = distinct[rng.integers(0, distinct.size, 20000)]
+    pcol = rng.integers(0, 2**63, (pkeys.size, 2), dtype=np.uint64)
+    ob = O.Relation(bkeys, [bcol], [G.U32])
+    op = O.Relation(pkeys, [pcol], [G.U128])
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    assert want.shape[0] == 20000 * 40
+    res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(bits))
+    assert res.out_rows == want.shape[0]
+    assert np.array_equal(sorted_rows_of(res.to_host()), want)
+    res.release()
+
+
 def test_join_rejects_bad_radix(ctx):
     ob, op = data()["uniform_half"]
     with pytest.raises(ValueError):
