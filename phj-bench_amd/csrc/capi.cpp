@@ -727,7 +727,8 @@ private:
             uint32_t *ctmp = scr_.get<uint32_t>(radix_scan_blocks(n_leaves + 1) + 1);
             leaf_chain_slots(ctx_->stream, a_, chain_first, ctmp);
             a_.chain_first = chain_first;
-            a_.next16 = scr_.get<uint16_t>(std::max<uint32_t>(read_scalar(ctx_, chain_first + n_leaves), 1));
+            // per work item: its row groups and their lengths, leaf rows each
+            a_.next16 = scr_.get<uint16_t>(2 * static_cast<uint64_t>(std::max<uint32_t>(read_scalar(ctx_, chain_first + n_leaves), 1)));
             a_.chain_flag = scr_.get<uint32_t>(1);
             launch_fill_u32(ctx_->stream, a_.chain_flag, 1, 0);
             leaf_probe(ctx_->stream, a_, max_work_, plan_.threads, ctx_->tm());

@@ -119,9 +119,13 @@ __device__ __forceinline__ bool locate_item(const LeafJoinArgs &a, uint32_t w, I
     return true;
 }
 
-// First chain-export slot of a work item (two-phase join).
+// u16 entries of an LDS array of n rows, padded to 16 bytes.
+__host__ __device__ constexpr uint32_t round_up16(uint32_t n) { return (n + 7u) & ~7u; }
+
+// First chain-export slot of a work item (two-phase join): the item exports
+// 2 * nr entries, its groups (perm, nr) and their lengths (glen, nr).
 __device__ __forceinline__ uint64_t item_chain_base(const LeafJoinArgs &a, const Item &it) {
-    return static_cast<uint64_t>(a.chain_first[it.leaf]) + static_cast<uint64_t>(it.item) * it.nr;
+    return 2 * (static_cast<uint64_t>(a.chain_first[it.leaf]) + static_cast<uint64_t>(it.item) * it.nr);
 }
 
 // Builds the leaf's open-addressing table of 16-byte cells in LDS (64-bit CAS
@@ -209,9 +213,13 @@ __device__ __forceinline__ uint32_t lookup_leaf(const LdsCell *cells, uint32_t c
 }
 
 // Probe pass over one (leaf, probe chunk) work item: build the leaf's table,
-// look up every probe row, and record per probe row the matched build row
-// (heads, 16 bits with a "chain continues" flag) and the item's output rows.
-// Chains are exported as next16 so the emit pass needs no table.
+// look up every probe row, and record per probe row its match (heads, 16
+// bits) and the item's output rows. Each key's chain (src/BlockStore.h:86-89,
+// newest row first) is laid out once per item as one contiguous group of a
+// row permutation: perm[start .. start + len) lists the chain in order, and
+// glen[start] = len. A probe row with one match records the build row; one
+// with several records start | kMore. The emit pass then reads groups instead
+// of walking chains.
 template <int THREADS, int U>
 __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -226,16 +234,48 @@ __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
     const uint32_t cap = table_cap(it.nr, a.load_pct);
     LdsCell *cells = reinterpret_cast<LdsCell *>(smem);
     uint16_t *next = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * sizeof(LdsCell));
+    uint16_t *gstart = next + round_up16(it.nr);  // per head row: its group's start
+    uint16_t *glen_h = gstart + round_up16(it.nr); // per head row: its group's length
     if (threadIdx.x == 0)
         blk_chain = 0;
     build_leaf_table<THREADS>(a, it, cap, cells, next, side_head, dup);
-    // Export this item's chains (successor plus whether the successor
-    // continues) to the item's own region: the heads below refer to them.
-    uint16_t *chains = a.next16 + item_chain_base(a, it);
-    for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
-        const uint16_t nx = next[i];
-        chains[i] = nx == kNoNext ? kNoNext : static_cast<uint16_t>(nx | (next[nx] != kNoNext ? kMore : 0));
+
+    // Groups in cell order, then the empty-key sentinel's side chain (group
+    // index cap). Each thread owns a contiguous range of group indices.
+    uint16_t *perm = a.next16 + item_chain_base(a, it);
+    uint16_t *glen = perm + it.nr;
+    const uint32_t per = (cap + 1 + THREADS - 1) / THREADS;
+    const uint32_t g0 = min(cap + 1, threadIdx.x * per), g1 = min(cap + 1, g0 + per);
+    auto head_of = [&](uint32_t g) -> uint32_t { return g < cap ? cells[g].row_no : side_head; };
+    uint32_t mine = 0;
+    for (uint32_t g = g0; g < g1; ++g) {
+        const uint32_t hd = head_of(g);
+        if (hd == kInvalidRow)
+            continue;
+        uint32_t len = 1;
+        for (uint16_t x = next[hd]; x != kNoNext; x = next[x])
+            ++len;
+        glen_h[hd] = static_cast<uint16_t>(len);
+        mine += len;
     }
+    uint32_t total;
+    uint32_t start = block_excl_scan<THREADS>(mine, tmp, &total);
+    for (uint32_t g = g0; g < g1; ++g) {
+        const uint32_t hd = head_of(g);
+        if (hd == kInvalidRow)
+            continue;
+        const uint32_t len = glen_h[hd];
+        gstart[hd] = static_cast<uint16_t>(start);
+        glen[start] = static_cast<uint16_t>(len);
+        uint32_t e = start;
+        for (uint32_t x = hd; x != kInvalidRow;) {
+            perm[e++] = static_cast<uint16_t>(x);
+            const uint16_t nx = next[x];
+            x = nx == kNoNext ? kInvalidRow : nx;
+        }
+        start += len;
+    }
+    __syncthreads();
 
     uint32_t cnt = 0;
     bool saw_chain = false;
@@ -254,13 +294,13 @@ __global__ __launch_bounds__(THREADS) void k_leaf_probe(LeafJoinArgs a) {
             const uint32_t hd = lookup_leaf(cells, cap, side_head, key[u]);
             uint16_t h16 = kNoNext;
             if (hd != kInvalidRow) {
-                h16 = static_cast<uint16_t>(hd | (next[hd] != kNoNext ? kMore : 0));
-                if (h16 & kMore)
-                    saw_chain = true;
-                for (uint32_t x = hd; x != kInvalidRow;) {
+                if (next[hd] == kNoNext) {
+                    h16 = static_cast<uint16_t>(hd);
                     ++cnt;
-                    const uint16_t nx = next[x];
-                    x = nx == kNoNext ? kInvalidRow : nx;
+                } else {
+                    h16 = static_cast<uint16_t>(gstart[hd] | kMore);
+                    cnt += glen_h[hd];
+                    saw_chain = true;
                 }
             }
             a.heads[j] = h16;
@@ -387,19 +427,20 @@ __device__ __forceinline__ void round_layout(const uint16_t (&h)[U], const uint3
 constexpr uint32_t kEmitBatch = 4096;
 constexpr int kEmitVec = 4;
 
+// LDS of the emit kernels for the item's groups: perm and glen, nr each.
 __host__ __device__ constexpr size_t emit_chain_lds(uint32_t lds_max_rows) {
-    return (static_cast<size_t>(lds_max_rows) * 2 + 15) & ~size_t{15};
+    return 2 * static_cast<size_t>(round_up16(lds_max_rows)) * 2;
 }
 
 // Emit pass: a streaming kernel over the probe rows of each work item using
 // the recorded heads; no table. Output row = probe key | build columns of the
 // matched row | probe columns (src/JoinOps.h:77-125), at the item's offset.
-// The item's chains are first copied to LDS (dynamic shared memory), so chain
-// walks are LDS reads. CHAINS == false writes the single-match rows column by
-// column (every load of a column issued before its stores); CHAINS == true
-// derives the same layout and writes only the multi-match rows: each thread
-// walks its chains into an LDS batch of (probe row, build row) pairs, then
-// the workgroup writes the batch's rows at consecutive output positions.
+// The item's groups (perm, glen) are first copied to LDS. CHAINS == false
+// writes the single-match rows column by column (every load of a column
+// issued before its stores); CHAINS == true derives the same layout and
+// writes only the multi-match rows: each thread lays its groups out in an LDS
+// batch of (probe row, build row) pairs, then the workgroup writes the
+// batch's rows at consecutive output positions.
 template <int THREADS, int U, bool CHAINS>
 __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -409,13 +450,16 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     const uint32_t w = xcd_share(blockIdx.x, a.work_first[a.n_leaves]);
     if (!locate_item(a, w, it))
         return;
-    uint16_t *chains = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *perm = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *glen = perm + round_up16(a.lds_max_rows);
     uint16_t *hl = reinterpret_cast<uint16_t *>(smem + emit_chain_lds(a.lds_max_rows)); // [U][THREADS]
     uint32_t *ent = reinterpret_cast<uint32_t *>(smem + emit_chain_lds(a.lds_max_rows) + U * THREADS * 2);
     {
         const uint16_t *src = a.next16 + item_chain_base(a, it);
-        for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS)
-            chains[i] = src[i];
+        for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
+            perm[i] = src[i];
+            glen[i] = src[it.nr + i];
+        }
         __syncthreads();
     }
     uint64_t round_base = a.out_base + a.item_off[w];
@@ -430,18 +474,8 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             h[u] = j < it.s1 ? hv : kNoNext;
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            chainlen[u] = 0;
-            if (h[u] != kNoNext && (h[u] & kMore)) {
-                uint32_t c = 1;
-                for (uint16_t x = chains[h[u] & kRowMask];; x = chains[x & kRowMask]) {
-                    ++c;
-                    if (!(x & kMore))
-                        break;
-                }
-                chainlen[u] = c;
-            }
-        }
+        for (int u = 0; u < U; ++u)
+            chainlen[u] = (h[u] != kNoNext && (h[u] & kMore)) ? glen[h[u] & kRowMask] : 0u;
         RoundLayout<THREADS, U> L;
         round_layout<THREADS, U>(h, chainlen, wave_cnt[parity], tmp, L);
         if (L.total == 0)
@@ -470,27 +504,32 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
             gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
         } else if (L.chain_total > 0) {
-            // this thread's heads of multi-match rows, indexable at run time
+            // this thread's multi-match groups (start | kMore), indexable at run time
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 hl[u * THREADS + threadIdx.x] = chainlen[u] ? h[u] : kNoNext;
-            uint32_t cu = 0; // slot being walked (U: done)
+            uint32_t cu = 0; // slot being laid out (U: done)
             while (cu < U && hl[cu * THREADS + threadIdx.x] == kNoNext)
                 ++cu;
-            uint16_t cx = cu < U ? hl[cu * THREADS + threadIdx.x] : kNoNext;
+            uint32_t gs = 0, ge = 0; // rest of the current group: perm[gs, ge)
+            if (cu < U) {
+                gs = hl[cu * THREADS + threadIdx.x] & kRowMask;
+                ge = gs + glen[gs];
+            }
             uint32_t cpos = L.chain_pos - L.chain_start; // next entry of this thread in the round's chain rows
             for (uint32_t b0 = 0; b0 < L.chain_total; b0 += kEmitBatch) {
                 const uint32_t b1 = min(L.chain_total, b0 + kEmitBatch);
                 while (cu < U && cpos < b1) {
-                    ent[cpos - b0] = ((cu * THREADS + threadIdx.x) << 16) | (cx & kRowMask);
+                    ent[cpos - b0] = ((cu * THREADS + threadIdx.x) << 16) | perm[gs];
                     ++cpos;
-                    if (cx & kMore) {
-                        cx = chains[cx & kRowMask];
-                    } else {
+                    if (++gs == ge) {
                         ++cu;
                         while (cu < U && hl[cu * THREADS + threadIdx.x] == kNoNext)
                             ++cu;
-                        cx = cu < U ? hl[cu * THREADS + threadIdx.x] : kNoNext;
+                        if (cu < U) {
+                            gs = hl[cu * THREADS + threadIdx.x] & kRowMask;
+                            ge = gs + glen[gs];
+                        }
                     }
                 }
                 __syncthreads();
@@ -1102,8 +1141,9 @@ uint64_t leaf_max_work(uint64_t probe_rows, uint32_t n_leaves, uint32_t chunk_ro
 }
 
 size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct) {
+    // 16-byte cells, then next, gstart and glen_h (u16 per build row each)
     return static_cast<size_t>(table_cap(lds_max_rows, load_pct)) * sizeof(LdsCell) +
-           ((static_cast<size_t>(lds_max_rows) * 2 + 15) & ~size_t{15});
+           3 * static_cast<size_t>(round_up16(lds_max_rows)) * 2;
 }
 
 uint32_t leaf_join_lds_max_rows(uint32_t lds_bytes, uint32_t load_pct) {

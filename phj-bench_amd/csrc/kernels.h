@@ -101,8 +101,8 @@ struct LeafJoinArgs {
     uint32_t *item_count;       // probe pass: output rows per work item (max_work + 1 entries)
     const uint32_t *item_off;   // emit pass: exclusive scan of item_count
     uint16_t *heads;            // per probe row: matched leaf-local build row | chain flag
-    uint16_t *next16;           // per (work item, leaf-local build row): chain successor | chain flag
-    const uint32_t *chain_first; // n_leaves + 1: first next16 slot of each leaf's items (nr slots per item)
+    uint16_t *next16;           // per work item, 2 x leaf rows: each key's rows as one group (perm), group lengths (glen)
+    const uint32_t *chain_first; // n_leaves + 1: first item slot of each leaf's items (nr slots per item, 2 entries each)
     uint32_t *chain_flag;       // probe pass sets it when any probe row has several matches
     OutSink out;                // emit pass: rows land at out_base + item_off[item] + ...
     uint64_t out_base;
