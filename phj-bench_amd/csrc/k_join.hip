@@ -421,10 +421,8 @@ __device__ __forceinline__ void round_layout(const uint16_t (&h)[U], const uint3
     L.total = running + chain_total;
 }
 
-// Multi-match rows are written kEmitBatch at a time: (round-relative probe
-// row, leaf-local build row) pairs are laid out in LDS in output order, then
-// the workgroup writes them to consecutive output positions.
-constexpr uint32_t kEmitBatch = 4096;
+// Multi-match rows are written by the whole workgroup, kEmitVec rows per
+// thread per step, at consecutive output positions.
 constexpr int kEmitVec = 4;
 
 // LDS of the emit kernels for the item's groups: perm and glen, nr each.
@@ -438,9 +436,9 @@ __host__ __device__ constexpr size_t emit_chain_lds(uint32_t lds_max_rows) {
 // The item's groups (perm, glen) are first copied to LDS. CHAINS == false
 // writes the single-match rows column by column (every load of a column
 // issued before its stores); CHAINS == true derives the same layout and
-// writes only the multi-match rows: each thread lays its groups out in an LDS
-// batch of (probe row, build row) pairs, then the workgroup writes the
-// batch's rows at consecutive output positions.
+// writes only the multi-match rows: each thread publishes where its groups
+// land in the round's output, and every thread then takes consecutive output
+// rows, finding each row's group by binary search.
 template <int THREADS, int U, bool CHAINS>
 __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -452,8 +450,8 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
         return;
     uint16_t *perm = reinterpret_cast<uint16_t *>(smem);
     uint16_t *glen = perm + round_up16(a.lds_max_rows);
-    uint16_t *hl = reinterpret_cast<uint16_t *>(smem + emit_chain_lds(a.lds_max_rows)); // [U][THREADS]
-    uint32_t *ent = reinterpret_cast<uint32_t *>(smem + emit_chain_lds(a.lds_max_rows) + U * THREADS * 2);
+    uint32_t *seg_end = reinterpret_cast<uint32_t *>(smem + emit_chain_lds(a.lds_max_rows)); // [THREADS * U]
+    uint16_t *seg_gs = reinterpret_cast<uint16_t *>(smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 4);
     {
         const uint16_t *src = a.next16 + item_chain_base(a, it);
         for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
@@ -504,65 +502,48 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
             gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
         } else if (L.chain_total > 0) {
-            // this thread's multi-match groups (start | kMore), indexable at run time
+            // Segment s = thread * U + u holds the rows of slot u's group, in
+            // output order; seg_end[s] is its exclusive end (round-relative).
+            // Output row o belongs to the first segment ending after o.
+            uint32_t end = L.chain_pos - L.chain_start;
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                hl[u * THREADS + threadIdx.x] = chainlen[u] ? h[u] : kNoNext;
-            uint32_t cu = 0; // slot being laid out (U: done)
-            while (cu < U && hl[cu * THREADS + threadIdx.x] == kNoNext)
-                ++cu;
-            uint32_t gs = 0, ge = 0; // rest of the current group: perm[gs, ge)
-            if (cu < U) {
-                gs = hl[cu * THREADS + threadIdx.x] & kRowMask;
-                ge = gs + glen[gs];
+            for (int u = 0; u < U; ++u) {
+                end += chainlen[u];
+                seg_end[threadIdx.x * U + u] = end;
+                seg_gs[threadIdx.x * U + u] = h[u] & kRowMask;
             }
-            uint32_t cpos = L.chain_pos - L.chain_start; // next entry of this thread in the round's chain rows
-            for (uint32_t b0 = 0; b0 < L.chain_total; b0 += kEmitBatch) {
-                const uint32_t b1 = min(L.chain_total, b0 + kEmitBatch);
-                while (cu < U && cpos < b1) {
-                    ent[cpos - b0] = ((cu * THREADS + threadIdx.x) << 16) | perm[gs];
-                    ++cpos;
-                    if (++gs == ge) {
-                        ++cu;
-                        while (cu < U && hl[cu * THREADS + threadIdx.x] == kNoNext)
-                            ++cu;
-                        if (cu < U) {
-                            gs = hl[cu * THREADS + threadIdx.x] & kRowMask;
-                            ge = gs + glen[gs];
-                        }
-                    }
+            __syncthreads();
+            const uint64_t pos0 = round_base + L.chain_start;
+            for (uint32_t k0 = 0; k0 < L.chain_total; k0 += THREADS * kEmitVec) {
+                uint32_t ridx[kEmitVec], sidx[kEmitVec];
+                uint64_t opos[kEmitVec];
+                bool on[kEmitVec];
+#pragma unroll
+                for (int v = 0; v < kEmitVec; ++v) {
+                    const uint32_t o = k0 + v * THREADS + threadIdx.x;
+                    on[v] = o < L.chain_total;
+                    const uint32_t oc = on[v] ? o : L.chain_total - 1;
+                    uint32_t sg = 0;
+#pragma unroll
+                    for (uint32_t step = THREADS * U / 2; step > 0; step >>= 1)
+                        sg += seg_end[sg + step - 1] <= oc ? step : 0u;
+                    const uint32_t e = oc - (sg ? seg_end[sg - 1] : 0u);
+                    sidx[v] = base + (sg % U) * THREADS + sg / U;
+                    ridx[v] = it.r0 + perm[seg_gs[sg] + e];
+                    opos[v] = pos0 + o;
                 }
-                __syncthreads();
-                // kEmitVec rows per thread per step, column by column (every
-                // load of a column issued before its stores)
-                const uint64_t pos0 = round_base + L.chain_start + b0;
-                const uint32_t n = b1 - b0;
-                for (uint32_t k0 = 0; k0 < n; k0 += THREADS * kEmitVec) {
-                    uint32_t ridx[kEmitVec], sidx[kEmitVec];
-                    uint64_t opos[kEmitVec];
-                    bool on[kEmitVec];
+                uint64_t kv[kEmitVec];
 #pragma unroll
-                    for (int v = 0; v < kEmitVec; ++v) {
-                        const uint32_t k = k0 + v * THREADS + threadIdx.x;
-                        on[v] = k < n;
-                        const uint32_t e = ent[on[v] ? k : 0u];
-                        sidx[v] = base + (e >> 16);
-                        ridx[v] = it.r0 + (e & 0xFFFFu);
-                        opos[v] = pos0 + k;
-                    }
-                    uint64_t kv[kEmitVec];
+                for (int v = 0; v < kEmitVec; ++v)
+                    kv[v] = a.s_keys[sidx[v]];
 #pragma unroll
-                    for (int v = 0; v < kEmitVec; ++v)
-                        kv[v] = a.s_keys[sidx[v]];
-#pragma unroll
-                    for (int v = 0; v < kEmitVec; ++v)
-                        if (on[v])
-                            a.out.keys[opos[v]] = kv[v];
-                    gather_set<kEmitVec>(a.r_cols, a.out.left, ridx, opos, on);
-                    gather_set<kEmitVec>(a.s_cols, a.out.right, sidx, opos, on);
-                }
-                __syncthreads();
+                for (int v = 0; v < kEmitVec; ++v)
+                    if (on[v])
+                        a.out.keys[opos[v]] = kv[v];
+                gather_set<kEmitVec>(a.r_cols, a.out.left, ridx, opos, on);
+                gather_set<kEmitVec>(a.s_cols, a.out.right, sidx, opos, on);
             }
+            __syncthreads(); // seg_end and seg_gs are rewritten by the next round
         }
         round_base += L.total;
     }
@@ -1215,7 +1196,7 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
     }
     if (chains) {
         TimedScope ts(tm, "leaf_emit_chains", s);
-        const size_t lds = lds_chains + 8 * 256 * 2 + kEmitBatch * 4;
+        const size_t lds = lds_chains + 8 * 256 * (4 + 2);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<256, 8, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         k_leaf_emit<256, 8, true><<<grid8(max_work), 256, lds, s>>>(a);
