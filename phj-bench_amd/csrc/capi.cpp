@@ -400,12 +400,12 @@ uint32_t validate_pass_bits(const uint8_t *pass_bits, uint32_t n) {
 }
 
 // Device pass plan: the leaves depend only on the total bit count (the top
-// `total` hash bits, consumed top-down), so the device splits them into passes
-// of at most 8 bits, each as even as possible.
+// `total` hash bits, consumed top-down), so the device splits them into the
+// fewest passes of at most kDeviceMaxPassBits bits, each as even as possible.
 std::vector<uint32_t> device_passes(uint32_t total) {
     if (total > 24)
         throw PhjError(PHJ_ERR_UNSUPPORTED, "more than 24 partition bits (16M leaves) is not supported on the device");
-    const uint32_t n = (total + 7) / 8;
+    const uint32_t n = (total + kDeviceMaxPassBits - 1) / kDeviceMaxPassBits;
     std::vector<uint32_t> v;
     for (uint32_t p = 0; p < n; ++p)
         v.push_back(total / n + (p < total % n ? 1 : 0));
@@ -1305,7 +1305,7 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
         base[j] = s_off[j];
         end[j] = s_off[j + 1];
         jbits[j] = batch_bits(d[j].build.rows);
-        n_passes = std::max(n_passes, (jbits[j] + 7) / 8);
+        n_passes = std::max(n_passes, (jbits[j] + kDeviceMaxPassBits - 1) / kDeviceMaxPassBits);
     }
     R.rows = r_off[m];
     S.rows = s_off[m];
@@ -1313,7 +1313,7 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
     for (uint32_t j = 0; j < m; ++j) {
         uint32_t left = jbits[j];
         for (uint32_t p = 0; p < n_passes; ++p) {
-            bits[p][j] = std::min(left, 8u);
+            bits[p][j] = std::min(left, kDeviceMaxPassBits);
             left -= bits[p][j];
         }
     }

@@ -32,7 +32,7 @@ constexpr int kTile = kPartThreads * kPartRpt; // rows per tile
 // where 512 threads x 8 rows needed 146 VGPRs and ran one workgroup per CU.
 constexpr int kScatThreads = 1024;
 constexpr int kScatRpt = kTile / kScatThreads;
-constexpr int kMaxDigits = 256;                // at most 8 bits per device pass
+constexpr int kMaxDigits = 1 << kDeviceMaxPassBits; // digits of one device pass
 constexpr int kChunkTiles = 256;               // tiles per column-scan chunk
 constexpr int kScanThreads = 512;
 constexpr int kScanPer = 8;
@@ -320,7 +320,7 @@ template <bool NT, class T> __device__ __forceinline__ void st(T *p, const T &v)
 
 struct ScatterShared {
     uint64_t stage[kTile];
-    uint8_t slot_digit[kTile];
+    uint16_t slot_digit[kTile];
     uint32_t cursor[kMaxDigits];
     uint32_t lstart[kMaxDigits];
     uint32_t gbase[kMaxDigits];
@@ -386,7 +386,7 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
         sh.cursor[i] = 0;
     __syncthreads();
 
-    // Rank each row inside its digit with one LDS atomic (rank << 8 | digit).
+    // Rank each row inside its digit with one LDS atomic (rank << kDeviceMaxPassBits | digit).
     uint64_t key[kScatRpt];
     uint32_t slot[kScatRpt];
 #pragma unroll
@@ -400,7 +400,7 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
         slot[r] = kNoSlot;
         if (idx < L.len) {
             const uint32_t d = digit_of(int_hash64(key[r]), shift, bits);
-            slot[r] = (atomicAdd(&sh.cursor[d], 1u) << 8) | d;
+            slot[r] = (atomicAdd(&sh.cursor[d], 1u) << kDeviceMaxPassBits) | d;
         }
     }
     __syncthreads();
@@ -418,9 +418,9 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
 #pragma unroll
     for (int r = 0; r < kScatRpt; ++r) {
         if (slot[r] != kNoSlot) {
-            const uint32_t d = slot[r] & 0xFFu;
-            const uint32_t sl = sh.lstart[d] + (slot[r] >> 8);
-            sh.slot_digit[sl] = static_cast<uint8_t>(d);
+            const uint32_t d = slot[r] & (kMaxDigits - 1u);
+            const uint32_t sl = sh.lstart[d] + (slot[r] >> kDeviceMaxPassBits);
+            sh.slot_digit[sl] = static_cast<uint16_t>(d);
             slot[r] = sl;
         }
     }

@@ -41,6 +41,10 @@ void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
 // One partition pass: every input segment [seg_off[s], seg_off[s+1]) is split
 // by hash bits [shift, shift+bits) into 2^bits output segments (output segment
 // s*2^bits + d). Rows stay inside their segment's range.
+// Bits of one device radix pass: up to 512 partitions per pass (a 9-bit
+// pass costs about 1.2x an 8-bit one; a second pass costs 2x).
+constexpr uint32_t kDeviceMaxPassBits = 9;
+
 struct RadixPassScratch {
     uint32_t *seg_tile_first; // 2 * (n_seg + 1): first tile, then first scan chunk, per segment
     uint32_t *hist;           // radix_max_tiles * 2^bits
