@@ -317,6 +317,9 @@ class DeviceRelation:
     def __init__(self, ctx, crel: _Relation):
         self.ctx = ctx
         self._rel = crel
+        self._root = self  # views share their parent's schema and memory
+        self._first = 0    # first row within the root
+        self._rows = int(crel.rows)
 
     @property
     def rows(self):
@@ -350,6 +353,8 @@ class DeviceRelation:
             v.cols[i].data = self._rel.cols[i].data + TYPE_SIZE[t] * a
         rel = DeviceRelation(self.ctx, v)
         rel._view = True
+        rel._root = self._root
+        rel._first = self._first + a
         return rel
 
     def release(self):
@@ -378,24 +383,86 @@ class JoinOutput:
                              [c[a:b] for c in self.right], self.right_types)
 
 
+def _batch_descs(pairs, cfgs, tuning):
+    """phj_join_desc array for join_batch. When every build side is a view of
+    one device relation and every probe side a view of another (the
+    back-to-back layout of generate_batch), the descs are filled column-wise
+    with numpy from a template desc; otherwise one by one."""
+    n = len(pairs)
+    descs = (_JoinDesc * max(n, 1))()
+    if n == 0:
+        return descs
+    tmpl = _JoinDesc()
+    tmpl.scheme = SCHEME_PHJ_PURE
+    tmpl.threads = 1
+    for k, v in (tuning or {}).items():
+        setattr(tmpl.tuning, k, v)
+    b0, p0 = pairs[0]
+    tmpl.build = b0._c()
+    tmpl.probe = p0._c()
+    rb, rp = getattr(b0, "_root", None), getattr(p0, "_root", None)
+    views = rb is not None and rp is not None and \
+        all(getattr(b, "_root", None) is rb and getattr(p, "_root", None) is rp for b, p in pairs)
+    if not views or any(len(c.pass_bits) > MAX_PASSES for c in cfgs):
+        for i, (b, p) in enumerate(pairs):
+            d = descs[i]
+            C.memmove(C.byref(d), C.byref(tmpl), C.sizeof(_JoinDesc))
+            d.build = b._c()
+            d.probe = p._c()
+            cfg = cfgs[i]
+            d.n_passes = len(cfg.pass_bits)
+            for k, bits in enumerate(cfg.pass_bits):
+                d.pass_bits[k] = int(bits)
+        return descs
+    size = C.sizeof(_JoinDesc)
+    buf = np.frombuffer(descs, dtype=np.uint8).reshape(n, size)
+    buf[:] = np.frombuffer(C.string_at(C.byref(tmpl), size), dtype=np.uint8)
+
+    def field(off, dtype):
+        return np.ndarray((n,), dtype=dtype, buffer=descs, offset=off, strides=(size,))
+
+    for side, root in (("build", rb), ("probe", rp)):
+        base = getattr(_JoinDesc, side).offset
+        views = [b if side == "build" else p for b, p in pairs]
+        first = np.array([v._first for v in views], dtype=np.uint64)
+        field(base + _Relation.rows.offset, np.uint64)[:] = [v._rows for v in views]
+        r = root._rel
+        field(base + _Relation.keys.offset, np.uint64)[:] = np.uint64(r.keys or 0) + np.uint64(8) * first
+        for c in range(r.n_cols):
+            off = base + _Relation.cols.offset + c * C.sizeof(_Column) + _Column.data.offset
+            width = np.uint64(TYPE_SIZE[int(r.cols[c].type)])
+            field(off, np.uint64)[:] = np.uint64(r.cols[c].data or 0) + width * first
+    field(_JoinDesc.n_passes.offset, np.uint32)[:] = [len(c.pass_bits) for c in cfgs]
+    pb = np.zeros((n, MAX_PASSES), dtype=np.uint8)
+    for i, c in enumerate(cfgs):
+        pb[i, :len(c.pass_bits)] = c.pass_bits
+    buf[:, _JoinDesc.pass_bits.offset:_JoinDesc.pass_bits.offset + MAX_PASSES] = pb
+    return descs
+
+
 class JoinResult:
     """ChjResult / PhjResult / PhjBepResult with the output left in HBM."""
+
+    _PHASES = ("build_shuffle", "build", "probe_shuffle", "probe", "eviction_overhead")
+    _SCALARS = ("e2e_wall_ms", "peak_mem_bytes", "bep_budget_mib", "bep_evictions", "bep_refinements",
+                "bep_peak_bytes", "bep_build_skip_retries")
 
     def __init__(self, ctx, res: _JoinResult):
         self.ctx = ctx
         self._res = res
-        self.build_shuffle = PhaseTiming(res.build_shuffle.wall_ms, res.build_shuffle.ns_per_row)
-        self.build = PhaseTiming(res.build.wall_ms, res.build.ns_per_row)
-        self.probe_shuffle = PhaseTiming(res.probe_shuffle.wall_ms, res.probe_shuffle.ns_per_row)
-        self.probe = PhaseTiming(res.probe.wall_ms, res.probe.ns_per_row)
-        self.eviction_overhead = PhaseTiming(res.eviction_overhead.wall_ms, res.eviction_overhead.ns_per_row)
-        self.e2e_wall_ms = res.e2e_wall_ms
-        self.peak_mem_bytes = res.peak_mem_bytes
-        self.bep_budget_mib = res.bep_budget_mib
-        self.bep_evictions = res.bep_evictions
-        self.bep_refinements = res.bep_refinements
-        self.bep_peak_bytes = res.bep_peak_bytes
-        self.bep_build_skip_retries = res.bep_build_skip_retries
+
+    def __getattr__(self, name):
+        # phase timings and counters are read from the C struct on first use
+        # (a batch returns thousands of results)
+        if name in JoinResult._PHASES:
+            t = getattr(self._res, name)
+            v = PhaseTiming(t.wall_ms, t.ns_per_row)
+        elif name in JoinResult._SCALARS:
+            v = getattr(self._res, name)
+        else:
+            raise AttributeError(name)
+        self.__dict__[name] = v
+        return v
 
     @property
     def out_rows(self):
@@ -651,27 +718,13 @@ class Context:
         """Batched PHJ-PURE joins (phj_join_batch): one JoinResult per (build, probe)
         pair, each equal to runPHJ on that pair."""
         n = len(pairs)
-        descs = (_JoinDesc * max(n, 1))()
-        for i, (b, p) in enumerate(pairs):
-            d = descs[i]
-            d.scheme = SCHEME_PHJ_PURE
-            d.threads = 1
-            d.build = b._c()
-            d.probe = p._c()
-            cfg = cfgs[i] if cfgs is not None else RadixConfig.for_build_rows(b.rows)
-            d.n_passes = len(cfg.pass_bits)
-            for k, bits in enumerate(cfg.pass_bits):
-                d.pass_bits[k] = int(bits)
-            for k, v in (tuning or {}).items():
-                setattr(d.tuning, k, v)
+        if cfgs is None:
+            cfgs = [RadixConfig.for_build_rows(b.rows) for b, _ in pairs]
+        descs = _batch_descs(pairs, cfgs, tuning)
         res = (_JoinResult * max(n, 1))()
         _raise(self._p, load_library().phj_join_batch(self._p, n, descs, res))
-        out = []
-        for i in range(n):
-            r = _JoinResult()
-            C.memmove(C.byref(r), C.byref(res[i]), C.sizeof(_JoinResult))
-            out.append(JoinResult(self, r))
-        return out
+        # each result is a view of its element of the array (which it keeps alive)
+        return [JoinResult(self, res[i]) for i in range(n)]
 
     def runCHJ(self, build, probe, threads=1, **kw) -> JoinResult:
         return self.join(SCHEME_CHJ, build, probe, None, threads, **kw)
