@@ -424,10 +424,73 @@ __device__ __forceinline__ void round_layout(const uint16_t (&h)[U], const uint3
 // Multi-match rows are written by the whole workgroup, kEmitVec rows per
 // thread per step, at consecutive output positions.
 constexpr int kEmitVec = 4;
+// LDS for the staged build columns of the multi-match emit (beyond it they
+// are gathered from HBM).
+constexpr size_t kEmitRcolLds = 64 * 1024;
 
 // LDS of the emit kernels for the item's groups: perm and glen, nr each.
 __host__ __device__ constexpr size_t emit_chain_lds(uint32_t lds_max_rows) {
     return 2 * static_cast<size_t>(round_up16(lds_max_rows)) * 2;
+}
+
+// Build columns of a leaf staged in LDS, one after another: column c takes
+// lds_max_rows elements of its width, 16-byte aligned.
+__host__ __device__ constexpr size_t emit_rcol_bytes(uint32_t lds_max_rows, uint32_t width) {
+    return (static_cast<size_t>(lds_max_rows) * width + 15) & ~size_t{15};
+}
+
+// Copies rows [r0, r0 + nr) of every build column to LDS (workgroup-wide).
+template <int THREADS>
+__device__ __forceinline__ void stage_rcols(const ColSet &rc, uint32_t r0, uint32_t nr, uint32_t lds_max_rows,
+                                            unsigned char *dst) {
+    size_t off = 0;
+    for (uint32_t c = 0; c < rc.n; ++c) {
+        const uint32_t t = col_type(rc, c);
+        const void *src = col_ptr(rc, c);
+        unsigned char *d = dst + off;
+        off += emit_rcol_bytes(lds_max_rows, type_size(t));
+        switch (t) {
+        case 0:
+            for (uint32_t i = threadIdx.x; i < nr; i += THREADS)
+                reinterpret_cast<uint8_t *>(d)[i] = static_cast<const uint8_t *>(src)[r0 + i];
+            break;
+        case 1:
+            for (uint32_t i = threadIdx.x; i < nr; i += THREADS)
+                reinterpret_cast<uint16_t *>(d)[i] = static_cast<const uint16_t *>(src)[r0 + i];
+            break;
+        case 2:
+            for (uint32_t i = threadIdx.x; i < nr; i += THREADS)
+                reinterpret_cast<uint32_t *>(d)[i] = static_cast<const uint32_t *>(src)[r0 + i];
+            break;
+        case 3:
+            for (uint32_t i = threadIdx.x; i < nr; i += THREADS)
+                reinterpret_cast<uint64_t *>(d)[i] = static_cast<const uint64_t *>(src)[r0 + i];
+            break;
+        default:
+            for (uint32_t i = threadIdx.x; i < 2 * nr; i += THREADS)
+                reinterpret_cast<uint64_t *>(d)[i] = static_cast<const uint64_t *>(src)[2 * static_cast<uint64_t>(r0) + i];
+            break;
+        }
+    }
+}
+
+// gather_set with the build columns read from their LDS copy (idx leaf-local).
+template <int U>
+__device__ __forceinline__ void gather_set_lds(const unsigned char *base, uint32_t lds_max_rows, const ColSet &in,
+                                               const OutColSet &out, const uint32_t (&idx)[U], const uint64_t (&pos)[U],
+                                               const bool (&on)[U]) {
+    size_t off = 0;
+    for (uint32_t c = 0; c < in.n; ++c) {
+        const unsigned char *src = base + off;
+        off += emit_rcol_bytes(lds_max_rows, type_size(col_type(in, c)));
+        switch (col_type(in, c)) {
+        case 0: gather_col<U>(reinterpret_cast<const uint8_t *>(src), optr<uint8_t>(out, c), idx, pos, on); break;
+        case 1: gather_col<U>(reinterpret_cast<const uint16_t *>(src), optr<uint16_t>(out, c), idx, pos, on); break;
+        case 2: gather_col<U>(reinterpret_cast<const uint32_t *>(src), optr<uint32_t>(out, c), idx, pos, on); break;
+        case 3: gather_col<U>(reinterpret_cast<const uint64_t *>(src), optr<uint64_t>(out, c), idx, pos, on); break;
+        default: gather_col128<U>(reinterpret_cast<const uint64_t *>(src), optr<uint64_t>(out, c), idx, pos, on); break;
+        }
+    }
 }
 
 // Emit pass: a streaming kernel over the probe rows of each work item using
@@ -452,12 +515,15 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     uint16_t *glen = perm + round_up16(a.lds_max_rows);
     uint32_t *seg_end = reinterpret_cast<uint32_t *>(smem + emit_chain_lds(a.lds_max_rows)); // [THREADS * U]
     uint16_t *seg_gs = reinterpret_cast<uint16_t *>(smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 4);
+    unsigned char *rlds = smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 6; // staged build columns
     {
         const uint16_t *src = a.next16 + item_chain_base(a, it);
         for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
             perm[i] = src[i];
             glen[i] = src[it.nr + i];
         }
+        if (CHAINS && a.r_lds)
+            stage_rcols<THREADS>(a.r_cols, it.r0, it.nr, a.lds_max_rows, rlds);
         __syncthreads();
     }
     uint64_t round_base = a.out_base + a.item_off[w];
@@ -529,7 +595,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                         sg += seg_end[sg + step - 1] <= oc ? step : 0u;
                     const uint32_t e = oc - (sg ? seg_end[sg - 1] : 0u);
                     sidx[v] = base + (sg % U) * THREADS + sg / U;
-                    ridx[v] = it.r0 + perm[seg_gs[sg] + e];
+                    ridx[v] = perm[seg_gs[sg] + e]; // leaf-local
                     opos[v] = pos0 + o;
                 }
                 uint64_t kv[kEmitVec];
@@ -540,7 +606,14 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                 for (int v = 0; v < kEmitVec; ++v)
                     if (on[v])
                         a.out.keys[opos[v]] = kv[v];
-                gather_set<kEmitVec>(a.r_cols, a.out.left, ridx, opos, on);
+                if (a.r_lds) {
+                    gather_set_lds<kEmitVec>(rlds, a.lds_max_rows, a.r_cols, a.out.left, ridx, opos, on);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < kEmitVec; ++v)
+                        ridx[v] += it.r0;
+                    gather_set<kEmitVec>(a.r_cols, a.out.left, ridx, opos, on);
+                }
                 gather_set<kEmitVec>(a.s_cols, a.out.right, sidx, opos, on);
             }
             __syncthreads(); // seg_end and seg_gs are rewritten by the next round
@@ -1196,10 +1269,17 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
     }
     if (chains) {
         TimedScope ts(tm, "leaf_emit_chains", s);
-        const size_t lds = lds_chains + 8 * 256 * (4 + 2);
+        // Build columns go to LDS when they fit beside the groups: the chain
+        // rows then gather them from LDS instead of from random HBM lines.
+        size_t rbytes = 0;
+        for (uint32_t c = 0; c < a.r_cols.n; ++c)
+            rbytes += emit_rcol_bytes(a.lds_max_rows, type_size(a.r_cols.types[c]));
+        LeafJoinArgs b = a;
+        b.r_lds = a.r_cols.n > 0 && rbytes <= kEmitRcolLds ? 1u : 0u;
+        const size_t lds = lds_chains + 8 * 256 * (4 + 2) + (b.r_lds ? rbytes : 0);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<256, 8, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        k_leaf_emit<256, 8, true><<<grid8(max_work), 256, lds, s>>>(a);
+        k_leaf_emit<256, 8, true><<<grid8(max_work), 256, lds, s>>>(b);
     }
 }
 

@@ -110,6 +110,7 @@ struct LeafJoinArgs {
     uint32_t *chain_flag;       // probe pass sets it when any probe row has several matches
     OutSink out;                // emit pass: rows land at out_base + item_off[item] + ...
     uint64_t out_base;
+    uint32_t r_lds;             // multi-match emit: nonzero when the leaf's build columns are staged in LDS
     // Batched joins (fused path): leaf -> join, and per join an output cursor,
     // the end of its output region (capacity) and a repeated-key flag. When
     // leaf_join is null the fused join uses out.cursor, out.cap, chain_flag.
