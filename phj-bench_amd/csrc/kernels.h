@@ -104,7 +104,7 @@ struct LeafJoinArgs {
     const uint32_t *item_leaf;  // leaf of each work item
     uint32_t *item_count;       // probe pass: output rows per work item (max_work + 1 entries)
     const uint32_t *item_off;   // emit pass: exclusive scan of item_count
-    uint16_t *heads;            // per probe row: matched leaf-local build row | chain flag
+    uint16_t *heads;            // per probe row: the matched leaf-local build row, or its group's start | kMore
     uint16_t *next16;           // per work item, 2 x leaf rows: each key's rows as one group (perm), group lengths (glen)
     const uint32_t *chain_first; // n_leaves + 1: first item slot of each leaf's items (nr slots per item, 2 entries each)
     uint32_t *chain_flag;       // probe pass sets it when any probe row has several matches
@@ -125,7 +125,7 @@ void leaf_work_plan(hipStream_t s, const LeafJoinArgs &a, uint32_t *work_first, 
                     KernelTimer *tm);
 uint64_t leaf_max_work(uint64_t probe_rows, uint32_t n_leaves, uint32_t chunk_rows);
 // chain_first (n_leaves + 1 entries) from the work plan: each work item of a
-// leaf exports its own chains (nr slots), scanned over leaves.
+// leaf exports its own row groups (nr slots of two entries), scanned over leaves.
 void leaf_chain_slots(hipStream_t s, const LeafJoinArgs &a, uint32_t *chain_first, uint32_t *scan_tmp);
 // Probe pass: LDS tables, heads/next16, item_count. Emit pass: the rows.
 void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_t threads, KernelTimer *tm);
