@@ -925,11 +925,28 @@ void phj_pure_probe(phj_ctx *ctx, const Partitioned &RP, const ColSet &r_cols, c
     ResultBuilder{ctx, res, RP.rows, S.rows}.finish(out, n);
 }
 
+// Leaf bits of a PHJ-PURE join. The output does not depend on the leaves, so
+// a configuration whose leaves would average more than kJoinLeafRows build
+// rows (too large for an LDS table) is refined to that size, up to the device
+// maximum; {4,4} on 2^27 build rows then joins like {8,8}. An explicit LDS
+// leaf limit or forced HBM tables keep the configuration as given.
+constexpr uint64_t kJoinLeafRows = 2048;
+constexpr uint32_t kDeviceMaxTotalBits = 24;
+
+uint32_t join_total_bits(uint32_t requested, uint64_t build_rows, const phj_tuning &tun) {
+    if (tun.lds_leaf_rows != 0 || (tun.flags & PHJ_TUNE_FORCE_GLOBAL_HT) != 0)
+        return requested;
+    uint32_t b = requested;
+    while (b < kDeviceMaxTotalBits && (build_rows >> b) > kJoinLeafRows)
+        ++b;
+    return b;
+}
+
 void run_phj_pure(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, const uint8_t *pb, uint32_t np,
                   const phj_tuning &tun, phj_join_result *res) {
     validate_rel(&b);
     validate_rel(&p);
-    const uint32_t total = validate_pass_bits(pb, np);
+    const uint32_t total = join_total_bits(validate_pass_bits(pb, np), b.rows, tun);
     const std::vector<uint32_t> passes = device_passes(total);
     std::memset(res, 0, sizeof(*res));
     Scratch scr(ctx);
@@ -1024,7 +1041,7 @@ namespace {
 void register_build(phj_ctx *ctx, const phj_relation &b, const uint8_t *pb, uint32_t np, const phj_tuning &tun,
                     phj_resident *r) {
     validate_rel(&b);
-    const uint32_t total = validate_pass_bits(pb, np);
+    const uint32_t total = join_total_bits(validate_pass_bits(pb, np), b.rows, tun);
     r->owner = ctx;
     r->passes = device_passes(total);
     Scratch scr(ctx);

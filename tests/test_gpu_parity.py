@@ -269,8 +269,10 @@ def test_chains_across_work_items(ctx, chunk):
                        build_types=(G.U16, G.U64), probe_types=(G.U32,))
     ob, op = O.gen(spec)
     want = O.serialize_sorted(O.reference_join(ob, op))
-    for bits in ([4], [2, 3]):  # leaves of about 3,800 and 1,900 build rows, in LDS
-        res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(bits), tuning=dict(probe_chunk_rows=chunk))
+    for bits in ([4], [2, 3]):  # leaves of about 3,800 and 1,900 build rows, in LDS (an explicit LDS limit
+        # keeps the configuration as given)
+        res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(bits),
+                         tuning=dict(probe_chunk_rows=chunk, lds_leaf_rows=4096))
         assert res.out_rows == want.shape[0]
         assert np.array_equal(sorted_rows_of(res.to_host()), want)
         res.release()
@@ -281,8 +283,9 @@ def test_long_chains_in_emit_batches(ctx, bits):
     """300 keys repeated 40 times each, every probe row matching: a round of
     2,048 probe rows yields ~80,000 multi-match output rows, far more than one
     LDS batch of the cooperative chain emission, and chains of 40 rows run
-    across batch boundaries. Leaves of ~6,000 build rows ([1]) go to HBM
-    tables; ~3,000 and ~1,500 ([2], [3]) stay in LDS."""
+    across batch boundaries. With an LDS leaf limit of 2,048 rows (which
+    keeps the configuration as given), leaves of ~6,000 and ~3,000 build rows
+    ([1], [2]) go to HBM tables and ~1,500 ([3]) stay in LDS."""
     rng = np.random.default_rng(77)
     distinct = rng.permutation(np.unique(rng.integers(1, 2**63, 400, dtype=np.uint64))[:300])
     bkeys = rng.permutation(np.repeat(distinct, 40))
@@ -293,10 +296,27 @@ def test_long_chains_in_emit_batches(ctx, bits):
     op = O.Relation(pkeys, [pcol], [G.U128])
     want = O.serialize_sorted(O.reference_join(ob, op))
     assert want.shape[0] == 20000 * 40
-    res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(bits))
+    res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(bits), tuning=dict(lds_leaf_rows=2048))
     assert res.out_rows == want.shape[0]
     assert np.array_equal(sorted_rows_of(res.to_host()), want)
     res.release()
+
+
+@pytest.mark.parametrize("bits", [[1], [2, 2], [8, 8]])
+def test_coarse_radix_config_refined(ctx, bits):
+    """A configuration with leaves too large for LDS joins through LDS-sized
+    leaves (the output does not depend on them): same rows as the oracle and
+    as the configuration kept exactly (HBM leaf tables)."""
+    spec = O.make_spec(seed=31, build_rows=60000, probe_rows=400000, distribution=0, match_rate=0.6,
+                       build_types=(G.U32, G.U64), probe_types=(G.U64, G.U128))
+    ob, op = O.gen(spec)
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    refined = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(bits))
+    exact = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(bits), tuning=dict(lds_leaf_rows=60000))
+    assert np.array_equal(sorted_rows_of(refined.to_host()), want)
+    assert refined.checksum() == exact.checksum() == O.checksum_rows(want)
+    refined.release()
+    exact.release()
 
 
 def test_join_rejects_bad_radix(ctx):
