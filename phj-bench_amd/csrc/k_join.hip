@@ -1096,6 +1096,14 @@ __device__ __forceinline__ uint32_t ht_lookup(const HtArgs &a, uint64_t k, uint3
     }
 }
 
+// Probe + emit against HBM tables (CHJ, oversize leaves). Per round of
+// THREADS*U probe rows: all U lookups of a thread advance in lock step (one
+// probe step each per iteration, their cell loads in flight together), chain
+// lengths likewise, then one output reservation per workgroup. Rows with one
+// match are written column by column with every load of a column issued
+// before its stores; rows with several matches (repeated build keys) walk
+// their chain row by row. Output row = probe key | build columns | probe
+// columns (src/JoinOps.h:77-125).
 template <int THREADS, int U>
 __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, ColSet sc, OutSink out) {
     __shared__ uint32_t tmp[THREADS / 64 + 1];
@@ -1103,29 +1111,74 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
     const uint64_t n = ht_total_rows(a);
     const uint64_t per_round = static_cast<uint64_t>(gridDim.x) * THREADS * U;
     for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * THREADS * U; base < n; base += per_round) {
-        uint64_t key[U];
-        uint64_t prow[U];
-        uint32_t head[U];
-        uint32_t cnt = 0;
+        uint64_t key[U], prow[U], cb[U], cm[U], cp[U];
+        uint32_t head[U], len[U];
+        uint32_t pend = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint64_t g = base + static_cast<uint64_t>(u) * THREADS + threadIdx.x;
+            uint64_t row;
+            uint32_t table, leaf;
+            ht_locate(a, min(g, n - 1), row, table, leaf); // rows past the end are masked below
+            prow[u] = row;
+            key[u] = a.rows.keys[row];
+            ht_home(a, int_hash64(key[u]), table, cb[u], cm[u], cp[u]);
             head[u] = kInvalidRow;
-            key[u] = 0;
-            prow[u] = 0;
             if (g < n) {
-                uint64_t row;
-                uint32_t table, leaf;
-                ht_locate(a, g, row, table, leaf);
-                const uint64_t k = a.rows.keys[row];
-                key[u] = k;
-                prow[u] = row;
-                const uint32_t hd = ht_lookup(a, k, table);
-                head[u] = hd;
-                for (uint32_t x = hd; x != kInvalidRow; x = a.ht.next[x])
-                    ++cnt;
+                if (key[u] == kEmptyKey)
+                    head[u] = a.ht.side_head[table];
+                else
+                    pend |= 1u << u;
             }
         }
+        while (pend) {
+            Cell16 c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                c[u] = a.ht.cells[cb[u] + cp[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!((pend >> u) & 1u))
+                    continue;
+                if (c[u].key == key[u]) {
+                    head[u] = ref_row(a, static_cast<uint64_t>(c[u].block_no) | (static_cast<uint64_t>(c[u].row_no) << 32));
+                    pend &= ~(1u << u);
+                } else if (c[u].key == kEmptyKey) {
+                    pend &= ~(1u << u);
+                } else {
+                    cp[u] = (cp[u] + 1) & cm[u];
+                }
+            }
+        }
+        // chain lengths, all chains of the thread one step per iteration
+        uint32_t x[U];
+        uint32_t walking = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            x[u] = head[u];
+            len[u] = 0;
+            if (x[u] != kInvalidRow)
+                walking |= 1u << u;
+        }
+        while (walking) {
+            uint32_t nx[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                nx[u] = a.ht.next[(walking >> u) & 1u ? x[u] : 0u];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!((walking >> u) & 1u))
+                    continue;
+                ++len[u];
+                x[u] = nx[u];
+                if (x[u] == kInvalidRow)
+                    walking &= ~(1u << u);
+            }
+        }
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            cnt += len[u];
         uint32_t round_total;
         const uint32_t off = block_excl_scan<THREADS>(cnt, tmp, &round_total);
         if (round_total == 0)
@@ -1133,11 +1186,33 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
         if (threadIdx.x == 0)
             out_base = atomicAdd(out.cursor, static_cast<unsigned long long>(round_total));
         __syncthreads();
+        uint64_t opos[U];
+        uint32_t ridx[U], sidx[U];
+        bool one[U];
         uint64_t pos = out_base + off;
 #pragma unroll
+        for (int u = 0; u < U; ++u) {
+            opos[u] = pos;
+            pos += len[u];
+            one[u] = len[u] == 1 && opos[u] < out.cap;
+            ridx[u] = one[u] ? head[u] : 0u;
+            sidx[u] = static_cast<uint32_t>(prow[u]);
+        }
+#pragma unroll
         for (int u = 0; u < U; ++u)
-            for (uint32_t x = head[u]; x != kInvalidRow; x = a.ht.next[x])
-                emit_row(out, pos++, key[u], rc, x, sc, prow[u]);
+            if (one[u])
+                out.keys[opos[u]] = key[u];
+        gather_set<U>(rc, out.left, ridx, opos, one);
+        gather_set<U>(sc, out.right, sidx, opos, one);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (len[u] < 2)
+                continue;
+            uint64_t p = opos[u];
+            for (uint32_t r = head[u]; r != kInvalidRow; r = a.ht.next[r])
+                emit_row(out, p++, key[u], rc, r, sc, prow[u]);
+        }
+        __syncthreads(); // out_base is rewritten by the next round
     }
 }
 
