@@ -1397,8 +1397,8 @@ void ht_probe_emit(hipStream_t s, const GlobalHT &ht, HtMode mode, const HtRows 
     if (max_rows == 0)
         return;
     TimedScope ts(tm, name, s);
-    constexpr int T = 256, U = 4;
-    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((max_rows + T * U - 1) / (T * U), 256 * 16));
+    constexpr int T = 256, U = 2; // measured on 2^30 CHJ probes: faster than U = 4 or 8
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((max_rows + T * U - 1) / (T * U), 256 * 64));
     k_ht_probe_emit<T, U><<<grid, T, 0, s>>>(make_args(ht, mode, probe), r_cols, s_cols, out);
 }
 
