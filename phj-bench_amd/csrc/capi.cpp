@@ -670,14 +670,18 @@ public:
     // bound()) and returns how many.
     uint64_t emit(const OutBuffers &out, uint64_t base) {
         if (fused_) {
-            unsigned long long *cursor = scr_.get<unsigned long long>(1);
-            launch_fill_u64(ctx_->stream, reinterpret_cast<uint64_t *>(cursor), 1, base);
-            a_.out = sink_of(out, cursor);
-            a_.chain_flag = scr_.get<uint32_t>(1);
-            launch_fill_u32(ctx_->stream, a_.chain_flag, 1, 0);
+            // cursor (word 0) and repeated-key flag (word 1) read back together
+            unsigned long long *cf = scr_.get<unsigned long long>(2);
+            launch_fill_u64(ctx_->stream, reinterpret_cast<uint64_t *>(cf), 1, base);
+            launch_fill_u64(ctx_->stream, reinterpret_cast<uint64_t *>(cf) + 1, 1, 0);
+            a_.out = sink_of(out, cf);
+            a_.chain_flag = reinterpret_cast<uint32_t *>(cf + 1);
             leaf_join_fused(ctx_->stream, a_, max_work_, ctx_->tm());
-            const uint64_t rows = read_scalar(ctx_, reinterpret_cast<const uint64_t *>(cursor)) - base;
-            if (read_scalar(ctx_, a_.chain_flag) == 0)
+            uint64_t h[2];
+            HIPCHK(hipMemcpyAsync(h, cf, sizeof(h), hipMemcpyDeviceToHost, ctx_->stream));
+            ctx_->sync();
+            const uint64_t rows = h[0] - base;
+            if (h[1] == 0)
                 return rows;
             fused_ = false;
             plan_two_phase();
