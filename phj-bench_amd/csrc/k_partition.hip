@@ -161,6 +161,33 @@ __global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restric
     }
 }
 
+// Exclusive scan of column d of rows [r0, r1) of a row-major table with B
+// columns, in place, starting from `run`; returns the column total. Loads are
+// issued kColUnroll rows at a time, so a column walk has that many reads in
+// flight instead of one.
+constexpr int kColUnroll = 8;
+__device__ __forceinline__ uint32_t excl_scan_column(uint32_t *__restrict__ t, uint32_t r0, uint32_t r1, uint32_t B,
+                                                     uint32_t d, uint32_t run) {
+    uint32_t r = r0;
+    for (; r + kColUnroll <= r1; r += kColUnroll) {
+        uint32_t v[kColUnroll];
+#pragma unroll
+        for (int i = 0; i < kColUnroll; ++i)
+            v[i] = t[(r + i) * B + d];
+#pragma unroll
+        for (int i = 0; i < kColUnroll; ++i) {
+            t[(r + i) * B + d] = run;
+            run += v[i];
+        }
+    }
+    for (; r < r1; ++r) {
+        const uint32_t v = t[r * B + d];
+        t[r * B + d] = run;
+        run += v;
+    }
+    return run;
+}
+
 // Exclusive column scan of the tile rows of one chunk (within one segment);
 // the chunk's per-digit totals go to chunk_tot[chunk * B + d].
 __global__ __launch_bounds__(kMaxDigits) void k_colscan_chunk(const uint32_t *__restrict__ seg_first, uint32_t n_seg,
@@ -178,13 +205,7 @@ __global__ __launch_bounds__(kMaxDigits) void k_colscan_chunk(const uint32_t *__
     const uint32_t d = threadIdx.x;
     if (d >= B)
         return;
-    uint32_t run = 0;
-    for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t v = hist[t * B + d];
-        hist[t * B + d] = run;
-        run += v;
-    }
-    chunk_tot[c * B + d] = run;
+    chunk_tot[c * B + d] = excl_scan_column(hist, t0, t1, B, d, 0u);
 }
 
 // Per segment: turn chunk totals into chunk bases (absolute row positions),
@@ -202,18 +223,21 @@ __global__ __launch_bounds__(kMaxDigits) void k_colscan_seg(const uint32_t *__re
     const uint32_t Bs = 1u << bits_s; // digits of this segment (the rest count zero)
     const uint32_t d = threadIdx.x;
     const uint32_t c0 = chunk_first[s], c1 = chunk_first[s + 1];
-    uint32_t run = 0;
-    if (d < B) {
-        for (uint32_t c = c0; c < c1; ++c) {
-            const uint32_t v = chunk_tot[c * B + d];
-            chunk_tot[c * B + d] = run;
-            run += v;
-        }
-    }
+    const uint32_t run = d < B ? excl_scan_column(chunk_tot, c0, c1, B, d, 0u) : 0u;
     uint32_t total;
-    const uint32_t base = seg_off[s] + block_excl_scan<kMaxDigits>(d < B ? run : 0u, tmp, &total);
+    const uint32_t base = seg_off[s] + block_excl_scan<kMaxDigits>(run, tmp, &total);
     if (d < B) {
-        for (uint32_t c = c0; c < c1; ++c)
+        uint32_t c = c0;
+        for (; c + kColUnroll <= c1; c += kColUnroll) {
+            uint32_t v[kColUnroll];
+#pragma unroll
+            for (int i = 0; i < kColUnroll; ++i)
+                v[i] = chunk_tot[(c + i) * B + d];
+#pragma unroll
+            for (int i = 0; i < kColUnroll; ++i)
+                chunk_tot[(c + i) * B + d] = v[i] + base;
+        }
+        for (; c < c1; ++c)
             chunk_tot[c * B + d] += base;
     }
     if (d < Bs)
