@@ -4,8 +4,9 @@
 // Scheme orchestration follows the reference's phase structure:
 //   runPHJ    (src/PHJ.cpp:19-150):    build shuffle -> probe shuffle -> per-leaf build+probe
 //   runCHJ    (src/CHJ.cpp:16-119):    sub-table build -> probe
-//   runPhjBep (src/PHJBep.cpp:113-790): build shuffle -> bounded-budget probe partitioning
-//                                       with per-leaf join as buffers fill -> final drain
+//   runPhjBep (src/PHJBep.cpp:113-790): build shuffle -> probe side in slices whose two
+//                                       pass buffers fit the budget, each slice
+//                                       partitioned to the leaves and joined at once
 // Every kernel is launched on the context's one HIP stream.
 #include <hip/hip_runtime.h>
 
@@ -614,10 +615,13 @@ uint32_t radix_flags(const phj_tuning &t) {
     return ((t.flags & PHJ_TUNE_CACHED_LOADS) ? 0u : kRadixNtLoads) | ((t.flags & PHJ_TUNE_NT_STORES) ? kRadixNtStores : 0u);
 }
 
-// Joins partitioned R and S leaf by leaf in two phases: construction counts
-// the exact output rows (LDS leaves by a count pass, oversize leaves through
-// HBM tables probed in count-only mode); emit() then writes them at a given
-// base row of an output buffer.
+// Joins partitioned R and S leaf by leaf. When every leaf fits LDS the fused
+// join runs first: bound() is then |S| and emit() writes the rows in one
+// launch, unless a build key repeats, in which case emit() returns kRetry
+// after setting up the two-phase plan. The two-phase plan counts the exact
+// output rows at construction (LDS leaves by a count pass, oversize leaves
+// through HBM tables probed in count-only mode); emit() then writes them from
+// a given base row of an output buffer.
 class LeafJoin {
 public:
     LeafJoin(phj_ctx *ctx, const Partitioned &R, const Partitioned &S, const LeafJoinPlan &plan)

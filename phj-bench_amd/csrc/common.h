@@ -49,19 +49,6 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Lanes (among `active`) whose low `bits` bits of v equal the caller's.
-__device__ __forceinline__ uint64_t match_any(uint32_t v, int bits, uint64_t active) {
-    uint64_t m = active;
-#pragma unroll
-    for (int b = 0; b < 16; ++b) {
-        if (b < bits) {
-            const uint64_t bal = __ballot((v >> b) & 1u);
-            m &= ((v >> b) & 1u) ? bal : ~bal;
-        }
-    }
-    return m;
-}
-
 // Inclusive wave scan of a 32-bit value (64 lanes).
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const uint32_t lane = lane_id();
@@ -122,21 +109,17 @@ __device__ __forceinline__ uint32_t upper_bound_idx(const T *a, uint32_t n, T x)
     return lo;
 }
 
-// Block index remapped so that consecutive logical indices run on one XCD
-// (workgroups are dispatched round-robin over the 8 XCDs of an MI355X).
-// `grid` must be a multiple of 8 (see grid8) so the map is a bijection.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t grid) {
-    const uint32_t per = grid / 8;
-    return (b % 8) * per + b / 8;
-}
-
+// Grid size rounded up to a multiple of the 8 XCDs of an MI355X.
 __host__ __device__ __forceinline__ uint32_t grid8(uint64_t n) {
     return static_cast<uint32_t>(((n + 7) / 8) * 8);
 }
 
-// Same idea when only `total` of the launched blocks carry work (total known
-// on the device, grid >= 8 * ceil(total / 8)): every XCD gets an equal,
-// contiguous share. Returns an index >= total for blocks without work.
+// Work index of block b when only `total` of the launched blocks carry work
+// (total known on the device, grid >= 8 * ceil(total / 8)). Workgroups are
+// dispatched round-robin over the 8 XCDs, so block b runs on XCD b % 8; each
+// XCD gets an equal, contiguous share of the work indices (neighbouring work
+// items share that XCD's L2). Returns an index >= total for blocks without
+// work.
 __device__ __forceinline__ uint32_t xcd_share(uint32_t b, uint32_t total) {
     const uint32_t per = (total + 7) / 8;
     const uint32_t q = b / 8;

@@ -1,13 +1,17 @@
 // k_join.hip — build, probe, chain expansion and projection gather on gfx950.
 //
-// Two table placements share one emit path:
-//   * LDS leaf join (PHJ leaves that fit a workgroup's LDS): one workgroup per
-//     (leaf, probe chunk) builds the leaf's open-addressing table of 16-byte
-//     (key, RowRefCell) cells in LDS with 64-bit CAS on the key slot, then
-//     probes the chunk.
+// Three ways to join, one output row format:
+//   * Fused leaf join (k_leaf_join; PHJ leaves that fit LDS, no repeated build
+//     key): one workgroup per (leaf, probe chunk) builds a compact table of the
+//     leaf in LDS (u64 keys + u16 rows, 64-bit CAS on the key slot), looks up
+//     its probe rows and writes their output rows, all in one launch.
+//   * Two-phase leaf join (k_leaf_probe + k_leaf_emit; repeated build keys):
+//     the probe pass builds 16-byte (key, RowRefCell) cells with next chains in
+//     LDS, counts each work item's output rows and exports the leaf's rows
+//     grouped by key; the emit pass writes rows at exact scanned offsets.
 //   * HBM tables (CHJ's 256 lock-free sub-tables, oversize leaves, the
-//     standalone JoinHashTable entry point): same cell format and protocol in
-//     global memory.
+//     standalone JoinHashTable entry point): 16-byte cells and next chains in
+//     global memory, the same insert protocol.
 // Semantics follow JoinHashTable::insert/find (src/HashTable.h:84-127) with the
 // BlockStore next-chain (src/BlockStore.h:86-89, src/JoinOps.h:222-227): a
 // repeated key swaps the head and threads the previous head into next[]; the
@@ -607,25 +611,13 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     }
 }
 
-// Cell index of key k in the leaf table, kSideSlot for the empty-key
-// sentinel's side chain, or kInvalidRow.
+// Fused leaf join: a matched slot of the leaf table, or kSideSlot for the
+// empty-key sentinel's side chain (rows whose key is kEmptyKey).
 constexpr uint32_t kSideSlot = 0xFFFEu;
+// Probe rows per lane and super-rounds per work item (chunk = 512 threads).
 constexpr int kFusedRowsPerThread = 16;
 constexpr int kFusedSuperRounds = 1;
 constexpr uint32_t kFusedChunkRows = kFusedSuperRounds * 512u * kFusedRowsPerThread;
-__device__ __forceinline__ uint32_t lookup_slot(const LdsCell *cells, uint32_t cap, uint32_t side_head, uint64_t k) {
-    if (k == kEmptyKey)
-        return side_head == kInvalidRow ? kInvalidRow : kSideSlot;
-    uint32_t pos = home_slot(k, cap);
-    while (true) {
-        const uint64_t ck = cells[pos].key;
-        if (ck == k)
-            return pos;
-        if (ck == kEmptyKey)
-            return kInvalidRow;
-        pos = (pos + 1 == cap) ? 0 : pos + 1;
-    }
-}
 
 // Element at byte offset `off` of a uniform base: the 32-bit offset keeps the
 // address arithmetic in the scalar base + vector offset form.
