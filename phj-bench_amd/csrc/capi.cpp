@@ -615,6 +615,28 @@ uint32_t radix_flags(const phj_tuning &t) {
     return ((t.flags & PHJ_TUNE_CACHED_LOADS) ? 0u : kRadixNtLoads) | ((t.flags & PHJ_TUNE_NT_STORES) ? kRadixNtStores : 0u);
 }
 
+// Largest leaf (build rows) the LDS join paths take for a build side whose
+// largest leaf has max_leaf rows: two workgroups per CU when the largest leaf
+// allows, otherwise one; 0 when HBM tables are forced.
+uint32_t lds_leaf_cap(const LeafJoinPlan &plan, uint32_t max_leaf) {
+    uint32_t cap = leaf_join_lds_max_rows(150u * 1024u, plan.load_pct);
+    const uint32_t two_per_cu = leaf_join_lds_max_rows(78u * 1024u, plan.load_pct);
+    if (max_leaf <= two_per_cu)
+        cap = two_per_cu;
+    if (plan.lds_limit_rows)
+        cap = std::min(cap, plan.lds_limit_rows);
+    return plan.force_global ? 0 : std::min(max_leaf, cap);
+}
+
+// Fused-join tables of every leaf of a build side, built once for joins that
+// probe it many times (PHJ best-effort slices).
+struct LeafTableImages {
+    const unsigned char *img = nullptr;
+    uint64_t stride = 0;
+    const uint32_t *side = nullptr;
+    uint32_t max_leaf = 0;
+};
+
 // Joins partitioned R and S leaf by leaf. When every leaf fits LDS the fused
 // join runs first: bound() is then |S| and emit() writes the rows in one
 // launch, unless a build key repeats, in which case emit() returns kRetry
@@ -624,20 +646,21 @@ uint32_t radix_flags(const phj_tuning &t) {
 // a given base row of an output buffer.
 class LeafJoin {
 public:
-    LeafJoin(phj_ctx *ctx, const Partitioned &R, const Partitioned &S, const LeafJoinPlan &plan)
+    // `img`: prebuilt fused tables of R's leaves (then R has no repeated key
+    // and every leaf fits LDS), or null.
+    LeafJoin(phj_ctx *ctx, const Partitioned &R, const Partitioned &S, const LeafJoinPlan &plan,
+             const LeafTableImages *img = nullptr)
         : ctx_(ctx), scr_(ctx), plan_(plan), R_(R), S_(S) {
         const uint32_t n_leaves = static_cast<uint32_t>(R.parts);
-        uint32_t *dmax = scr_.get<uint32_t>(1);
-        leaf_max_rows(ctx->stream, R.off, n_leaves, dmax);
-        const uint32_t max_leaf = read_scalar(ctx, dmax);
-        // Two workgroups per CU when the largest leaf allows; otherwise one.
-        uint32_t lds_cap_rows = leaf_join_lds_max_rows(150u * 1024u, plan.load_pct);
-        const uint32_t two_per_cu = leaf_join_lds_max_rows(78u * 1024u, plan.load_pct);
-        if (max_leaf <= two_per_cu)
-            lds_cap_rows = two_per_cu;
-        if (plan.lds_limit_rows)
-            lds_cap_rows = std::min(lds_cap_rows, plan.lds_limit_rows);
-        lds_max_ = plan.force_global ? 0 : std::min(max_leaf, lds_cap_rows);
+        uint32_t max_leaf;
+        if (img) {
+            max_leaf = img->max_leaf;
+        } else {
+            uint32_t *dmax = scr_.get<uint32_t>(1);
+            leaf_max_rows(ctx->stream, R.off, n_leaves, dmax);
+            max_leaf = read_scalar(ctx, dmax);
+        }
+        lds_max_ = lds_leaf_cap(plan, max_leaf);
 
         a_.r_keys = R.keys;
         a_.r_cols = as_in(R.cols);
@@ -653,6 +676,11 @@ public:
         // build key repeats (each probe row then has at most one match, so
         // S.rows bounds the output); a repeated key sends emit() back here.
         fused_ = !plan.two_phase && lds_max_ > 0 && max_leaf <= lds_max_;
+        if (fused_ && img) {
+            a_.table_img = img->img;
+            a_.img_stride = img->stride;
+            a_.img_side = img->side;
+        }
         if (fused_) {
             plan_work(leaf_join_fused_chunk());
             bound_ = S.rows;
@@ -688,6 +716,7 @@ public:
             if (h[1] == 0)
                 return rows;
             fused_ = false;
+            a_.table_img = nullptr;
             plan_two_phase();
             return kRetry;
         }
@@ -1088,6 +1117,50 @@ __global__ void k_count_nonempty(const uint32_t *off, uint32_t n, uint32_t *out)
 // partitioned slice (two pass buffers) fits the global budget; every slice is
 // partitioned to the leaves and joined against the leaves' tables at once (an
 // eviction of every leaf the slice touched), then its buffers are reused.
+// Builds the fused tables of every leaf of RP into `img` (buffers go to
+// `keep`); false, with nothing kept, when a leaf exceeds the LDS cap, a key
+// repeats, or the plan asks for another join path. The build time is added
+// to `build_ms`.
+bool build_table_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan &plan, std::vector<void *> &keep,
+                        LeafTableImages &img, double &build_ms) {
+    if (plan.two_phase || plan.force_global || RP.parts == 0)
+        return false;
+    const uint32_t n_leaves = static_cast<uint32_t>(RP.parts);
+    Scratch scr(ctx);
+    uint32_t *word = scr.get<uint32_t>(2); // [0] largest leaf, [1] repeated-key flag
+    leaf_max_rows(ctx->stream, RP.off, n_leaves, word);
+    const uint32_t max_leaf = read_scalar(ctx, word);
+    if (max_leaf == 0 || lds_leaf_cap(plan, max_leaf) < max_leaf)
+        return false;
+    EventPair ev;
+    HIPCHK(hipEventRecord(ev.a, ctx->stream));
+    const uint64_t stride = leaf_table_bytes(max_leaf, plan.load_pct);
+    unsigned char *tables = ctx->alloc<unsigned char>(stride * n_leaves);
+    uint32_t *side = ctx->alloc<uint32_t>(n_leaves);
+    LeafJoinArgs a{};
+    a.r_keys = RP.keys;
+    a.r_off = RP.off;
+    a.n_leaves = n_leaves;
+    a.lds_max_rows = max_leaf;
+    a.load_pct = plan.load_pct;
+    launch_fill_u32(ctx->stream, word + 1, 1, 0);
+    leaf_table_images(ctx->stream, a, tables, stride, side, word + 1, ctx->tm());
+    HIPCHK(hipEventRecord(ev.b, ctx->stream));
+    if (read_scalar(ctx, word + 1) != 0) {
+        ctx->arena.free(tables);
+        ctx->arena.free(side);
+        return false;
+    }
+    build_ms += ev.ms();
+    keep.push_back(tables);
+    keep.push_back(side);
+    img.img = tables;
+    img.stride = stride;
+    img.side = side;
+    img.max_leaf = max_leaf;
+    return true;
+}
+
 void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, const uint8_t *pb, uint32_t np,
                  uint64_t budget_bytes, const phj_tuning &tun, phj_join_result *res) {
     validate_rel(&b);
@@ -1114,6 +1187,11 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
 
     double shuffle_ms = 0, probe_ms = 0, big_build_ms = 0;
+    // With several slices every slice would rebuild every leaf table: when
+    // all leaves fit LDS and no key repeats, the fused tables are built once
+    // (the build phase) and each slice copies them.
+    LeafTableImages images;
+    const bool use_images = S.rows > slice_rows && build_table_images(ctx, RP, plan, keep, images, big_build_ms);
     uint64_t evictions = 0, refinements = 0, peak_bytes = 0;
     uint32_t *nonempty = scr.get<uint32_t>(1);
     const uint32_t p1_parts = 1u << passes[0];
@@ -1136,7 +1214,7 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
         HIPCHK(hipEventRecord(ej.a, ctx->stream));
         double bb = 0;
         {
-            LeafJoin lj(ctx, RP, SP, plan);
+            LeafJoin lj(ctx, RP, SP, plan, use_images ? &images : nullptr);
             uint64_t add;
             do {
                 if (n + lj.bound() > out.cap)

@@ -673,6 +673,10 @@ __host__ __device__ __forceinline__ size_t fused_lds_bytes(uint32_t max_rows, ui
     const size_t cap = table_cap(max_rows, load_pct);
     return cap * 8 + ((cap * 2 + 15) & ~size_t{15});
 }
+// The same rounded up to 16 bytes: the unit of a prebuilt table's copies.
+__host__ __device__ __forceinline__ size_t fused_table_bytes(uint32_t rows, uint32_t load_pct) {
+    return (fused_lds_bytes(rows, load_pct) + 15) & ~size_t{15};
+}
 
 // Builds the fused table (64-bit CAS on the key slot, then the row). Sets
 // `dup` (LDS) when a key occurs twice. Ends with a barrier.
@@ -777,7 +781,30 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
         out_cap = a.join_end[j];
         flag = a.join_flag + j;
     }
-    build_fused_table<THREADS>(a, it, cap, tkeys, trows, side_head, dup);
+    if (a.table_img) {
+        // prebuilt table: 16-byte copies, CB per thread in flight
+        constexpr int CB = 4;
+        const uint32_t words = static_cast<uint32_t>(fused_table_bytes(it.nr, a.load_pct) / 16);
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.table_img + static_cast<uint64_t>(it.leaf) * a.img_stride);
+        uint4 *dst = reinterpret_cast<uint4 *>(smem);
+        for (uint32_t base = 0; base < words; base += CB * THREADS) {
+            uint4 v[CB];
+#pragma unroll
+            for (int q = 0; q < CB; ++q)
+                v[q] = src[min(base + q * THREADS + threadIdx.x, words - 1)];
+#pragma unroll
+            for (int q = 0; q < CB; ++q)
+                if (base + q * THREADS + threadIdx.x < words)
+                    dst[base + q * THREADS + threadIdx.x] = v[q];
+        }
+        if (threadIdx.x == 0) {
+            side_head = a.img_side[it.leaf];
+            dup = 0;
+        }
+        __syncthreads();
+    } else {
+        build_fused_table<THREADS>(a, it, cap, tkeys, trows, side_head, dup);
+    }
     if (dup) {
         if (threadIdx.x == 0)
             atomicOr(flag, 1u);
@@ -918,6 +945,35 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
             }
         }
         __syncthreads(); // slot_off and sr_* are rewritten by the next super-round
+    }
+}
+
+// One workgroup per leaf: builds the leaf's fused-join table in LDS and
+// writes it, as laid out in LDS, to the leaf's slot of the image.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_leaf_table_image(LeafJoinArgs a, unsigned char *__restrict__ img,
+                                                              uint64_t stride, uint32_t *__restrict__ side,
+                                                              uint32_t *__restrict__ dup_any) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t side_head;
+    __shared__ uint32_t dup;
+    Item it{};
+    it.leaf = blockIdx.x;
+    it.r0 = a.r_off[it.leaf];
+    it.nr = a.r_off[it.leaf + 1] - it.r0;
+    if (it.nr == 0 || it.nr > a.lds_max_rows)
+        return; // uniform: such leaves have no fused work items
+    const uint32_t cap = table_cap(it.nr, a.load_pct);
+    build_fused_table<THREADS>(a, it, cap, reinterpret_cast<uint64_t *>(smem),
+                               reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * 8), side_head, dup);
+    const uint32_t words = static_cast<uint32_t>(fused_table_bytes(it.nr, a.load_pct) / 16);
+    uint4 *dst = reinterpret_cast<uint4 *>(img + static_cast<uint64_t>(it.leaf) * stride);
+    for (uint32_t i = threadIdx.x; i < words; i += THREADS)
+        dst[i] = reinterpret_cast<const uint4 *>(smem)[i];
+    if (threadIdx.x == 0) {
+        side[it.leaf] = side_head;
+        if (dup)
+            atomicOr(dup_any, 1u);
     }
 }
 
@@ -1283,10 +1339,23 @@ void leaf_chain_slots(hipStream_t s, const LeafJoinArgs &a, uint32_t *chain_firs
     scan_u32(s, chain_first, a.n_leaves + 1, nullptr, 1, scan_tmp, nullptr, "join_plan");
 }
 
+size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct) { return fused_table_bytes(rows, load_pct); }
+
+void leaf_table_images(hipStream_t s, const LeafJoinArgs &a, unsigned char *img, uint64_t stride, uint32_t *side,
+                       uint32_t *dup, KernelTimer *tm) {
+    if (a.n_leaves == 0)
+        return;
+    const size_t lds = fused_table_bytes(a.lds_max_rows, a.load_pct);
+    TimedScope ts(tm, "leaf_table_image", s);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_table_image<512>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    k_leaf_table_image<512><<<a.n_leaves, 512, lds, s>>>(a, img, stride, side, dup);
+}
+
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm) {
     if (max_work == 0)
         return;
-    const size_t lds = fused_lds_bytes(a.lds_max_rows, a.load_pct);
+    const size_t lds = fused_table_bytes(a.lds_max_rows, a.load_pct);
     TimedScope ts(tm, "leaf_join", s);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2>), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(lds));

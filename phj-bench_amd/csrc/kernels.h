@@ -118,6 +118,12 @@ struct LeafJoinArgs {
     unsigned long long *join_cursor;
     const uint64_t *join_end;
     uint32_t *join_flag;
+    // Fused path with prebuilt tables (null: each work item builds its leaf's
+    // table): leaf L's compact table is the leaf_table_bytes() bytes at
+    // table_img + L * img_stride, its empty-key side head img_side[L].
+    const unsigned char *table_img;
+    uint64_t img_stride;
+    const uint32_t *img_side;
 };
 // Work items per leaf for the LDS path (0 for empty or oversize leaves),
 // scanned, and the leaf of each item.
@@ -136,6 +142,14 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
 // nothing. Work items are probe chunks of leaf_join_fused_chunk() rows.
 uint32_t leaf_join_fused_chunk();
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm);
+// Bytes of the fused join's compact table of a leaf of `rows` build rows
+// (16-byte multiple), and the tables of every non-empty leaf built once into
+// img + L * stride (stride >= leaf_table_bytes(lds_max_rows)) with the side
+// heads in side[L]; *dup (zeroed by the caller) is set when a leaf repeats a
+// key. For joins that probe the same build side many times.
+size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct);
+void leaf_table_images(hipStream_t s, const LeafJoinArgs &a, unsigned char *img, uint64_t stride, uint32_t *side,
+                       uint32_t *dup, KernelTimer *tm);
 size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct);
 uint32_t leaf_join_lds_max_rows(uint32_t lds_bytes, uint32_t load_pct);
 
