@@ -680,9 +680,12 @@ public:
             a_.table_img = img->img;
             a_.img_stride = img->stride;
             a_.img_side = img->side;
+            // probe rows per leaf under a quarter of the largest leaf: lookups
+            // in HBM read less than copying every table to LDS
+            a_.img_direct = 4 * S.rows < static_cast<uint64_t>(n_leaves) * max_leaf ? 1u : 0u;
         }
         if (fused_) {
-            plan_work(leaf_join_fused_chunk());
+            plan_work(leaf_join_fused_chunk(a_.img_direct != 0));
             bound_ = S.rows;
             return;
         }
@@ -1488,7 +1491,7 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
     a.n_leaves = n_leaves;
     a.lds_max_rows = lds_max;
     a.load_pct = plan.load_pct;
-    a.chunk_rows = leaf_join_fused_chunk();
+    a.chunk_rows = leaf_join_fused_chunk(false);
     a.leaf_join = dleaf;
     a.join_cursor = cur;
     a.join_end = dend;

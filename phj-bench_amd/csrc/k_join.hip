@@ -618,6 +618,10 @@ constexpr uint32_t kSideSlot = 0xFFFEu;
 constexpr int kFusedRowsPerThread = 16;
 constexpr int kFusedSuperRounds = 1;
 constexpr uint32_t kFusedChunkRows = kFusedSuperRounds * 512u * kFusedRowsPerThread;
+// Lookups in prebuilt HBM tables need no LDS table, so small workgroups keep
+// many more of them in flight (the work items are small probe chunks).
+constexpr int kDirectThreads = 128;
+constexpr uint32_t kDirectChunkRows = kFusedSuperRounds * kDirectThreads * kFusedRowsPerThread;
 
 // Element at byte offset `off` of a uniform base: the 32-bit offset keeps the
 // address arithmetic in the scalar base + vector offset form.
@@ -750,7 +754,7 @@ __device__ __forceinline__ void build_fused_table(const LeafJoinArgs &a, const I
 // beyond the sink capacity are counted, not written. A leaf with a repeated
 // key raises the flag and writes nothing: the caller then joins with the
 // two-phase path.
-template <int THREADS, int K, int U>
+template <int THREADS, int K, int U, bool DIRECT>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_leaf_join(LeafJoinArgs a) {
     static_assert(K % U == 0 && K % 2 == 0 && K <= 64, "K slots per lane in rounds of U");
     constexpr int W = THREADS / 64;
@@ -767,8 +771,13 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
     if (!locate_item(a, w, it))
         return;
     const uint32_t cap = table_cap(it.nr, a.load_pct);
-    uint64_t *tkeys = reinterpret_cast<uint64_t *>(smem);
-    uint16_t *trows = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * 8);
+    // DIRECT: the prebuilt table is read where it lies in HBM (probe chunks
+    // much smaller than the leaf, where copying the table would cost more
+    // than the lookups); otherwise the table lives in LDS.
+    unsigned char *tbase = DIRECT ? const_cast<unsigned char *>(a.table_img) + static_cast<uint64_t>(it.leaf) * a.img_stride
+                                  : smem;
+    uint64_t *tkeys = reinterpret_cast<uint64_t *>(tbase);
+    uint16_t *trows = reinterpret_cast<uint16_t *>(tbase + static_cast<size_t>(cap) * 8);
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / 64;
 
@@ -781,7 +790,13 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
         out_cap = a.join_end[j];
         flag = a.join_flag + j;
     }
-    if (a.table_img) {
+    if (DIRECT) {
+        if (threadIdx.x == 0) {
+            side_head = a.img_side[it.leaf];
+            dup = 0;
+        }
+        __syncthreads();
+    } else if (a.table_img) {
         // prebuilt table: 16-byte copies, CB per thread in flight
         constexpr int CB = 4;
         const uint32_t words = static_cast<uint32_t>(fused_table_bytes(it.nr, a.load_pct) / 16);
@@ -1332,7 +1347,7 @@ void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_
         launch_probe<256, 16>(s, a, max_work, lds);
 }
 
-uint32_t leaf_join_fused_chunk() { return kFusedChunkRows; }
+uint32_t leaf_join_fused_chunk(bool direct) { return direct ? kDirectChunkRows : kFusedChunkRows; }
 
 void leaf_chain_slots(hipStream_t s, const LeafJoinArgs &a, uint32_t *chain_first, uint32_t *scan_tmp) {
     k_leaf_chain_slots<<<(a.n_leaves + 1 + 255) / 256, 256, 0, s>>>(a.r_off, a.work_first, a.n_leaves, chain_first);
@@ -1357,9 +1372,13 @@ void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, Ke
         return;
     const size_t lds = fused_table_bytes(a.lds_max_rows, a.load_pct);
     TimedScope ts(tm, "leaf_join", s);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(lds));
-    k_leaf_join<512, kFusedRowsPerThread, 2><<<grid8(max_work), 512, lds, s>>>(a);
+    if (a.table_img && a.img_direct) {
+        k_leaf_join<kDirectThreads, kFusedRowsPerThread, 2, true><<<grid8(max_work), kDirectThreads, 0, s>>>(a);
+        return;
+    }
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    k_leaf_join<512, kFusedRowsPerThread, 2, false><<<grid8(max_work), 512, lds, s>>>(a);
 }
 
 void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm) {

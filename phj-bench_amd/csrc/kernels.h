@@ -124,6 +124,7 @@ struct LeafJoinArgs {
     const unsigned char *table_img;
     uint64_t img_stride;
     const uint32_t *img_side;
+    uint32_t img_direct; // nonzero: look up in the prebuilt tables in HBM instead of copying them to LDS
 };
 // Work items per leaf for the LDS path (0 for empty or oversize leaves),
 // scanned, and the leaf of each item.
@@ -139,8 +140,9 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
 // Fused probe + emit for build sides without repeated keys: one output
 // reservation per work item on out.cursor (rows at or beyond out.cap are
 // counted only); a leaf with a repeated key sets *chain_flag and writes
-// nothing. Work items are probe chunks of leaf_join_fused_chunk() rows.
-uint32_t leaf_join_fused_chunk();
+// nothing. Work items are probe chunks of leaf_join_fused_chunk(img_direct)
+// rows.
+uint32_t leaf_join_fused_chunk(bool direct);
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm);
 // Bytes of the fused join's compact table of a leaf of `rows` build rows
 // (16-byte multiple), and the tables of every non-empty leaf built once into

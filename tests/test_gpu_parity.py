@@ -264,6 +264,19 @@ def test_fused_and_two_phase_leaf_join(ctx, pass_bits, flags):
     res.release()
 
 
+def test_bep_sparse_slices_read_tables_in_hbm(ctx):
+    """PHJ best-effort in slices of ~29K probe rows against 256 leaves of
+    ~1,200 build rows: with far fewer probe rows per leaf than build rows, the
+    slices look up in the prebuilt leaf tables where they lie in HBM."""
+    rng = np.random.default_rng(23)
+    ob, op = unique_with_sentinels(rng, 300000, 600000)
+    want = O.checksum_rows(O.serialize_sorted(O.reference_join(ob, op)))
+    res = ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig([4, 4]), bep_budget_mib=1)
+    assert res.build.wall_ms > 0  # tables built once
+    assert res.checksum() == want
+    res.release()
+
+
 @pytest.mark.parametrize("chunk", [700, 5000])
 def test_chains_across_work_items(ctx, chunk):
     """Repeated build keys in leaves of thousands of rows (tables built by many
