@@ -628,8 +628,9 @@ uint32_t lds_leaf_cap(const LeafJoinPlan &plan, uint32_t max_leaf) {
     return plan.force_global ? 0 : std::min(max_leaf, cap);
 }
 
-// Fused-join tables of every leaf of a build side, built once for joins that
-// probe it many times (PHJ best-effort slices).
+// Fused-join tables of every leaf of a build side as 16-byte cells in HBM,
+// built once for joins that probe it many times with few probe rows per leaf
+// (PHJ best-effort slices); lookups read them in place.
 struct LeafTableImages {
     const unsigned char *img = nullptr;
     uint64_t stride = 0;
@@ -680,12 +681,9 @@ public:
             a_.table_img = img->img;
             a_.img_stride = img->stride;
             a_.img_side = img->side;
-            // probe rows per leaf under a quarter of the largest leaf: lookups
-            // in HBM read less than copying every table to LDS
-            a_.img_direct = 4 * S.rows < static_cast<uint64_t>(n_leaves) * max_leaf ? 1u : 0u;
         }
         if (fused_) {
-            plan_work(leaf_join_fused_chunk(a_.img_direct != 0));
+            plan_work(leaf_join_fused_chunk(a_.table_img != nullptr));
             bound_ = S.rows;
             return;
         }
@@ -1120,12 +1118,15 @@ __global__ void k_count_nonempty(const uint32_t *off, uint32_t n, uint32_t *out)
 // partitioned slice (two pass buffers) fits the global budget; every slice is
 // partitioned to the leaves and joined against the leaves' tables at once (an
 // eviction of every leaf the slice touched), then its buffers are reused.
-// Builds the fused tables of every leaf of RP into `img` (buffers go to
-// `keep`); false, with nothing kept, when a leaf exceeds the LDS cap, a key
-// repeats, or the plan asks for another join path. The build time is added
-// to `build_ms`.
-bool build_table_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan &plan, std::vector<void *> &keep,
-                        LeafTableImages &img, double &build_ms) {
+// Builds the tables of every leaf of RP into `img` (buffers go to `keep`) for
+// probes of up to `probe_rows` rows each. False, with nothing kept, when those
+// probes hold a quarter of the largest leaf or more per leaf (each work item
+// then builds its table in LDS: one sequential read of the leaf's keys beats
+// a random HBM read per probe row), when a leaf exceeds the LDS cap, when a
+// key repeats, or when the plan asks for another join path. The build time
+// is added to `build_ms`.
+bool build_table_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan &plan, uint64_t probe_rows,
+                        std::vector<void *> &keep, LeafTableImages &img, double &build_ms) {
     if (plan.two_phase || plan.force_global || RP.parts == 0)
         return false;
     const uint32_t n_leaves = static_cast<uint32_t>(RP.parts);
@@ -1133,7 +1134,8 @@ bool build_table_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan 
     uint32_t *word = scr.get<uint32_t>(2); // [0] largest leaf, [1] repeated-key flag
     leaf_max_rows(ctx->stream, RP.off, n_leaves, word);
     const uint32_t max_leaf = read_scalar(ctx, word);
-    if (max_leaf == 0 || lds_leaf_cap(plan, max_leaf) < max_leaf)
+    if (max_leaf == 0 || lds_leaf_cap(plan, max_leaf) < max_leaf ||
+        4 * probe_rows >= static_cast<uint64_t>(n_leaves) * max_leaf)
         return false;
     EventPair ev;
     HIPCHK(hipEventRecord(ev.a, ctx->stream));
@@ -1190,11 +1192,12 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
 
     double shuffle_ms = 0, probe_ms = 0, big_build_ms = 0;
-    // With several slices every slice would rebuild every leaf table: when
-    // all leaves fit LDS and no key repeats, the fused tables are built once
-    // (the build phase) and each slice copies them.
+    // With several small slices every slice would rebuild every leaf table
+    // for a few probe rows per leaf: when all leaves fit LDS and no key
+    // repeats, the tables are built once (the build phase) and every slice
+    // looks up in them in HBM.
     LeafTableImages images;
-    const bool use_images = S.rows > slice_rows && build_table_images(ctx, RP, plan, keep, images, big_build_ms);
+    const bool use_images = S.rows > slice_rows && build_table_images(ctx, RP, plan, slice_rows, keep, images, big_build_ms);
     uint64_t evictions = 0, refinements = 0, peak_bytes = 0;
     uint32_t *nonempty = scr.get<uint32_t>(1);
     const uint32_t p1_parts = 1u << passes[0];

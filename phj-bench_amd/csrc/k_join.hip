@@ -618,8 +618,8 @@ constexpr uint32_t kSideSlot = 0xFFFEu;
 constexpr int kFusedRowsPerThread = 16;
 constexpr int kFusedSuperRounds = 1;
 constexpr uint32_t kFusedChunkRows = kFusedSuperRounds * 512u * kFusedRowsPerThread;
-// Lookups in prebuilt HBM tables need no LDS table, so small workgroups keep
-// many more of them in flight (the work items are small probe chunks).
+// Lookups in prebuilt HBM tables need no LDS table; small workgroups keep
+// more of them in flight (the work items are small probe chunks).
 constexpr int kDirectThreads = 128;
 constexpr uint32_t kDirectChunkRows = kFusedSuperRounds * kDirectThreads * kFusedRowsPerThread;
 
@@ -771,13 +771,14 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
     if (!locate_item(a, w, it))
         return;
     const uint32_t cap = table_cap(it.nr, a.load_pct);
-    // DIRECT: the prebuilt table is read where it lies in HBM (probe chunks
-    // much smaller than the leaf, where copying the table would cost more
-    // than the lookups); otherwise the table lives in LDS.
-    unsigned char *tbase = DIRECT ? const_cast<unsigned char *>(a.table_img) + static_cast<uint64_t>(it.leaf) * a.img_stride
-                                  : smem;
-    uint64_t *tkeys = reinterpret_cast<uint64_t *>(tbase);
-    uint16_t *trows = reinterpret_cast<uint16_t *>(tbase + static_cast<size_t>(cap) * 8);
+    // DIRECT: the leaf's prebuilt table of 16-byte (key, RowRefCell) cells is
+    // read where it lies in HBM (probe chunks much smaller than the leaf), one
+    // line per lookup for key and row. Otherwise the compact table is built in
+    // LDS.
+    uint64_t *tkeys = reinterpret_cast<uint64_t *>(smem);
+    uint16_t *trows = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * 8);
+    const Cell16 *cells =
+        DIRECT ? reinterpret_cast<const Cell16 *>(a.table_img + static_cast<uint64_t>(it.leaf) * a.img_stride) : nullptr;
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / 64;
 
@@ -791,27 +792,6 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
         flag = a.join_flag + j;
     }
     if (DIRECT) {
-        if (threadIdx.x == 0) {
-            side_head = a.img_side[it.leaf];
-            dup = 0;
-        }
-        __syncthreads();
-    } else if (a.table_img) {
-        // prebuilt table: 16-byte copies, CB per thread in flight
-        constexpr int CB = 4;
-        const uint32_t words = static_cast<uint32_t>(fused_table_bytes(it.nr, a.load_pct) / 16);
-        const uint4 *src = reinterpret_cast<const uint4 *>(a.table_img + static_cast<uint64_t>(it.leaf) * a.img_stride);
-        uint4 *dst = reinterpret_cast<uint4 *>(smem);
-        for (uint32_t base = 0; base < words; base += CB * THREADS) {
-            uint4 v[CB];
-#pragma unroll
-            for (int q = 0; q < CB; ++q)
-                v[q] = src[min(base + q * THREADS + threadIdx.x, words - 1)];
-#pragma unroll
-            for (int q = 0; q < CB; ++q)
-                if (base + q * THREADS + threadIdx.x < words)
-                    dst[base + q * THREADS + threadIdx.x] = v[q];
-        }
         if (threadIdx.x == 0) {
             side_head = a.img_side[it.leaf];
             dup = 0;
@@ -865,7 +845,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                 uint64_t ck[LB];
 #pragma unroll
                 for (int q = 0; q < LB; ++q)
-                    ck[q] = tkeys[pos[q]];
+                    ck[q] = DIRECT ? cells[pos[q]].key : tkeys[pos[q]];
 #pragma unroll
                 for (int q = 0; q < LB; ++q) {
                     const bool p = (pend >> q) & 1u;
@@ -944,8 +924,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                             okey[u] = kEmptyKey;
                             row = side_head;
                         } else {
-                            okey[u] = tkeys[c - 1];
-                            row = trows[c - 1];
+                            okey[u] = DIRECT ? cells[c - 1].key : tkeys[c - 1];
+                            row = DIRECT ? cells[c - 1].row_no : trows[c - 1];
                         }
                     }
                     ridx[u] = row;
@@ -964,7 +944,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
 }
 
 // One workgroup per leaf: builds the leaf's fused-join table in LDS and
-// writes it, as laid out in LDS, to the leaf's slot of the image.
+// writes it to the leaf's slot of the image as 16-byte (key, RowRefCell)
+// cells {key, leaf, leaf-local row} (src/HashTable.h:35-40).
 template <int THREADS>
 __global__ __launch_bounds__(THREADS) void k_leaf_table_image(LeafJoinArgs a, unsigned char *__restrict__ img,
                                                               uint64_t stride, uint32_t *__restrict__ side,
@@ -981,10 +962,16 @@ __global__ __launch_bounds__(THREADS) void k_leaf_table_image(LeafJoinArgs a, un
     const uint32_t cap = table_cap(it.nr, a.load_pct);
     build_fused_table<THREADS>(a, it, cap, reinterpret_cast<uint64_t *>(smem),
                                reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * 8), side_head, dup);
-    const uint32_t words = static_cast<uint32_t>(fused_table_bytes(it.nr, a.load_pct) / 16);
-    uint4 *dst = reinterpret_cast<uint4 *>(img + static_cast<uint64_t>(it.leaf) * stride);
-    for (uint32_t i = threadIdx.x; i < words; i += THREADS)
-        dst[i] = reinterpret_cast<const uint4 *>(smem)[i];
+    const uint64_t *tkeys = reinterpret_cast<const uint64_t *>(smem);
+    const uint16_t *trows = reinterpret_cast<const uint16_t *>(smem + static_cast<size_t>(cap) * 8);
+    Cell16 *dst = reinterpret_cast<Cell16 *>(img + static_cast<uint64_t>(it.leaf) * stride);
+    for (uint32_t i = threadIdx.x; i < cap; i += THREADS) {
+        Cell16 c;
+        c.key = tkeys[i];
+        c.block_no = it.leaf;
+        c.row_no = c.key == kEmptyKey ? kInvalidRow : trows[i];
+        dst[i] = c;
+    }
     if (threadIdx.x == 0) {
         side[it.leaf] = side_head;
         if (dup)
@@ -1347,14 +1334,14 @@ void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_
         launch_probe<256, 16>(s, a, max_work, lds);
 }
 
-uint32_t leaf_join_fused_chunk(bool direct) { return direct ? kDirectChunkRows : kFusedChunkRows; }
+uint32_t leaf_join_fused_chunk(bool prebuilt) { return prebuilt ? kDirectChunkRows : kFusedChunkRows; }
 
 void leaf_chain_slots(hipStream_t s, const LeafJoinArgs &a, uint32_t *chain_first, uint32_t *scan_tmp) {
     k_leaf_chain_slots<<<(a.n_leaves + 1 + 255) / 256, 256, 0, s>>>(a.r_off, a.work_first, a.n_leaves, chain_first);
     scan_u32(s, chain_first, a.n_leaves + 1, nullptr, 1, scan_tmp, nullptr, "join_plan");
 }
 
-size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct) { return fused_table_bytes(rows, load_pct); }
+size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct) { return static_cast<size_t>(table_cap(rows, load_pct)) * sizeof(Cell16); }
 
 void leaf_table_images(hipStream_t s, const LeafJoinArgs &a, unsigned char *img, uint64_t stride, uint32_t *side,
                        uint32_t *dup, KernelTimer *tm) {
@@ -1372,7 +1359,7 @@ void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, Ke
         return;
     const size_t lds = fused_table_bytes(a.lds_max_rows, a.load_pct);
     TimedScope ts(tm, "leaf_join", s);
-    if (a.table_img && a.img_direct) {
+    if (a.table_img) {
         k_leaf_join<kDirectThreads, kFusedRowsPerThread, 2, true><<<grid8(max_work), kDirectThreads, 0, s>>>(a);
         return;
     }

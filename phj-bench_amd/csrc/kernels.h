@@ -119,12 +119,11 @@ struct LeafJoinArgs {
     const uint64_t *join_end;
     uint32_t *join_flag;
     // Fused path with prebuilt tables (null: each work item builds its leaf's
-    // table): leaf L's compact table is the leaf_table_bytes() bytes at
-    // table_img + L * img_stride, its empty-key side head img_side[L].
+    // table in LDS): leaf L's table of 16-byte cells is at table_img + L *
+    // img_stride, its empty-key side head img_side[L]; lookups read it in HBM.
     const unsigned char *table_img;
     uint64_t img_stride;
     const uint32_t *img_side;
-    uint32_t img_direct; // nonzero: look up in the prebuilt tables in HBM instead of copying them to LDS
 };
 // Work items per leaf for the LDS path (0 for empty or oversize leaves),
 // scanned, and the leaf of each item.
@@ -140,15 +139,16 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
 // Fused probe + emit for build sides without repeated keys: one output
 // reservation per work item on out.cursor (rows at or beyond out.cap are
 // counted only); a leaf with a repeated key sets *chain_flag and writes
-// nothing. Work items are probe chunks of leaf_join_fused_chunk(img_direct)
-// rows.
-uint32_t leaf_join_fused_chunk(bool direct);
+// nothing. Work items are probe chunks of leaf_join_fused_chunk(prebuilt)
+// rows (prebuilt: table_img is set).
+uint32_t leaf_join_fused_chunk(bool prebuilt);
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm);
-// Bytes of the fused join's compact table of a leaf of `rows` build rows
-// (16-byte multiple), and the tables of every non-empty leaf built once into
-// img + L * stride (stride >= leaf_table_bytes(lds_max_rows)) with the side
-// heads in side[L]; *dup (zeroed by the caller) is set when a leaf repeats a
-// key. For joins that probe the same build side many times.
+// Bytes of a prebuilt leaf table (16-byte cells, the fused join's capacity)
+// for a leaf of `rows` build rows, and the tables of every non-empty leaf
+// built once into img + L * stride (stride >= leaf_table_bytes(lds_max_rows))
+// with the side heads in side[L]; *dup (zeroed by the caller) is set when a
+// leaf repeats a key. For joins that probe the same build side many times
+// with few probe rows per leaf.
 size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct);
 void leaf_table_images(hipStream_t s, const LeafJoinArgs &a, unsigned char *img, uint64_t stride, uint32_t *side,
                        uint32_t *dup, KernelTimer *tm);
