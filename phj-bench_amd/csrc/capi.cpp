@@ -678,12 +678,17 @@ public:
         // S.rows bounds the output); a repeated key sends emit() back here.
         fused_ = !plan.two_phase && lds_max_ > 0 && max_leaf <= lds_max_;
         if (fused_ && img) {
+            // fixed probe-row ranges against the prebuilt tables: no work plan
             a_.table_img = img->img;
             a_.img_stride = img->stride;
             a_.img_side = img->side;
+            while ((1ull << leaf_bits_) < R.parts)
+                ++leaf_bits_;
+            bound_ = S.rows;
+            return;
         }
         if (fused_) {
-            plan_work(leaf_join_fused_chunk(a_.table_img != nullptr));
+            plan_work(leaf_join_fused_chunk());
             bound_ = S.rows;
             return;
         }
@@ -709,7 +714,10 @@ public:
             launch_fill_u64(ctx_->stream, reinterpret_cast<uint64_t *>(cf) + 1, 1, 0);
             a_.out = sink_of(out, cf);
             a_.chain_flag = reinterpret_cast<uint32_t *>(cf + 1);
-            leaf_join_fused(ctx_->stream, a_, max_work_, ctx_->tm());
+            if (a_.table_img)
+                prebuilt_join(ctx_->stream, a_, S_.rows, leaf_bits_, ctx_->tm());
+            else
+                leaf_join_fused(ctx_->stream, a_, max_work_, ctx_->tm());
             uint64_t h[2];
             HIPCHK(hipMemcpyAsync(h, cf, sizeof(h), hipMemcpyDeviceToHost, ctx_->stream));
             ctx_->sync();
@@ -717,7 +725,6 @@ public:
             if (h[1] == 0)
                 return rows;
             fused_ = false;
-            a_.table_img = nullptr;
             plan_two_phase();
             return kRetry;
         }
@@ -869,6 +876,7 @@ private:
     bool fused_ = false;
     uint64_t bound_ = 0;
     uint32_t max_leaf_ = 0;
+    uint32_t leaf_bits_ = 0; // prebuilt tables: log2 of the leaf count
 };
 
 } // namespace
@@ -1494,7 +1502,7 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
     a.n_leaves = n_leaves;
     a.lds_max_rows = lds_max;
     a.load_pct = plan.load_pct;
-    a.chunk_rows = leaf_join_fused_chunk(false);
+    a.chunk_rows = leaf_join_fused_chunk();
     a.leaf_join = dleaf;
     a.join_cursor = cur;
     a.join_end = dend;

@@ -618,10 +618,6 @@ constexpr uint32_t kSideSlot = 0xFFFEu;
 constexpr int kFusedRowsPerThread = 16;
 constexpr int kFusedSuperRounds = 1;
 constexpr uint32_t kFusedChunkRows = kFusedSuperRounds * 512u * kFusedRowsPerThread;
-// Lookups in prebuilt HBM tables need no LDS table; small workgroups keep
-// more of them in flight (the work items are small probe chunks).
-constexpr int kDirectThreads = 128;
-constexpr uint32_t kDirectChunkRows = kFusedSuperRounds * kDirectThreads * kFusedRowsPerThread;
 
 // Element at byte offset `off` of a uniform base: the 32-bit offset keeps the
 // address arithmetic in the scalar base + vector offset form.
@@ -754,7 +750,7 @@ __device__ __forceinline__ void build_fused_table(const LeafJoinArgs &a, const I
 // beyond the sink capacity are counted, not written. A leaf with a repeated
 // key raises the flag and writes nothing: the caller then joins with the
 // two-phase path.
-template <int THREADS, int K, int U, bool DIRECT>
+template <int THREADS, int K, int U>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_leaf_join(LeafJoinArgs a) {
     static_assert(K % U == 0 && K % 2 == 0 && K <= 64, "K slots per lane in rounds of U");
     constexpr int W = THREADS / 64;
@@ -771,14 +767,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
     if (!locate_item(a, w, it))
         return;
     const uint32_t cap = table_cap(it.nr, a.load_pct);
-    // DIRECT: the leaf's prebuilt table of 16-byte (key, RowRefCell) cells is
-    // read where it lies in HBM (probe chunks much smaller than the leaf), one
-    // line per lookup for key and row. Otherwise the compact table is built in
-    // LDS.
     uint64_t *tkeys = reinterpret_cast<uint64_t *>(smem);
     uint16_t *trows = reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * 8);
-    const Cell16 *cells =
-        DIRECT ? reinterpret_cast<const Cell16 *>(a.table_img + static_cast<uint64_t>(it.leaf) * a.img_stride) : nullptr;
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / 64;
 
@@ -791,15 +781,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
         out_cap = a.join_end[j];
         flag = a.join_flag + j;
     }
-    if (DIRECT) {
-        if (threadIdx.x == 0) {
-            side_head = a.img_side[it.leaf];
-            dup = 0;
-        }
-        __syncthreads();
-    } else {
-        build_fused_table<THREADS>(a, it, cap, tkeys, trows, side_head, dup);
-    }
+    build_fused_table<THREADS>(a, it, cap, tkeys, trows, side_head, dup);
     if (dup) {
         if (threadIdx.x == 0)
             atomicOr(flag, 1u);
@@ -845,7 +827,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                 uint64_t ck[LB];
 #pragma unroll
                 for (int q = 0; q < LB; ++q)
-                    ck[q] = DIRECT ? cells[pos[q]].key : tkeys[pos[q]];
+                    ck[q] = tkeys[pos[q]];
 #pragma unroll
                 for (int q = 0; q < LB; ++q) {
                     const bool p = (pend >> q) & 1u;
@@ -924,8 +906,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                             okey[u] = kEmptyKey;
                             row = side_head;
                         } else {
-                            okey[u] = DIRECT ? cells[c - 1].key : tkeys[c - 1];
-                            row = DIRECT ? cells[c - 1].row_no : trows[c - 1];
+                            okey[u] = tkeys[c - 1];
+                            row = trows[c - 1];
                         }
                     }
                     ridx[u] = row;
@@ -957,8 +939,11 @@ __global__ __launch_bounds__(THREADS) void k_leaf_table_image(LeafJoinArgs a, un
     it.leaf = blockIdx.x;
     it.r0 = a.r_off[it.leaf];
     it.nr = a.r_off[it.leaf + 1] - it.r0;
-    if (it.nr == 0 || it.nr > a.lds_max_rows)
-        return; // uniform: such leaves have no fused work items
+    if (it.nr == 0 || it.nr > a.lds_max_rows) { // uniform
+        if (threadIdx.x == 0)
+            side[it.leaf] = kInvalidRow; // no table: probe rows find nothing
+        return;
+    }
     const uint32_t cap = table_cap(it.nr, a.load_pct);
     build_fused_table<THREADS>(a, it, cap, reinterpret_cast<uint64_t *>(smem),
                                reinterpret_cast<uint16_t *>(smem + static_cast<size_t>(cap) * 8), side_head, dup);
@@ -976,6 +961,139 @@ __global__ __launch_bounds__(THREADS) void k_leaf_table_image(LeafJoinArgs a, un
         side[it.leaf] = side_head;
         if (dup)
             atomicOr(dup_any, 1u);
+    }
+}
+
+// Join of partitioned probe rows against prebuilt leaf tables (16-byte cells
+// in HBM, no repeated build key): a work item is a fixed range of THREADS*K
+// probe rows, whatever leaves they fall in. Each row's leaf is the top
+// leaf_bits of its intHash64, the bits the radix passes consumed, so no leaf
+// lookup table is needed; the rows of a leaf are adjacent, so neighbouring
+// lanes read the same table. Lookups advance LB keys in lock step; the work
+// item's output rows are reserved with one atomic add and laid out slot-major
+// as in the fused join; then rows are written in rounds of U slots.
+template <int THREADS, int K, int U>
+__global__ __launch_bounds__(THREADS) void k_prebuilt_join(LeafJoinArgs a, uint64_t n_rows, uint32_t leaf_bits) {
+    static_assert(K % U == 0 && K <= 64, "K slots per lane in rounds of U");
+    constexpr int W = THREADS / 64;
+    constexpr int LB = 4;
+    static_assert(K % LB == 0, "lookup batches");
+    __shared__ uint32_t slot_off[K][W];
+    __shared__ uint32_t sr_total;
+    __shared__ unsigned long long sr_base;
+    const uint64_t sb = static_cast<uint64_t>(blockIdx.x) * THREADS * K;
+    if (sb >= n_rows)
+        return; // uniform
+    const uint64_t s1 = min(n_rows, sb + static_cast<uint64_t>(THREADS) * K);
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64;
+    uint32_t ref[K]; // matched build row + 1 (build-side row index), 0: none
+#pragma unroll
+    for (int g = 0; g < K / LB; ++g) {
+        uint64_t key[LB], base[LB];
+        uint32_t pos[LB], cap[LB], r0[LB], res[LB];
+        uint32_t pend = 0;
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            const uint64_t j = sb + static_cast<uint64_t>((g * LB + q) * THREADS + threadIdx.x);
+            key[q] = a.s_keys[min(j, s1 - 1)]; // rows past the range are masked below
+        }
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            const uint64_t j = sb + static_cast<uint64_t>((g * LB + q) * THREADS + threadIdx.x);
+            const uint32_t leaf = leaf_bits ? static_cast<uint32_t>(int_hash64(key[q]) >> (64 - leaf_bits)) : 0u;
+            r0[q] = a.r_off[leaf];
+            const uint32_t nr = a.r_off[leaf + 1] - r0[q];
+            cap[q] = table_cap(nr, a.load_pct);
+            base[q] = static_cast<uint64_t>(leaf) * a.img_stride;
+            pos[q] = home_slot(key[q], cap[q]);
+            res[q] = kInvalidRow;
+            if (j < s1) {
+                if (key[q] == kEmptyKey)
+                    res[q] = a.img_side[leaf];
+                else if (nr > 0)
+                    pend |= 1u << q;
+            }
+        }
+        while (pend) {
+            Cell16 c[LB];
+#pragma unroll
+            for (int q = 0; q < LB; ++q)
+                c[q] = *reinterpret_cast<const Cell16 *>(a.table_img + base[q] + static_cast<uint64_t>(pos[q]) * sizeof(Cell16));
+#pragma unroll
+            for (int q = 0; q < LB; ++q) {
+                const bool p = (pend >> q) & 1u;
+                const bool hit = c[q].key == key[q];
+                const bool stop = hit || c[q].key == kEmptyKey;
+                res[q] = (p && hit) ? c[q].row_no : res[q];
+                pend &= ~((p && stop ? 1u : 0u) << q);
+                pos[q] = (p && !stop) ? (pos[q] + 1 == cap[q] ? 0u : pos[q] + 1) : pos[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < LB; ++q)
+            ref[g * LB + q] = res[q] == kInvalidRow ? 0u : r0[q] + res[q] + 1;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t m = __ballot(ref[k] != 0);
+        if (lane == 0)
+            slot_off[k][wave] = static_cast<uint32_t>(__popcll(m));
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t pre[W];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int ww = 0; ww < W; ++ww) {
+            pre[ww] = tot;
+            tot += lane < static_cast<uint32_t>(K) ? slot_off[lane < K ? lane : 0][ww] : 0u;
+        }
+        const uint32_t incl = wave_incl_scan(tot);
+        if (lane < static_cast<uint32_t>(K)) {
+#pragma unroll
+            for (int ww = 0; ww < W; ++ww)
+                slot_off[lane][ww] = incl - tot + pre[ww];
+        }
+        if (lane == 63) {
+            sr_total = incl;
+            if (incl)
+                sr_base = atomicAdd(a.out.cursor, static_cast<unsigned long long>(incl));
+        }
+    }
+    __syncthreads();
+    if (sr_total == 0)
+        return; // uniform
+    const uint64_t rb = sr_base;
+#pragma unroll
+    for (int r = 0; r < K / U; ++r) {
+        if (sb + static_cast<uint64_t>(r) * U * THREADS >= s1)
+            break; // uniform
+        uint32_t ridx[U], sidx[U];
+        uint64_t opos[U];
+        bool on[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = r * U + u;
+            const uint64_t m = __ballot(ref[k] != 0);
+            opos[u] = rb + slot_off[k][wave] + mbcnt(m);
+            on[u] = ref[k] != 0 && opos[u] < a.out.cap;
+            ridx[u] = on[u] ? ref[k] - 1 : 0u;
+            sidx[u] = static_cast<uint32_t>(sb) + static_cast<uint32_t>(k * THREADS) + threadIdx.x;
+        }
+        {
+            // the output key is the probe key (equal to the matched build key)
+            uint64_t kv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                kv[u] = a.s_keys[on[u] ? sidx[u] : 0u];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (on[u])
+                    a.out.keys[opos[u]] = kv[u];
+        }
+        gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
+        gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
     }
 }
 
@@ -1334,11 +1452,19 @@ void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_
         launch_probe<256, 16>(s, a, max_work, lds);
 }
 
-uint32_t leaf_join_fused_chunk(bool prebuilt) { return prebuilt ? kDirectChunkRows : kFusedChunkRows; }
+uint32_t leaf_join_fused_chunk() { return kFusedChunkRows; }
 
 void leaf_chain_slots(hipStream_t s, const LeafJoinArgs &a, uint32_t *chain_first, uint32_t *scan_tmp) {
     k_leaf_chain_slots<<<(a.n_leaves + 1 + 255) / 256, 256, 0, s>>>(a.r_off, a.work_first, a.n_leaves, chain_first);
     scan_u32(s, chain_first, a.n_leaves + 1, nullptr, 1, scan_tmp, nullptr, "join_plan");
+}
+
+void prebuilt_join(hipStream_t s, const LeafJoinArgs &a, uint64_t probe_rows, uint32_t leaf_bits, KernelTimer *tm) {
+    constexpr int T = 256, K = 8;
+    if (probe_rows == 0)
+        return;
+    TimedScope ts(tm, "leaf_join", s);
+    k_prebuilt_join<T, K, 2><<<static_cast<uint32_t>((probe_rows + T * K - 1) / (T * K)), T, 0, s>>>(a, probe_rows, leaf_bits);
 }
 
 size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct) { return static_cast<size_t>(table_cap(rows, load_pct)) * sizeof(Cell16); }
@@ -1359,13 +1485,9 @@ void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, Ke
         return;
     const size_t lds = fused_table_bytes(a.lds_max_rows, a.load_pct);
     TimedScope ts(tm, "leaf_join", s);
-    if (a.table_img) {
-        k_leaf_join<kDirectThreads, kFusedRowsPerThread, 2, true><<<grid8(max_work), kDirectThreads, 0, s>>>(a);
-        return;
-    }
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2, false>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    k_leaf_join<512, kFusedRowsPerThread, 2, false><<<grid8(max_work), 512, lds, s>>>(a);
+    k_leaf_join<512, kFusedRowsPerThread, 2><<<grid8(max_work), 512, lds, s>>>(a);
 }
 
 void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm) {

@@ -118,9 +118,8 @@ struct LeafJoinArgs {
     unsigned long long *join_cursor;
     const uint64_t *join_end;
     uint32_t *join_flag;
-    // Fused path with prebuilt tables (null: each work item builds its leaf's
-    // table in LDS): leaf L's table of 16-byte cells is at table_img + L *
-    // img_stride, its empty-key side head img_side[L]; lookups read it in HBM.
+    // Prebuilt tables (prebuilt_join): leaf L's table of 16-byte cells is at
+    // table_img + L * img_stride, its empty-key side head img_side[L].
     const unsigned char *table_img;
     uint64_t img_stride;
     const uint32_t *img_side;
@@ -139,9 +138,8 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
 // Fused probe + emit for build sides without repeated keys: one output
 // reservation per work item on out.cursor (rows at or beyond out.cap are
 // counted only); a leaf with a repeated key sets *chain_flag and writes
-// nothing. Work items are probe chunks of leaf_join_fused_chunk(prebuilt)
-// rows (prebuilt: table_img is set).
-uint32_t leaf_join_fused_chunk(bool prebuilt);
+// nothing. Work items are probe chunks of leaf_join_fused_chunk() rows.
+uint32_t leaf_join_fused_chunk();
 void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, KernelTimer *tm);
 // Bytes of a prebuilt leaf table (16-byte cells, the fused join's capacity)
 // for a leaf of `rows` build rows, and the tables of every non-empty leaf
@@ -150,6 +148,12 @@ void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, Ke
 // leaf repeats a key. For joins that probe the same build side many times
 // with few probe rows per leaf.
 size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct);
+// Join of probe_rows partitioned probe rows (a.s_keys, a.s_cols) against the
+// prebuilt tables (a.table_img, a.img_stride, a.img_side) of a build side
+// with 2^leaf_bits leaves (a.r_off, a.r_cols) and no repeated key; output
+// rows are reserved on a.out.cursor (rows at or beyond a.out.cap are counted
+// only).
+void prebuilt_join(hipStream_t s, const LeafJoinArgs &a, uint64_t probe_rows, uint32_t leaf_bits, KernelTimer *tm);
 void leaf_table_images(hipStream_t s, const LeafJoinArgs &a, unsigned char *img, uint64_t stride, uint32_t *side,
                        uint32_t *dup, KernelTimer *tm);
 size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct);

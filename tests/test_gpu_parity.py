@@ -264,12 +264,19 @@ def test_fused_and_two_phase_leaf_join(ctx, pass_bits, flags):
     res.release()
 
 
-def test_bep_sparse_slices_read_tables_in_hbm(ctx):
+@pytest.mark.parametrize("half_empty", [False, True])
+def test_bep_sparse_slices_read_tables_in_hbm(ctx, half_empty):
     """PHJ best-effort in slices of ~29K probe rows against 256 leaves of
     ~1,200 build rows: with far fewer probe rows per leaf than build rows, the
-    slices look up in the prebuilt leaf tables where they lie in HBM."""
+    slices look up in the prebuilt leaf tables where they lie in HBM. With
+    half_empty, no build key hashes into the upper half of the leaves, so half
+    the probe rows (and possibly the sentinels) land in leaves without a
+    table."""
     rng = np.random.default_rng(23)
-    ob, op = unique_with_sentinels(rng, 300000, 600000)
+    ob, op = unique_with_sentinels(rng, 600000 if half_empty else 300000, 600000)
+    if half_empty:
+        keep = (O.int_hash64(ob.keys) >> np.uint64(63)) == 0
+        ob = O.Relation(ob.keys[keep], [c[keep] for c in ob.cols], ob.types)
     want = O.checksum_rows(O.serialize_sorted(O.reference_join(ob, op)))
     res = ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig([4, 4]), bep_budget_mib=1)
     assert res.build.wall_ms > 0  # tables built once
