@@ -1114,18 +1114,13 @@ void probe_resident(phj_ctx *ctx, const phj_resident &r, const phj_relation &p, 
     phj_pure_probe(ctx, r.RP, r.r_cols, S, r.passes, tun, e2e, nullptr, keep, res);
 }
 
-__global__ void k_count_nonempty(const uint32_t *off, uint32_t n, uint32_t *out) {
+// Adds the number of non-empty partitions to *out (one count over all slices).
+__global__ void k_count_nonempty(const uint32_t *off, uint32_t n, unsigned long long *out) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n && off[g + 1] > off[g])
-        atomicAdd(out, 1u);
+        atomicAdd(out, 1ull);
 }
 
-// ---- PHJ (best-effort partitioning): src/PHJBep.cpp:113-790 ----
-// The build side is shuffled to full leaf depth exactly as PHJ-PURE. The probe
-// side is never materialised whole: it is consumed in slices sized so that one
-// partitioned slice (two pass buffers) fits the global budget; every slice is
-// partitioned to the leaves and joined against the leaves' tables at once (an
-// eviction of every leaf the slice touched), then its buffers are reused.
 // Builds the tables of every leaf of RP into `img` (buffers go to `keep`) for
 // probes of up to `probe_rows` rows each. False, with nothing kept, when those
 // probes hold a quarter of the largest leaf or more per leaf (each work item
@@ -1174,6 +1169,12 @@ bool build_table_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan 
     return true;
 }
 
+// ---- PHJ (best-effort partitioning): src/PHJBep.cpp:113-790 ----
+// The build side is shuffled to full leaf depth exactly as PHJ-PURE. The probe
+// side is never materialised whole: it is consumed in slices sized so that one
+// partitioned slice (two pass buffers) fits the global budget; every slice is
+// partitioned to the leaves and joined against the leaves' tables at once (an
+// eviction of every leaf the slice touched), then its buffers are reused.
 void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, const uint8_t *pb, uint32_t np,
                  uint64_t budget_bytes, const phj_tuning &tun, phj_join_result *res) {
     validate_rel(&b);
@@ -1206,8 +1207,13 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     // looks up in them in HBM.
     LeafTableImages images;
     const bool use_images = S.rows > slice_rows && build_table_images(ctx, RP, plan, slice_rows, keep, images, big_build_ms);
-    uint64_t evictions = 0, refinements = 0, peak_bytes = 0;
-    uint32_t *nonempty = scr.get<uint32_t>(1);
+    uint64_t refinements = 0, peak_bytes = 0;
+    // evictions: non-empty (slice, leaf) pairs, counted on the device over all
+    // slices; slice timings are read after the last slice (no wait per slice)
+    unsigned long long *nonempty = scr.get<unsigned long long>(1);
+    launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(nonempty), 1, 0);
+    std::vector<std::unique_ptr<EventPair>> shuffle_ev, join_ev;
+    std::vector<double> join_build_ms;
     const uint32_t p1_parts = 1u << passes[0];
     OutBuffers out;
     // Output grows geometrically; each slice's exact row count is known
@@ -1220,7 +1226,9 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
         slice.keys = S.keys + a;
         for (uint32_t c = 0; c < S.cols.n; ++c)
             slice.cols.ptr[c] = static_cast<const uint8_t *>(S.cols.ptr[c]) + a * type_size(S.cols.types[c]);
-        EventPair es, ej;
+        shuffle_ev.push_back(std::make_unique<EventPair>());
+        join_ev.push_back(std::make_unique<EventPair>());
+        EventPair &es = *shuffle_ev.back(), &ej = *join_ev.back();
         HIPCHK(hipEventRecord(es.a, ctx->stream));
         std::vector<void *> sk;
         const Partitioned SP = radix_partition(ctx, slice, nullptr, 1, 0, passes, sk, "probe", radix_flags(tun));
@@ -1238,21 +1246,21 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
             bb = lj.big_build_ms();
         }
         HIPCHK(hipEventRecord(ej.b, ctx->stream));
-        launch_fill_u32(ctx->stream, nonempty, 1, 0);
         k_count_nonempty<<<static_cast<uint32_t>((SP.parts + 255) / 256), 256, 0, ctx->stream>>>(
             SP.off, static_cast<uint32_t>(SP.parts), nonempty);
-        const uint32_t ne = read_scalar(ctx, nonempty);
-        evictions += ne;
         refinements += passes.size() > 1 ? p1_parts : 0;
         peak_bytes = std::max<uint64_t>(peak_bytes, 2 * slice.rows * row_bytes);
-        shuffle_ms += es.ms();
-        probe_ms += ej.ms() - bb;
-        big_build_ms += bb;
+        join_build_ms.push_back(bb);
         for (void *k : sk)
             ctx->arena.free(k);
     }
     HIPCHK(hipEventRecord(e2e.b, ctx->stream));
-    ctx->sync();
+    const uint64_t evictions = read_scalar(ctx, reinterpret_cast<const uint64_t *>(nonempty)); // waits for the stream
+    for (size_t i = 0; i < join_ev.size(); ++i) {
+        shuffle_ms += shuffle_ev[i]->ms();
+        probe_ms += join_ev[i]->ms() - join_build_ms[i];
+        big_build_ms += join_build_ms[i];
+    }
     for (void *k : keep)
         ctx->arena.free(k);
     set_phase(res->build_shuffle, ebs.ms(), R.rows);

@@ -281,6 +281,12 @@ def test_bep_sparse_slices_read_tables_in_hbm(ctx, half_empty):
     res = ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig([4, 4]), bep_budget_mib=1)
     assert res.build.wall_ms > 0  # tables built once
     assert res.checksum() == want
+    # slices of 2^20 // (2 * 18) probe rows; each slice's rows fill all 256
+    # leaves; the 8 bits run as one device pass, so nothing is refined
+    slices = -(-op.keys.size // ((1 << 20) // 36))
+    assert slices == 21
+    assert res.bep_evictions == slices * 256
+    assert res.bep_refinements == 0
     res.release()
 
 
