@@ -23,7 +23,7 @@
  *     PHJ_ERR_OUT_OF_MEMORY maps std::bad_alloc.
  *   - Relations are column-major: a uint64 key column plus n_cols payload
  *     columns of PayloadType widths 1/2/4/8/16 bytes (src/Types.h:27-52,
- *     src/Block.h:182-188). on_device selects host or device pointers.
+ *     src/Block.h:21-63). on_device selects host or device pointers.
  *   - Results live in device memory owned by the context until released.
  *   - threads == 0 is accepted and treated as 1 (src/PHJ.cpp:21-22); the GPU
  *     path does not use it.
@@ -92,14 +92,14 @@ typedef struct phj_u128 {
     uint64_t high;
 } phj_u128;
 
-/* src/Block.h:131-173 — Column{type, data}. */
+/* src/Block.h:21-63 — Column{type, data}. */
 typedef struct phj_column {
     uint32_t type; /* phj_payload_type */
     uint32_t reserved;
     const void *data;
 } phj_column;
 
-/* src/Block.h:269-274 — a BlockStream flattened to contiguous columns. */
+/* src/Block.h:159-164 — a BlockStream flattened to contiguous columns. */
 typedef struct phj_relation {
     uint64_t rows;
     const uint64_t *keys;
@@ -207,6 +207,12 @@ phj_status phj_memcpy_h2d(phj_ctx *ctx, void *dst, const void *src, uint64_t byt
 phj_status phj_memcpy_d2h(phj_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 phj_status phj_mem_stats(phj_ctx *ctx, uint64_t *current_bytes, uint64_t *peak_bytes);
 phj_status phj_reset_peak(phj_ctx *ctx);
+/* Upper bound on the device bytes the context's live allocations may hold
+ * (0, the default: none). An allocation beyond it fails with
+ * PHJ_ERR_OUT_OF_MEMORY, like std::bad_alloc in the reference; a call that
+ * fails this way returns every block it held. No reference counterpart: the
+ * reference tracks peak memory (src/MemTracker.h) but never caps it. */
+phj_status phj_set_mem_limit(phj_ctx *ctx, uint64_t bytes);
 
 /* ---- hash: src/Hash.h:18-58 ---- */
 phj_status phj_int_hash64_batch(phj_ctx *ctx, const uint64_t *in, uint64_t n, uint64_t *out, int on_device);
@@ -242,8 +248,9 @@ phj_status phj_run_phj(phj_ctx *ctx, const phj_relation *build, const phj_relati
  * after the build shuffle). The reference re-shuffles the build stream on
  * every call (README.md:146-151); this handle is what a serving loop keeps
  * between requests. phj_probe_registered may run from any context on the same
- * device, concurrently with other probes of the same handle; release through
- * the registering context once no probe is in flight. */
+ * device, concurrently with other probes of the same handle. Release goes
+ * through the registering context; it fails with PHJ_ERR_INVALID_ARGUMENT
+ * while any probe of the handle is running (counted on the host). */
 typedef struct phj_resident phj_resident;
 phj_status phj_build_register(phj_ctx *ctx, const phj_relation *build, const uint8_t *pass_bits, uint32_t n_passes,
                               const phj_tuning *tuning, phj_resident **out);
@@ -299,8 +306,10 @@ phj_status phj_gen_relations(phj_ctx *ctx, const phj_gen_spec *spec, phj_relatio
 phj_status phj_gen_batch(phj_ctx *ctx, uint32_t n, const phj_gen_spec *specs, phj_relation *build, phj_relation *probe,
                          uint64_t *build_off, uint64_t *probe_off);
 phj_status phj_relation_release(phj_ctx *ctx, phj_relation *rel);
-/* Checksum of the exact join output implied by the generator (PHJ_DIST_UNIFORM
- * and PHJ_DIST_REFERENCE), computed without joining. */
+/* Checksum of the exact join output implied by the generator, computed
+ * without joining: each matching probe row pairs with the build row it was
+ * drawn from (PHJ_DIST_UNIFORM, PHJ_DIST_REFERENCE), or with every build row
+ * of the distinct key it drew (PHJ_DIST_ZIPF). */
 phj_status phj_gen_expected_checksum(phj_ctx *ctx, const phj_gen_spec *spec, phj_checksum *out);
 
 /* ---- measurement helpers ---- */

@@ -511,9 +511,17 @@ int orc_reference_join_checksum(const orc_relation *build, const orc_relation *p
 }
 
 int orc_gen_expected_checksum(const orc_gen_spec *spec, uint32_t threads, orc_checksum *out) {
-    if (spec->distribution == 2)
-        return 1; // duplicated build keys: use orc_reference_join_checksum
     GenState g(*spec);
+    // Zipf: build row of each shuffled slot, tabulated from the forward walk
+    // (slot_row[sp(i)] = i), so a probe hit on distinct key d pairs with rows
+    // slot_row[prefix[d] .. prefix[d+1]).
+    std::vector<uint64_t> slot_row;
+    if (spec->distribution == 2) {
+        const ShufflePerm sp(g.b_shuf, g.build_total);
+        slot_row.assign(g.build_total, 0);
+        for (uint64_t i = 0; i < g.build_total; ++i)
+            slot_row[sp(i)] = i;
+    }
     uint32_t lt[ORC_MAX_COLS], rt[ORC_MAX_COLS];
     for (uint32_t c = 0; c < ORC_MAX_COLS; ++c) {
         lt[c] = spec->build_types[c];
@@ -532,34 +540,42 @@ int orc_gen_expected_checksum(const orc_gen_spec *spec, uint32_t threads, orc_ch
             const uint64_t key = g.probeKey(j, pick);
             if (pick == ~uint64_t{0} || spec->build_rows == 0)
                 continue;
-            uint8_t *dst = row.data();
-            std::memcpy(dst, &key, 8);
-            dst += 8;
-            for (uint32_t c = 0; c < spec->build_n_cols; ++c) {
-                uint64_t lo, hi;
-                g.payload(0, c, pick, lo, hi);
-                const size_t sz = typeSize(lt[c]);
-                uint8_t v[16];
-                std::memcpy(v, &lo, 8);
-                std::memcpy(v + 8, &hi, 8);
-                std::memcpy(dst, v, sz);
-                dst += sz;
+            uint64_t y0 = 0, y1 = 1; // build rows: pick itself, or the slots of distinct key pick
+            if (spec->distribution == 2) {
+                y0 = g.prefix[pick];
+                y1 = g.prefix[pick + 1];
             }
-            for (uint32_t c = 0; c < spec->probe_n_cols; ++c) {
-                uint64_t lo, hi;
-                g.payload(1, c, j, lo, hi);
-                const size_t sz = typeSize(rt[c]);
-                uint8_t v[16];
-                std::memcpy(v, &lo, 8);
-                std::memcpy(v + 8, &hi, 8);
-                std::memcpy(dst, v, sz);
-                dst += sz;
+            for (uint64_t y = y0; y < y1; ++y) {
+                const uint64_t brow = spec->distribution == 2 ? slot_row[y] : pick;
+                uint8_t *dst = row.data();
+                std::memcpy(dst, &key, 8);
+                dst += 8;
+                for (uint32_t c = 0; c < spec->build_n_cols; ++c) {
+                    uint64_t lo, hi;
+                    g.payload(0, c, brow, lo, hi);
+                    const size_t sz = typeSize(lt[c]);
+                    uint8_t v[16];
+                    std::memcpy(v, &lo, 8);
+                    std::memcpy(v + 8, &hi, 8);
+                    std::memcpy(dst, v, sz);
+                    dst += sz;
+                }
+                for (uint32_t c = 0; c < spec->probe_n_cols; ++c) {
+                    uint64_t lo, hi;
+                    g.payload(1, c, j, lo, hi);
+                    const size_t sz = typeSize(rt[c]);
+                    uint8_t v[16];
+                    std::memcpy(v, &lo, 8);
+                    std::memcpy(v + 8, &hi, 8);
+                    std::memcpy(dst, v, sz);
+                    dst += sz;
+                }
+                uint64_t la, lc;
+                orc_row_digest(row.data(), rs, &la, &lc);
+                s.rows += 1;
+                s.lane0 += la;
+                s.lane1 += lc;
             }
-            uint64_t la, lc;
-            orc_row_digest(row.data(), rs, &la, &lc);
-            s.rows += 1;
-            s.lane0 += la;
-            s.lane1 += lc;
         }
         part[t] = s;
     };

@@ -105,8 +105,11 @@ void orc_checksum_columns(uint64_t n, const uint64_t *keys, const uint32_t *left
                           uint32_t n_left, const uint32_t *right_types, const void *const *right, uint32_t n_right,
                           orc_checksum *out);
 
-/* Checksum of the join output implied by a uniform or reference-stream
- * generator spec, computed from the closed forms (no join). */
+/* Checksum of the join output implied by a generator spec, computed from the
+ * closed forms (no join): uniform and reference streams pair each matching
+ * probe row with the build row it was drawn from; Zipf pairs a probe row that
+ * drew distinct key d with every build row of d (the rows whose shuffled
+ * slots are d's prefix range, found from a tabulated forward shuffle). */
 int orc_gen_expected_checksum(const orc_gen_spec *spec, uint32_t threads, orc_checksum *out);
 
 #ifdef __cplusplus

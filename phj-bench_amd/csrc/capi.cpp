@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -69,6 +70,9 @@ public:
 
     void *alloc(uint64_t bytes) {
         const uint64_t sz = (std::max<uint64_t>(bytes, 1) + kAlign - 1) & ~(kAlign - 1);
+        if (limit_ != 0 && current_ + sz > limit_)
+            throw PhjError(PHJ_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(sz) +
+                                                      " bytes exceeds the context's memory limit");
         Block *b = take(sz);
         if (b == nullptr)
             b = grow(sz);
@@ -102,6 +106,14 @@ public:
             release_cached();
     }
 
+    // free() for cleanup paths that must not throw (destructors during unwinding).
+    void free_quiet(void *p) noexcept {
+        try {
+            free(p);
+        } catch (...) {
+        }
+    }
+
     bool owns(const void *p) const { return live_.count(const_cast<void *>(p)) != 0; }
 
     // Returns segments with no live block to the device (after the stream's
@@ -127,6 +139,8 @@ public:
     uint64_t current() const { return current_; }
     uint64_t peak() const { return peak_; }
     void reset_peak() { peak_ = current_; }
+    // Upper bound on the bytes held by live blocks (0: none).
+    void set_limit(uint64_t bytes) { limit_ = bytes; }
 
 private:
     static constexpr uint64_t kAlign = 256;
@@ -210,6 +224,7 @@ private:
     std::unordered_map<char *, uint64_t> segments_;
     uint64_t current_ = 0, peak_ = 0, free_bytes_ = 0;
     uint64_t cache_limit_ = 64ull << 30;
+    uint64_t limit_ = 0;
 };
 
 // ---------------------------------------------------------------- kernel timer
@@ -318,7 +333,7 @@ struct Scratch {
     explicit Scratch(phj_ctx *c) : ctx(c) {}
     ~Scratch() {
         for (void *p : bufs)
-            ctx->arena.free(p);
+            ctx->arena.free_quiet(p);
     }
     template <class T> T *get(uint64_t n) {
         T *p = ctx->alloc<T>(std::max<uint64_t>(n, 1));
@@ -331,6 +346,23 @@ struct Scratch {
             ctx->arena.free(p);
             bufs.erase(it);
         }
+    }
+};
+
+// Device blocks one call holds beyond its scope's Scratch (pass outputs that
+// outlive the function making them, tables kept across slices): returned to
+// the context's pool when the call unwinds, unless their ownership has moved
+// to a result first (clear()).
+struct Held : std::vector<void *> {
+    phj_ctx *ctx;
+    explicit Held(phj_ctx *c) : ctx(c) {}
+    Held(const Held &) = delete;
+    Held &operator=(const Held &) = delete;
+    ~Held() { free_all(); }
+    void free_all() noexcept {
+        for (void *p : *this)
+            ctx->arena.free_quiet(p);
+        clear();
     }
 };
 
@@ -458,7 +490,7 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
     }
     const uint64_t *cur_keys = in.keys;
     ColSet cur_cols = in.cols;
-    std::vector<void *> prev_bufs; // buffers of the previous pass output
+    Held prev_bufs(ctx); // buffers of the previous pass output
     Partitioned out;
     out.rows = rows;
     uint32_t *cur_seg = seg;
@@ -469,7 +501,7 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
         const uint32_t shift = 64 - consumed;
         const uint64_t B = 1ull << bits;
         const uint64_t out_nseg = static_cast<uint64_t>(cur_nseg) * B;
-        std::vector<void *> bufs;
+        Held bufs(ctx);
         uint64_t *nkeys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
         bufs.push_back(nkeys);
         OutColSet ncols = alloc_cols(ctx, cur_cols, rows, bufs);
@@ -483,9 +515,8 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
         radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, uniform_bits(shift, bits), nseg, rs,
                    flags, ctx->tm());
         // previous pass output is dead once this pass has consumed it
-        for (void *b : prev_bufs)
-            ctx->arena.free(b);
-        prev_bufs = bufs;
+        prev_bufs.free_all();
+        prev_bufs.swap(bufs);
         cur_keys = nkeys;
         cur_cols = as_in(ncols);
         cur_seg = nseg;
@@ -497,19 +528,53 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
     }
     for (void *b : prev_bufs)
         keep.push_back(b);
+    prev_bufs.clear();
     return out;
 }
 
-// Output buffers of a join result (capacity rows).
+// Output buffers of a join result (capacity rows). They own their blocks:
+// an unwinding call returns them to the context's pool; own_result() moves
+// them to a result handle.
 struct OutBuffers {
+    phj_ctx *ctx = nullptr;
     uint64_t cap = 0;
     uint64_t *keys = nullptr;
     OutColSet left{}, right{};
     std::vector<void *> bufs;
+
+    OutBuffers() = default;
+    OutBuffers(const OutBuffers &) = delete;
+    OutBuffers &operator=(const OutBuffers &) = delete;
+    OutBuffers(OutBuffers &&o) noexcept { take(o); }
+    OutBuffers &operator=(OutBuffers &&o) noexcept {
+        if (this != &o) {
+            release();
+            take(o);
+        }
+        return *this;
+    }
+    ~OutBuffers() { release(); }
+    void release() noexcept {
+        for (void *p : bufs)
+            ctx->arena.free_quiet(p);
+        bufs.clear();
+    }
+
+private:
+    void take(OutBuffers &o) noexcept {
+        ctx = o.ctx;
+        cap = o.cap;
+        keys = o.keys;
+        left = o.left;
+        right = o.right;
+        bufs = std::move(o.bufs);
+        o.bufs.clear();
+    }
 };
 
 OutBuffers alloc_output(phj_ctx *ctx, const ColSet &l, const ColSet &r, uint64_t cap) {
     OutBuffers o;
+    o.ctx = ctx;
     o.cap = cap;
     o.keys = ctx->alloc<uint64_t>(std::max<uint64_t>(cap, 1));
     o.bufs.push_back(o.keys);
@@ -518,11 +583,7 @@ OutBuffers alloc_output(phj_ctx *ctx, const ColSet &l, const ColSet &r, uint64_t
     return o;
 }
 
-void free_output(phj_ctx *ctx, OutBuffers &o) {
-    for (void *p : o.bufs)
-        ctx->arena.free(p);
-    o.bufs.clear();
-}
+void free_output(phj_ctx *, OutBuffers &o) { o.release(); }
 
 OutSink sink_of(const OutBuffers &o, unsigned long long *cursor) {
     OutSink s{};
@@ -937,7 +998,7 @@ template <class F> uint64_t join_with_capacity(phj_ctx *ctx, const ColSet &l, co
 // started by the caller; `ebs` brackets the build shuffle (null: resident).
 void phj_pure_probe(phj_ctx *ctx, const Partitioned &RP, const ColSet &r_cols, const DevRel &S,
                     const std::vector<uint32_t> &passes, const phj_tuning &tun, EventPair &e2e, EventPair *ebs,
-                    std::vector<void *> &keep, phj_join_result *res) {
+                    Held &keep, phj_join_result *res) {
     const LeafJoinPlan plan = plan_of(tun);
     EventPair eps, ej;
     HIPCHK(hipEventRecord(eps.a, ctx->stream));
@@ -959,8 +1020,7 @@ void phj_pure_probe(phj_ctx *ctx, const Partitioned &RP, const ColSet &r_cols, c
     HIPCHK(hipEventRecord(ej.b, ctx->stream));
     HIPCHK(hipEventRecord(e2e.b, ctx->stream));
     ctx->sync();
-    for (void *k : keep)
-        ctx->arena.free(k);
+    keep.free_all();
     set_phase(res->build_shuffle, ebs ? ebs->ms() : 0.0, RP.rows);
     set_phase(res->probe_shuffle, eps.ms(), S.rows);
     // Leaf build and probe run fused in one kernel; the fused time is
@@ -977,9 +1037,17 @@ void phj_pure_probe(phj_ctx *ctx, const Partitioned &RP, const ColSet &r_cols, c
 // maximum; {4,4} on 2^27 build rows then joins like {8,8}. An explicit LDS
 // leaf limit or forced HBM tables keep the configuration as given.
 constexpr uint64_t kJoinLeafRows = 2048;
-constexpr uint32_t kDeviceMaxTotalBits = 24;
+// Joins run at most 2^21 leaves: enough for ~2,048-row leaves at the 2^32-row
+// relation limit, and every leaf-join grid (work items = leaves + probe
+// chunks, up to 1,024 threads each) stays below HIP's 2^32 work-items per
+// launch dimension.
+constexpr uint32_t kDeviceMaxTotalBits = 21;
 
+// Configurations of 25..32 total bits, which the reference accepts
+// (src/CLI.cpp:350-355, src/RadixPartition.cpp:556-557), join through the
+// device maximum of 24 bits: the output is the same for any leaf split.
 uint32_t join_total_bits(uint32_t requested, uint64_t build_rows, const phj_tuning &tun) {
+    requested = std::min(requested, kDeviceMaxTotalBits);
     if (tun.lds_leaf_rows != 0 || (tun.flags & PHJ_TUNE_FORCE_GLOBAL_HT) != 0)
         return requested;
     uint32_t b = requested;
@@ -1003,7 +1071,7 @@ void run_phj_pure(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, co
     EventPair e2e, ebs;
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
     HIPCHK(hipEventRecord(ebs.a, ctx->stream));
-    std::vector<void *> keep;
+    Held keep(ctx);
     const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
     phj_pure_probe(ctx, RP, R.cols, S, passes, tun, e2e, &ebs, keep, res);
@@ -1075,6 +1143,9 @@ void run_chj(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, phj_joi
 // equivalent): probes against it skip the build shuffle.
 struct phj_resident {
     phj_ctx *owner = nullptr;
+    // Probes running against this build side (calls on any context); release
+    // is refused while it is nonzero.
+    std::atomic<uint32_t> inflight{0};
     Partitioned RP;
     ColSet r_cols{}; // column types (pointers unused)
     std::vector<uint32_t> passes;
@@ -1110,7 +1181,7 @@ void probe_resident(phj_ctx *ctx, const phj_resident &r, const phj_relation &p, 
     ctx->arena.reset_peak();
     EventPair e2e;
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
-    std::vector<void *> keep;
+    Held keep(ctx);
     phj_pure_probe(ctx, r.RP, r.r_cols, S, r.passes, tun, e2e, nullptr, keep, res);
 }
 
@@ -1179,7 +1250,7 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
                  uint64_t budget_bytes, const phj_tuning &tun, phj_join_result *res) {
     validate_rel(&b);
     validate_rel(&p);
-    const uint32_t total = validate_pass_bits(pb, np);
+    const uint32_t total = std::min(validate_pass_bits(pb, np), kDeviceMaxTotalBits);
     const std::vector<uint32_t> passes = device_passes(total);
     std::memset(res, 0, sizeof(*res));
     Scratch scr(ctx);
@@ -1196,7 +1267,7 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     EventPair e2e, ebs;
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
     HIPCHK(hipEventRecord(ebs.a, ctx->stream));
-    std::vector<void *> keep;
+    Held keep(ctx);
     const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
 
@@ -1261,8 +1332,7 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
         probe_ms += join_ev[i]->ms() - join_build_ms[i];
         big_build_ms += join_build_ms[i];
     }
-    for (void *k : keep)
-        ctx->arena.free(k);
+    keep.free_all();
     set_phase(res->build_shuffle, ebs.ms(), R.rows);
     set_phase(res->build, big_build_ms, R.rows);
     set_phase(res->probe_shuffle, shuffle_ms, S.rows);
@@ -1343,7 +1413,7 @@ Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t
     ColSet cur_cols = in.cols;
     const uint32_t *cur_seg = seg_off;
     uint32_t cur_nseg = n_joins;
-    std::vector<void *> prev_bufs;
+    Held prev_bufs(ctx);
     Partitioned out;
     out.rows = rows;
     for (size_t p = 0; p < bits.size(); ++p) {
@@ -1371,7 +1441,7 @@ Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t
         HIPCHK(hipMemcpyAsync(dsb, sb.data(), sb.size() * 2, hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipMemcpyAsync(dfirst, first.data(), first.size() * 4, hipMemcpyHostToDevice, ctx->stream));
         const uint64_t B = 1ull << maxb;
-        std::vector<void *> bufs;
+        Held bufs(ctx);
         uint64_t *nkeys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
         bufs.push_back(nkeys);
         OutColSet ncols = alloc_cols(ctx, cur_cols, rows, bufs);
@@ -1383,9 +1453,8 @@ Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t
         rs.block_sums = scr.get<uint32_t>(radix_max_chunks(rows, cur_nseg) * B);
         radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, PassBits{dsb, dfirst, 0, maxb},
                    nseg, rs, flags, ctx->tm());
-        for (void *b : prev_bufs)
-            ctx->arena.free(b);
-        prev_bufs = bufs;
+        prev_bufs.free_all();
+        prev_bufs.swap(bufs);
         cur_keys = nkeys;
         cur_cols = as_in(ncols);
         cur_seg = nseg;
@@ -1398,6 +1467,7 @@ Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t
     }
     for (void *b : prev_bufs)
         keep.push_back(b);
+    prev_bufs.clear();
     leaf_join.swap(join_of);
     return out;
 }
@@ -1450,7 +1520,7 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
     EventPair e2e, ebs, eps, ej;
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
     HIPCHK(hipEventRecord(ebs.a, ctx->stream));
-    std::vector<void *> keep;
+    Held keep(ctx);
     std::vector<uint32_t> leaf_join;
     LeafJoinArgs a{};
     a.r_cols = R.cols;
@@ -1533,8 +1603,7 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
     HIPCHK(hipMemcpyAsync(h_cur.data(), cur, m * 8ull, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(h_flag.data(), dflag, m * 4ull, hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
-    for (void *k : keep)
-        ctx->arena.free(k);
+    keep.free_all();
     const double t_bs = ebs.ms(), t_ps = eps.ms(), t_j = ej.ms(), t_e2e = e2e.ms();
     const uint64_t peak = ctx->arena.peak();
     auto bufs = std::make_shared<std::vector<void *>>(out.bufs);
@@ -1581,7 +1650,9 @@ void run_phj_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *d, phj_join_re
         validate_rel(&d[i].probe);
         validate_pass_bits(d[i].pass_bits, d[i].n_passes);
         std::memset(&res[i], 0, sizeof(res[i]));
-        batchable = batchable && d[i].build.on_device && same_schema(d[0].build, d[i].build) &&
+        // a group reads its relations in place: both sides must be in device memory
+        // (host relations run on the plain path, which stages them)
+        batchable = batchable && d[i].build.on_device && d[i].probe.on_device && same_schema(d[0].build, d[i].build) &&
                     same_schema(d[0].probe, d[i].probe) && !(d[i].tuning.flags & PHJ_TUNE_TWO_PHASE_JOIN);
     }
     uint32_t i = 0;
@@ -1638,6 +1709,7 @@ void fill_partitioned(phj_ctx *ctx, const Partitioned &P, std::vector<void *> &k
     }
     const uint64_t h = ctx->next_handle++;
     ctx->owned[h] = keep;
+    keep.clear(); // owned by the handle now
     out->handle = h;
     ctx->sync();
 }
@@ -1723,6 +1795,8 @@ GenParams make_gen_params(phj_ctx *ctx, const phj_gen_spec &s, Scratch &scr) {
         g.sh_o1 = rand64(sb, 10) | 1;
         g.sh_o2 = rand64(sb, 11) | 1;
         g.sh_c1 = rand64(sb, 12);
+        g.sh_i1 = inverse_odd(g.sh_o1);
+        g.sh_i2 = inverse_odd(g.sh_o2);
     }
     return g;
 }
@@ -1767,6 +1841,9 @@ template <class F> phj_status guard(phj_ctx *ctx, F &&f) {
     } catch (const PhjError &e) {
         ctx->err = e.what();
         return e.code;
+    } catch (const LaunchTooLarge &e) {
+        ctx->err = e.what();
+        return PHJ_ERR_UNSUPPORTED;
     } catch (const std::bad_alloc &) {
         ctx->err = "host allocation failed";
         return PHJ_ERR_OUT_OF_MEMORY;
@@ -1864,6 +1941,10 @@ phj_status phj_reset_peak(phj_ctx *ctx) {
     return guard(ctx, [&] { ctx->arena.reset_peak(); });
 }
 
+phj_status phj_set_mem_limit(phj_ctx *ctx, uint64_t bytes) {
+    return guard(ctx, [&] { ctx->arena.set_limit(bytes); });
+}
+
 phj_status phj_int_hash64_batch(phj_ctx *ctx, const uint64_t *in, uint64_t n, uint64_t *out, int on_device) {
     return guard(ctx, [&] {
         if (n && (in == nullptr || out == nullptr))
@@ -1893,7 +1974,7 @@ phj_status phj_radix_shuffle(phj_ctx *ctx, const phj_relation *input, const uint
         const uint32_t total = validate_pass_bits(pass_bits, n_passes);
         Scratch scr(ctx);
         const DevRel R = stage_relation(ctx, scr, *input);
-        std::vector<void *> keep;
+        Held keep(ctx);
         const Partitioned P = radix_partition(ctx, R, nullptr, 1, 0, device_passes(total), keep, "shuffle");
         fill_partitioned(ctx, P, keep, out);
     });
@@ -1921,7 +2002,7 @@ phj_status phj_refine_to_leaves(phj_ctx *ctx, const phj_partitioned *pass1, cons
         uint32_t *off32 = scr.get<uint32_t>(pass1->parts + 1);
         k_u64_to_u32<<<static_cast<uint32_t>((pass1->parts + 1 + 255) / 256), 256, 0, ctx->stream>>>(pass1->offsets, off32,
                                                                                                      pass1->parts + 1);
-        std::vector<void *> keep;
+        Held keep(ctx);
         Partitioned P;
         if (n_passes == 1) {
             // One pass: the pass-1 buckets are the leaves (src/RadixPartition.cpp:437-447); copy them.
@@ -2058,6 +2139,12 @@ phj_status phj_probe_registered(phj_ctx *ctx, const phj_resident *build, const p
         if (build->owner->device != ctx->device)
             invalid("resident build side lives on another device");
         const phj_tuning t = tuning ? *tuning : phj_tuning{};
+        auto *r = const_cast<phj_resident *>(build);
+        struct InFlight {
+            phj_resident *r;
+            explicit InFlight(phj_resident *x) : r(x) { r->inflight.fetch_add(1, std::memory_order_acq_rel); }
+            ~InFlight() { r->inflight.fetch_sub(1, std::memory_order_acq_rel); }
+        } counted(r);
         probe_resident(ctx, *build, *probe, t, result);
     });
 }
@@ -2068,6 +2155,8 @@ phj_status phj_build_release(phj_ctx *ctx, phj_resident *build) {
             return;
         if (build->owner != ctx)
             invalid("release a resident build side through the context that registered it");
+        if (build->inflight.load(std::memory_order_acquire) != 0)
+            invalid("resident build side has probes in flight; release it after they return");
         for (void *b : build->bufs)
             ctx->arena.free(b);
         delete build;
@@ -2264,8 +2353,6 @@ phj_status phj_gen_expected_checksum(phj_ctx *ctx, const phj_gen_spec *spec, phj
     return guard(ctx, [&] {
         if (spec == nullptr || out == nullptr)
             invalid("null argument");
-        if (spec->distribution == 2)
-            throw PhjError(PHJ_ERR_UNSUPPORTED, "closed-form checksum needs distinct build keys");
         Scratch scr(ctx);
         const GenParams g = make_gen_params(ctx, *spec, scr);
         unsigned long long *acc = scr.get<unsigned long long>(3);

@@ -51,8 +51,9 @@ struct GenParams {
     uint64_t b_sel, b_pick, b_miss, b_rank;
     uint64_t b_bpay[kMaxCols], b_bpayhi[kMaxCols], b_ppay[kMaxCols], b_ppayhi[kMaxCols];
     uint64_t m1, m2, m3; // 63-bit bijection multipliers
-    // cycle-walking shuffle of the Zipf build rows
-    uint64_t sh_mask, sh_o1, sh_o2, sh_c1;
+    // cycle-walking shuffle of the Zipf build rows (sh_i1, sh_i2: inverses of
+    // sh_o1, sh_o2 modulo 2^64, for the inverse walk of the closed form)
+    uint64_t sh_mask, sh_o1, sh_o2, sh_c1, sh_i1, sh_i2;
     uint32_t sh_r;
     uint32_t n_rank;
     const uint64_t *prefix; // Zipf: distinct key d owns shuffled slots [prefix[d], prefix[d+1])
@@ -76,6 +77,25 @@ __host__ __device__ __forceinline__ uint64_t shuffle_step(const GenParams &g, ui
     x ^= x >> g.sh_r;
     x = (x * g.sh_o2) & g.sh_mask;
     x ^= x >> g.sh_r;
+    return x;
+}
+
+// Inverse of shuffle_step on [0, 2^m): x ^= x >> r is its own inverse when
+// 2r >= m (the shifted bits never reach the low half twice), and an odd
+// multiplier is inverted by its inverse modulo 2^64.
+__host__ __device__ __forceinline__ uint64_t shuffle_step_inv(const GenParams &g, uint64_t x) {
+    x ^= x >> g.sh_r;
+    x = (x * g.sh_i2) & g.sh_mask;
+    x ^= x >> g.sh_r;
+    return ((x - g.sh_c1) * g.sh_i1) & g.sh_mask;
+}
+
+// Inverse of an odd a modulo 2^64 by Newton's iteration (a * a == 1 mod 8,
+// and each step doubles the number of correct low bits: 3, 6, 12, 24, 48, 96).
+__host__ __device__ __forceinline__ uint64_t inverse_odd(uint64_t a) {
+    uint64_t x = a;
+    for (int i = 0; i < 5; ++i)
+        x *= 2 - a * x;
     return x;
 }
 

@@ -77,6 +77,42 @@ __global__ __launch_bounds__(256) void k_gen(GenParams g, int side, uint64_t row
     }
 }
 
+// Build row whose shuffled slot is y (Zipf): the inverse cycle walk of
+// gen_build_key's forward walk (the forward walk steps until it lands below
+// build_total, so the inverse steps back until it does).
+__device__ __forceinline__ uint64_t zipf_row_of_slot(const GenParams &g, uint64_t y) {
+    if (g.sh_mask == 0)
+        return 0;
+    uint64_t x = shuffle_step_inv(g, y);
+    while (x >= g.build_total)
+        x = shuffle_step_inv(g, x);
+    return x;
+}
+
+__device__ __forceinline__ void digest_row(const GenParams &g, uint32_t row_size, uint64_t key, uint64_t build_row,
+                                           uint64_t j, uint64_t &s0, uint64_t &s1) {
+    Packer p(row_size);
+    p.push_le8(key, 8);
+    for (uint32_t c = 0; c < g.n_build_cols; ++c) {
+        uint64_t lo, hi;
+        gen_payload(g, 0, c, build_row, lo, hi);
+        p.push(lo, hi, type_size(g.build_types[c]));
+    }
+    for (uint32_t c = 0; c < g.n_probe_cols; ++c) {
+        uint64_t lo, hi;
+        gen_payload(g, 1, c, j, lo, hi);
+        p.push(lo, hi, type_size(g.probe_types[c]));
+    }
+    p.finish();
+    s0 += p.d.a;
+    s1 += p.d.c;
+}
+
+// Order-independent checksum of the exact join output the generator implies.
+// Uniform and reference streams: each matching probe row pairs with the one
+// build row it was drawn from. Zipf: a probe row that drew distinct key d
+// pairs with the k_d build rows of d, the rows whose shuffled slots are
+// [prefix[d], prefix[d+1]).
 __global__ __launch_bounds__(256) void k_gen_checksum(GenParams g, uint32_t row_size, unsigned long long *__restrict__ acc) {
     __shared__ unsigned long long part[3][4];
     uint64_t s0 = 0, s1 = 0, cnt = 0;
@@ -86,21 +122,14 @@ __global__ __launch_bounds__(256) void k_gen_checksum(GenParams g, uint32_t row_
         const uint64_t key = gen_probe_key(g, j, pick);
         if (pick == ~0ull || g.build_rows == 0)
             continue;
-        Packer p(row_size);
-        p.push_le8(key, 8);
-        for (uint32_t c = 0; c < g.n_build_cols; ++c) {
-            uint64_t lo, hi;
-            gen_payload(g, 0, c, pick, lo, hi);
-            p.push(lo, hi, type_size(g.build_types[c]));
+        if (g.dist == 2) {
+            const uint64_t y0 = g.prefix[pick], y1 = g.prefix[pick + 1];
+            for (uint64_t y = y0; y < y1; ++y)
+                digest_row(g, row_size, key, zipf_row_of_slot(g, y), j, s0, s1);
+            cnt += y1 - y0;
+            continue;
         }
-        for (uint32_t c = 0; c < g.n_probe_cols; ++c) {
-            uint64_t lo, hi;
-            gen_payload(g, 1, c, j, lo, hi);
-            p.push(lo, hi, type_size(g.probe_types[c]));
-        }
-        p.finish();
-        s0 += p.d.a;
-        s1 += p.d.c;
+        digest_row(g, row_size, key, pick, j, s0, s1);
         cnt += 1;
     }
     s0 = wave_sum_u64(s0);

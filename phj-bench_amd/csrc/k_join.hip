@@ -1436,7 +1436,7 @@ void leaf_work_plan(hipStream_t s, const LeafJoinArgs &a, uint32_t *work_first, 
 template <int T, int U> static void launch_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, size_t lds) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_probe<T, U>), hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(lds));
-    k_leaf_probe<T, U><<<grid8(max_work), T, lds, s>>>(a);
+    k_leaf_probe<T, U><<<checked_grid(grid8(max_work), T, "leaf_probe"), T, lds, s>>>(a);
 }
 
 void leaf_probe(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, uint32_t threads, KernelTimer *tm) {
@@ -1487,7 +1487,7 @@ void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, Ke
     TimedScope ts(tm, "leaf_join", s);
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    k_leaf_join<512, kFusedRowsPerThread, 2><<<grid8(max_work), 512, lds, s>>>(a);
+    k_leaf_join<512, kFusedRowsPerThread, 2><<<checked_grid(grid8(max_work), 512, "leaf_join"), 512, lds, s>>>(a);
 }
 
 void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm) {
@@ -1498,7 +1498,7 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
         TimedScope ts(tm, "leaf_emit", s);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<256, 8, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_chains));
-        k_leaf_emit<256, 8, false><<<grid8(max_work), 256, lds_chains, s>>>(a);
+        k_leaf_emit<256, 8, false><<<checked_grid(grid8(max_work), 256, "leaf_emit"), 256, lds_chains, s>>>(a);
     }
     if (chains) {
         TimedScope ts(tm, "leaf_emit_chains", s);
@@ -1512,7 +1512,7 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
         const size_t lds = lds_chains + 8 * 256 * (4 + 2) + (b.r_lds ? rbytes : 0);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<256, 8, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        k_leaf_emit<256, 8, true><<<grid8(max_work), 256, lds, s>>>(b);
+        k_leaf_emit<256, 8, true><<<checked_grid(grid8(max_work), 256, "leaf_emit"), 256, lds, s>>>(b);
     }
 }
 

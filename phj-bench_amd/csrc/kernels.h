@@ -5,9 +5,25 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <stdexcept>
+#include <string>
+
 #include "common.h"
 
 namespace phjd {
+
+// A launch whose grid would exceed 2^32 - 1 work-items in x (HIP's limit).
+struct LaunchTooLarge : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+// Grid of `blocks` workgroups of `threads`, refused on the host when the
+// launch would exceed that limit.
+inline uint32_t checked_grid(uint64_t blocks, uint32_t threads, const char *what) {
+    if (blocks * threads > 0xFFFFFFFFull)
+        throw LaunchTooLarge(std::string(what) + ": " + std::to_string(blocks) + " workgroups of " +
+                             std::to_string(threads) + " threads exceed 2^32 work-items");
+    return static_cast<uint32_t>(blocks);
+}
 
 // Optional per-launch timing: events recorded around each named launch on
 // the launch stream, resolved after the stream synchronises.

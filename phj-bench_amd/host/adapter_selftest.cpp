@@ -4,17 +4,22 @@
 // key | build cols | probe cols per row (src/Reference.cpp:79-110), sorted, and
 // written for the caller to compare against its oracle.
 //
-// usage: adapter_selftest <input.bin> <output.bin>
+// usage: adapter_selftest <input.bin> <output.bin> [callers]
+//   callers: run the join from this many threads at once (default 1); every
+//            caller's sorted rows must equal the first caller's, which are
+//            written out
 //   input:  u64 magic, u32 scheme, u32 n_passes, u8 pass_bits[8], u64 budget_mib,
 //           u64 threads, then build and probe sides: u64 rows, u32 n_cols,
 //           u32 types[n_cols], u64 keys[rows], column bytes in order
 //   output: u64 rows, u64 row_size, rows * row_size bytes (sorted)
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "phj_schemes.h"
@@ -77,14 +82,33 @@ std::vector<std::string> serialize(const JoinOutput &out) {
     return rows;
 }
 
+// One call of the chosen scheme, as the reference's driver makes it.
+JoinOutput runScheme(uint32_t scheme, const BlockStream &build, const BlockStream &probe, const RadixConfig &cfg,
+                     uint64_t threads, uint64_t budget, double &e2e) {
+    if (scheme == 1) {
+        ChjResult r = runCHJ(build, probe, threads);
+        e2e = r.e2e_wall_ms;
+        return std::move(r.output);
+    }
+    if (scheme == 3) {
+        PhjResult r = runPHJ(build, probe, cfg, threads);
+        e2e = r.e2e_wall_ms;
+        return std::move(r.output);
+    }
+    PhjBepResult r = runPhjBep(build, probe, cfg, threads, budget);
+    e2e = r.e2e_wall_ms;
+    return std::move(r.output);
+}
+
 } // namespace
 
 int main(int argc, char **argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s <input.bin> <output.bin>\n", argv[0]);
+    if (argc != 3 && argc != 4) {
+        std::fprintf(stderr, "usage: %s <input.bin> <output.bin> [callers]\n", argv[0]);
         return 2;
     }
     try {
+        const int callers = argc == 4 ? std::max(1, std::atoi(argv[3])) : 1;
         std::ifstream in(argv[1], std::ios::binary);
         if (rd<uint64_t>(in) != 0x31464c45534a4850ull) // "PHJSELF1"
             throw std::runtime_error("bad magic");
@@ -98,31 +122,50 @@ int main(int argc, char **argv) {
         const BlockStream probe = readSide(in);
         RadixConfig cfg;
         cfg.pass_bits.assign(pb, pb + npass);
-        JoinOutput output;
-        double e2e = 0;
-        if (scheme == 1) {
-            ChjResult r = runCHJ(build, probe, threads);
-            e2e = r.e2e_wall_ms;
-            output = std::move(r.output);
-        } else if (scheme == 3) {
-            PhjResult r = runPHJ(build, probe, cfg, threads);
-            e2e = r.e2e_wall_ms;
-            output = std::move(r.output);
+        // every caller starts its call together, each on its own thread
+        std::vector<std::vector<std::string>> rows(callers);
+        std::vector<size_t> blocks(callers, 0);
+        std::vector<double> e2e(callers, 0.0);
+        std::vector<std::string> errors(callers);
+        std::vector<int> invalid(callers, 0);
+        auto call = [&](int t) {
+            try {
+                const JoinOutput output = runScheme(scheme, build, probe, cfg, threads, budget, e2e[t]);
+                blocks[t] = output.workers.empty() ? size_t{0} : output.workers[0].blocks.size();
+                rows[t] = serialize(output);
+            } catch (const std::invalid_argument &e) {
+                invalid[t] = 1;
+                errors[t] = e.what();
+            } catch (const std::exception &e) {
+                errors[t] = e.what();
+            }
+        };
+        if (callers == 1) {
+            call(0);
         } else {
-            PhjBepResult r = runPhjBep(build, probe, cfg, threads, budget);
-            e2e = r.e2e_wall_ms;
-            output = std::move(r.output);
+            std::vector<std::thread> ts;
+            for (int t = 0; t < callers; ++t)
+                ts.emplace_back(call, t);
+            for (auto &t : ts)
+                t.join();
         }
-        const std::vector<std::string> rows = serialize(output);
+        for (int t = 0; t < callers; ++t) {
+            if (invalid[t])
+                throw std::invalid_argument(errors[t]);
+            if (!errors[t].empty())
+                throw std::runtime_error("caller " + std::to_string(t) + ": " + errors[t]);
+            if (rows[t] != rows[0])
+                throw std::runtime_error("caller " + std::to_string(t) + " returned other rows than caller 0");
+        }
         const uint64_t row_size = 8 + build.schema.rowByteSize() + probe.schema.rowByteSize();
         std::ofstream out(argv[2], std::ios::binary);
-        const uint64_t n = rows.size();
+        const uint64_t n = rows[0].size();
         out.write(reinterpret_cast<const char *>(&n), 8);
         out.write(reinterpret_cast<const char *>(&row_size), 8);
-        for (const auto &r : rows)
+        for (const auto &r : rows[0])
             out.write(r.data(), static_cast<std::streamsize>(r.size()));
-        std::printf("{\"out_rows\": %llu, \"e2e_wall_ms\": %.4f, \"blocks\": %zu}\n", static_cast<unsigned long long>(n), e2e,
-                    output.workers.empty() ? size_t{0} : output.workers[0].blocks.size());
+        std::printf("{\"out_rows\": %llu, \"e2e_wall_ms\": %.4f, \"blocks\": %zu, \"callers\": %d}\n",
+                    static_cast<unsigned long long>(n), e2e[0], blocks[0], callers);
         return 0;
     } catch (const std::invalid_argument &e) {
         std::fprintf(stderr, "invalid_argument: %s\n", e.what());

@@ -1,6 +1,6 @@
 // phj_types.h — the reference's data model for drop-in C++ callers.
 //
-// Same names, field order and widths as src/Types.h:11-108, src/Block.h:131-274
+// Same names, field order and widths as src/Types.h:11-108, src/Block.h:21-164
 // and src/JoinOutput.h:18-84 (std::vector instead of std::pmr; no allocator
 // plumbing), so code written against runCHJ/runPHJ/runPhjBep compiles against
 // this header and the GPU adapter in phj_schemes.h.

@@ -164,6 +164,7 @@ _SIGNATURES = {
     "phj_memcpy_d2h": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
     "phj_mem_stats": ([C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "phj_reset_peak": ([C.c_void_p], C.c_int),
+    "phj_set_mem_limit": ([C.c_void_p, C.c_uint64], C.c_int),
     "phj_int_hash64_batch": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int], C.c_int),
     "phj_radix_shuffle": ([C.c_void_p, C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.POINTER(_Partitioned)], C.c_int),
     "phj_refine_to_leaves": ([C.c_void_p, C.POINTER(_Partitioned), C.c_void_p, C.c_uint32, C.POINTER(_Partitioned)],
@@ -652,6 +653,10 @@ class Context:
 
     def synchronize(self):
         _raise(self._p, load_library().phj_synchronize(self._p))
+
+    def set_mem_limit(self, nbytes):
+        """Cap the device bytes this context's live allocations may hold (0: no cap)."""
+        _raise(self._p, load_library().phj_set_mem_limit(self._p, int(nbytes)))
 
     # -- primitives
     def intHash64Batch(self, keys) -> np.ndarray:

@@ -34,6 +34,7 @@ class RequestInput:
     build_rows: int
     probe_rows: int
     footprint: int  # bytes reserved while in flight
+    spec: object = None  # generator spec of (build, probe): its closed-form checksum is the exact result
 
 
 @dataclass
@@ -80,7 +81,7 @@ def request_pool(ctx, seed=42, n_new=32, n_resident=64, n_resident_probes=16, lo
                           match_rate=float(rng.uniform(match_lo, match_hi)), build_types=bt, probe_types=pt)
         b, p = ctx.generate(spec)
         kinds.append(RequestInput(b, None, p, G.RadixConfig.for_build_rows(r_rows), r_rows, s_rows,
-                                  footprint(r_rows, s_rows, bt, pt, False)))
+                                  footprint(r_rows, s_rows, bt, pt, False), spec))
     residents = []
     for i in range(n_resident):
         r_rows = int(2.0 ** rng.uniform(lo_log2, hi_log2))
@@ -102,8 +103,14 @@ def request_pool(ctx, seed=42, n_new=32, n_resident=64, n_resident_probes=16, lo
         b, p = ctx.generate(spec)
         b.release()
         kinds.append(RequestInput(None, res, p, G.RadixConfig.for_build_rows(r_rows), r_rows, s_rows,
-                                  footprint(r_rows, s_rows, bt, pt, True)))
+                                  footprint(r_rows, s_rows, bt, pt, True), spec))
     return kinds, [r[0] for r in residents], n_new
+
+
+def expected(ctx, kind: RequestInput):
+    """(out_rows, checksum) of the request's exact result, from the generator's closed form."""
+    cs = ctx.expected_checksum(kind.spec)
+    return cs[0], cs
 
 
 def run_request(ctx, kind: RequestInput, want_checksum=False):

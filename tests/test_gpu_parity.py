@@ -110,21 +110,33 @@ def test_radix_shuffle_rejects_bad_config(ctx):
             ctx.radixShuffle(rel, G.RadixConfig(bad))
 
 
-@pytest.mark.parametrize("pass_bits", [[4, 4], [6, 5], [3, 3, 3], [8]])
+@pytest.mark.parametrize("pass_bits", [[4, 4], [6, 5], [3, 3, 3], [8], [8, 8]])
 def test_refine_to_leaves(ctx, pass_bits):
+    """Pass-1 buckets refined to leaves (src/RadixPartition.cpp:417-539): every
+    row sits in the leaf of its top hash bits as the oracle's intHash64 gives
+    them, and each leaf holds exactly the input rows with that leaf id."""
     rng = np.random.default_rng(sum(pass_bits))
-    rel = random_relation(rng, 50_000, [G.U16, G.U64])
+    rel = random_relation(rng, 50_000, [G.U16, G.U64, G.U128])
+    total = sum(pass_bits)
     p1 = ctx.radixShuffle(host_rel(rel), G.RadixConfig(pass_bits[:1]))
     leaves = ctx.refineToLeaves(p1, pass_bits)
-    direct = ctx.radixShuffle(host_rel(rel), G.RadixConfig(pass_bits))
-    a, offa = leaves.to_host()
-    b, offb = direct.to_host()
-    assert np.array_equal(offa, offb)
-    for L in range(0, leaves.parts, max(1, leaves.parts // 64)):
-        ia = np.arange(int(offa[L]), int(offa[L + 1]))
-        ib = np.arange(int(offb[L]), int(offb[L + 1]))
-        assert np.array_equal(rows_multiset(a, ia), rows_multiset(b, ib))
-    for x in (p1, leaves, direct):
+    assert leaves.parts == 1 << total
+    out, off = leaves.to_host()
+    lens = np.diff(off.astype(np.int64))
+    assert np.array_equal(leaf_ids(out.keys, total), np.repeat(np.arange(leaves.parts), lens))
+    want_lid = leaf_ids(rel.keys, total)
+    assert np.array_equal(lens, np.bincount(want_lid, minlength=leaves.parts))
+    # per-leaf row multisets: rows ordered by (oracle leaf id, row bytes) on both sides
+    order = np.argsort(want_lid, kind="stable")
+    want_rows = [rows_multiset(host_rel(rel), order[int(off[L]):int(off[L + 1])])
+                 for L in range(0, leaves.parts, max(1, leaves.parts // 97))]
+    got_rows = [rows_multiset(out, np.arange(int(off[L]), int(off[L + 1])))
+                for L in range(0, leaves.parts, max(1, leaves.parts // 97))]
+    for a, b in zip(got_rows, want_rows):
+        assert np.array_equal(a, b)
+    # and the whole relation's multiset
+    assert np.array_equal(rows_multiset(out), rows_multiset(host_rel(rel)))
+    for x in (p1, leaves):
         x.release()
 
 
@@ -383,8 +395,12 @@ def _write_side(f, rel: O.Relation):
         f.write(np.ascontiguousarray(c).tobytes())
 
 
-@pytest.mark.parametrize("scheme,pass_bits", [(1, [8]), (3, [8]), (3, [4, 4]), (2, [4, 4])])
-def test_cpp_adapter_drop_in(scheme, pass_bits):
+@pytest.mark.parametrize("scheme,pass_bits,callers", [(1, [8], 1), (3, [8], 1), (3, [4, 4], 1), (2, [4, 4], 1),
+                                                      (3, [8, 8], 2), (1, [8], 3), (2, [6], 2)])
+def test_cpp_adapter_drop_in(scheme, pass_bits, callers):
+    """The drop-in signatures through the C++ adapter, from `callers` threads at
+    once (the reference's scheme functions may run concurrently): every
+    caller's rows equal the oracle's."""
     exe = os.path.join(ROOT, "phj-bench_amd", "host", "adapter_selftest")
     ob, op = data()["reference_streams"]
     want = O.serialize_sorted(O.reference_join(ob, op))
@@ -395,7 +411,7 @@ def test_cpp_adapter_drop_in(scheme, pass_bits):
             f.write(struct.pack("<QII", 0x31464C45534A4850, scheme, len(pass_bits)) + pb + struct.pack("<QQ", 1, 4))
             _write_side(f, ob)
             _write_side(f, op)
-        r = subprocess.run([exe, inp, outp], capture_output=True, text=True, timeout=120)
+        r = subprocess.run([exe, inp, outp, str(callers)], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         with open(outp, "rb") as f:
             n, rs = struct.unpack("<QQ", f.read(16))

@@ -1,5 +1,9 @@
-"""Resident build sides and the serving loop: a request's result is the same
-alone and in the mix, and a resident probe equals runPHJ on the same pair."""
+"""Resident build sides and the serving loop: every served request's result
+equals the closed form of its generator spec (the exact join output), alone
+and in the mix, and a resident probe equals runPHJ on the same pair."""
+import threading
+import time
+
 import numpy as np
 import pytest
 
@@ -52,19 +56,22 @@ def test_resident_probe_with_repeated_keys(ctx):
 def test_serving_loop_results_alone_and_in_mix(ctx):
     kinds, residents, _ = serve.request_pool(ctx, seed=3, n_new=4, n_resident=3, n_resident_probes=3, lo_log2=8,
                                              hi_log2=15, probe_lo=2, probe_hi=6)
+    exact = [serve.expected(ctx, k) for k in kinds]
     alone = [serve.run_request(ctx, k, want_checksum=True) for k in kinds]
+    assert alone == exact
     srv = serve.JoinServer(workers=3, budget_bytes=1 << 30)
     sched = serve.poisson_schedule(rate=2000, n_requests=60, n_kinds=len(kinds), seed=4)
     outs = srv.run(kinds, sched, verify_every=1)
+    assert {o.kind for o in outs} == set(range(len(kinds)))
     for o in outs:
-        assert (o.out_rows, o.checksum) == alone[o.kind]
+        assert (o.out_rows, o.checksum) == exact[o.kind]
         assert o.done >= o.admitted >= o.arrival - 1e-6
     summary = serve.summarize(outs, kinds)
     assert summary["requests"] == 60 and summary["p99_ms"] >= summary["p50_ms"] > 0
     # a tiny budget admits one request at a time and still completes all
     srv_small = serve.JoinServer(workers=2, budget_bytes=1)
     outs = srv_small.run(kinds, sched[:10], verify_every=1)
-    assert all((o.out_rows, o.checksum) == alone[o.kind] for o in outs)
+    assert all((o.out_rows, o.checksum) == exact[o.kind] for o in outs)
     for s in (srv, srv_small):
         s.close()
     for k in kinds:
@@ -73,3 +80,38 @@ def test_serving_loop_results_alone_and_in_mix(ctx):
                 rel.release()
     for r in residents:
         r.release()
+
+
+def test_resident_release_refused_while_probing(ctx):
+    """Release of a resident build side fails while a probe of it runs on
+    another thread (counted on the host), and succeeds once it has returned.
+    The probe (2^29 rows) runs for tens of milliseconds; the release is tried
+    a few milliseconds after the probing thread has entered the call."""
+    spec = G.gen_spec(seed=21, build_rows=1 << 20, probe_rows=1 << 29, match_rate=0.5)
+    b, p = ctx.generate(spec)
+    res = ctx.register_build(b)
+    other = G.Context(0)
+    out, entering = {}, threading.Event()
+
+    def prober():
+        entering.set()
+        r = res.probe(p, ctx=other)
+        out["t"] = time.perf_counter()
+        out["cs"] = r.checksum()
+        r.release()
+
+    th = threading.Thread(target=prober)
+    th.start()
+    entering.wait(timeout=60)
+    time.sleep(0.003)
+    t_try = time.perf_counter()
+    with pytest.raises(ValueError, match="in flight"):
+        res.release()
+    th.join()
+    assert out["t"] > t_try  # the probe was still running when release was refused
+    assert out["cs"] == ctx.expected_checksum(spec)
+    res.release()
+    assert res._h is None
+    other.close()
+    b.release()
+    p.release()

@@ -146,6 +146,26 @@ def test_zipf_generator_invariants():
     assert 0.88 < hit.mean() < 0.92
 
 
+@pytest.mark.parametrize("seed,distinct,probes,maxdup", [(1, 4096, 1 << 16, 64), (7, 300, 20000, 5), (3, 1, 500, 64),
+                                                          (9, 5000, 0, 64)])
+def test_zipf_closed_form_checksum(seed, distinct, probes, maxdup):
+    """The Zipf closed form (each hit pairs with every build row of its
+    distinct key, found through the inverse of the build-row shuffle) equals
+    the checksum of the hash-map join over the generated rows."""
+    s = O.make_spec(seed=seed, build_rows=distinct, probe_rows=probes, distribution=2, match_rate=0.9,
+                    build_types=(0, 1, 3), probe_types=(3, 4), zipf_max_dup=maxdup)
+    b, p = O.gen(s)
+    want = O.reference_join_checksum(b, p)
+    assert O.expected_checksum(s) == want
+    # a join that drops one build row of a repeated key must not pass
+    keys, counts = np.unique(b.keys, return_counts=True)
+    if probes and counts.max() > 1:
+        drop = int(np.flatnonzero(b.keys == keys[np.argmax(counts)])[0])
+        keep = np.arange(b.rows) != drop
+        short = O.Relation(b.keys[keep], [c[keep] for c in b.cols], b.types)
+        assert O.reference_join_checksum(short, p) != want
+
+
 def test_golden_reference_join():
     """Fixture written by the compiled reference (tests/golden/make_golden.py)."""
     g = np.load(os.path.join(GOLDEN, "ref_join_small.npz"))
