@@ -235,6 +235,20 @@ phj_status phj_hashtable_build_find(phj_ctx *ctx, const uint64_t *build_keys, ui
                                     uint64_t n_probe, uint32_t block_rows, int on_device, phj_row_ref *heads,
                                     phj_row_ref *next);
 
+/* JoinHashTable build + find that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * results as phj_hashtable_build_find with on_device = 1 (every pointer is a
+ * device pointer, heads/next written in device memory). Its scratch (hash
+ * cells, chain successors, one chunk of probe rows) is a fixed 40 MiB array
+ * that each GPU holds for the life of the process: at most 2^20 build rows
+ * (more fail with PHJ_ERR_CAPACITY); probe rows are unlimited (looked up in
+ * chunks of 2^20). All calls on one GPU share that array, from any context:
+ * calls that overlap on one GPU race and must be serialised by the caller. */
+phj_status phj_hashtable_build_find_noalloc(phj_ctx *ctx, const uint64_t *build_keys, uint64_t n_build,
+                                            const uint64_t *probe_keys, uint64_t n_probe, uint32_t block_rows,
+                                            phj_row_ref *heads, phj_row_ref *next);
+
 /* ---- schemes ---- */
 phj_status phj_join(phj_ctx *ctx, const phj_join_desc *desc, phj_join_result *result);
 phj_status phj_run_chj(phj_ctx *ctx, const phj_relation *build, const phj_relation *probe, uint32_t threads,

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -302,8 +303,49 @@ private:
 
 } // namespace
 
+// ---------------------------------------------------------------- allocation-free scratch
+// Device memory of phj_hashtable_build_find_noalloc: the hash-table cells, the
+// build rows' chain successors and one chunk of probe-row results, in one
+// file-scope array that every GPU holds once for the life of the process.
+// Host code never passes `scratch` itself to a kernel (on the host the name
+// is only a stand-in): it uses the address hipGetSymbolAddress returns for
+// the current device, looked up once per device when its first context is
+// created.
+constexpr uint64_t kNoAllocMaxBuild = 1ull << 20;              // build rows
+constexpr uint64_t kNoAllocMaxCells = 2 * kNoAllocMaxBuild;    // bit_ceil(2 * rows), src/HashTable.h:72-77
+constexpr uint64_t kNoAllocProbeChunk = 1ull << 20;            // probe rows per lookup launch
+constexpr uint64_t kNoAllocCellsOff = 0;
+constexpr uint64_t kNoAllocNextOff = kNoAllocCellsOff + kNoAllocMaxCells * 16;
+constexpr uint64_t kNoAllocRowsOff = kNoAllocNextOff + kNoAllocMaxBuild * 4;
+constexpr uint64_t kNoAllocWordsOff = kNoAllocRowsOff + kNoAllocProbeChunk * 4; // cell offsets, side head
+constexpr uint64_t kNoAllocBytes = kNoAllocWordsOff + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), looked up once.
+hipError_t scratch_address(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices)
+        return hipErrorInvalidDevice;
+    if (g_scratch_ptr[device] == nullptr) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+} // namespace
+
 struct phj_ctx {
     int device = 0;
+    void *scratch = nullptr; // this device's `scratch` (device address)
     hipStream_t stream = nullptr;
     std::string err;
     Arena arena;
@@ -1874,6 +1916,11 @@ phj_status phj_ctx_create(int device, phj_ctx **out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
         return PHJ_ERR_HIP;
+    if (scratch_address(device, &ctx->scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamDestroy(ctx->stream);
+        return PHJ_ERR_HIP;
+    }
     ctx->arena.bind(ctx->stream);
     *out = ctx.release();
     return PHJ_OK;
@@ -2091,6 +2138,51 @@ phj_status phj_hashtable_build_find(phj_ctx *ctx, const uint64_t *build_keys, ui
             if (n_build && next)
                 HIPCHK(hipMemcpyAsync(next, dn, n_build * 8, hipMemcpyDeviceToHost, ctx->stream));
         }
+        ctx->sync();
+    });
+}
+
+phj_status phj_hashtable_build_find_noalloc(phj_ctx *ctx, const uint64_t *build_keys, uint64_t n_build,
+                                            const uint64_t *probe_keys, uint64_t n_probe, uint32_t block_rows,
+                                            phj_row_ref *heads, phj_row_ref *next) {
+    return guard(ctx, [&] {
+        if (block_rows == 0)
+            invalid("block_rows must be positive");
+        if ((n_build && (build_keys == nullptr || next == nullptr)) || (n_probe && (probe_keys == nullptr || heads == nullptr)))
+            invalid("null device buffer");
+        if (n_build > kNoAllocMaxBuild)
+            throw PhjError(PHJ_ERR_CAPACITY, "phj_hashtable_build_find_noalloc: " + std::to_string(n_build) +
+                                                 " build rows exceed its scratch (at most " +
+                                                 std::to_string(kNoAllocMaxBuild) + ")");
+        unsigned char *base = static_cast<unsigned char *>(ctx->scratch);
+        uint64_t cap = 2; // src/HashTable.h:72-77 reserve: bit_ceil(2 * rows)
+        while (cap < 2 * n_build)
+            cap <<= 1;
+        uint64_t *words = reinterpret_cast<uint64_t *>(base + kNoAllocWordsOff);
+        // cell offsets {0, cap} and the table's side head, filled by kernels
+        launch_fill_u64(ctx->stream, words, 1, 0);
+        launch_fill_u64(ctx->stream, words + 1, 1, cap);
+        GlobalHT ht{};
+        ht.cells = reinterpret_cast<Cell16 *>(base + kNoAllocCellsOff);
+        ht.cell_off = words;
+        ht.next = reinterpret_cast<uint32_t *>(base + kNoAllocNextOff);
+        ht.side_head = reinterpret_cast<uint32_t *>(words + 2);
+        ht.n_tables = 1;
+        ht.block_rows = block_rows;
+        ht_init(ctx->stream, ht, cap, n_build, ctx->tm());
+        HtRows hr{};
+        hr.keys = build_keys;
+        hr.rows = n_build;
+        ht_build(ctx->stream, ht, kHtSingle, hr, n_build, ctx->tm(), "ht_build");
+        uint32_t *rows = reinterpret_cast<uint32_t *>(base + kNoAllocRowsOff);
+        for (uint64_t a = 0; a < n_probe; a += kNoAllocProbeChunk) {
+            const uint64_t m = std::min(kNoAllocProbeChunk, n_probe - a);
+            ht_find(ctx->stream, ht, probe_keys + a, m, nullptr, rows, ctx->tm());
+            k_rows_to_refs<<<static_cast<uint32_t>((m + 255) / 256), 256, 0, ctx->stream>>>(rows, m, block_rows, heads + a);
+        }
+        if (n_build)
+            k_rows_to_refs<<<static_cast<uint32_t>((n_build + 255) / 256), 256, 0, ctx->stream>>>(ht.next, n_build,
+                                                                                                  block_rows, next);
         ctx->sync();
     });
 }

@@ -172,6 +172,8 @@ _SIGNATURES = {
     "phj_partitioned_release": ([C.c_void_p, C.POINTER(_Partitioned)], C.c_int),
     "phj_hashtable_build_find": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int,
                                   C.c_void_p, C.c_void_p], C.c_int),
+    "phj_hashtable_build_find_noalloc": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint32,
+                                          C.c_void_p, C.c_void_p], C.c_int),
     "phj_join": ([C.c_void_p, C.POINTER(_JoinDesc), C.POINTER(_JoinResult)], C.c_int),
     "phj_build_register": ([C.c_void_p, C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.POINTER(_Tuning),
                             C.POINTER(C.c_void_p)], C.c_int),
@@ -688,6 +690,38 @@ class Context:
         _raise(self._p, load_library().phj_hashtable_build_find(self._p, bk.ctypes.data, bk.size, pk.ctypes.data, pk.size,
                                                                 block_rows, 0, heads.ctypes.data, nxt.ctypes.data))
         return heads, nxt
+
+    def hashtable_build_find_noalloc(self, build_keys, probe_keys, block_rows=PIPELINE_BLOCK_ROWS):
+        """hashtable_build_find through the entry point that allocates nothing:
+        the keys are copied into device buffers this call allocates up front
+        (the caller's inputs and outputs), then the library works in its
+        fixed per-GPU scratch. Returns (heads, next) like hashtable_build_find."""
+        lib = load_library()
+        bk = np.ascontiguousarray(build_keys, dtype=np.uint64)
+        pk = np.ascontiguousarray(probe_keys, dtype=np.uint64)
+        bufs = []
+        try:
+            def dev(nbytes):
+                ptr = C.c_void_p()
+                _raise(self._p, lib.phj_dev_alloc(self._p, max(nbytes, 1), C.byref(ptr)))
+                bufs.append(ptr)
+                return ptr
+            db, dpk, dh, dn = dev(bk.nbytes), dev(pk.nbytes), dev(pk.size * 8), dev(bk.size * 8)
+            for d, a in ((db, bk), (dpk, pk)):
+                if a.nbytes:
+                    _raise(self._p, lib.phj_memcpy_h2d(self._p, d, a.ctypes.data, a.nbytes))
+            before = self.mem_stats()
+            _raise(self._p, lib.phj_hashtable_build_find_noalloc(self._p, db, bk.size, dpk, pk.size, block_rows, dh, dn))
+            if self.mem_stats() != before:
+                raise RuntimeError("phj_hashtable_build_find_noalloc changed the context's device memory")
+            heads = np.zeros((pk.size, 2), dtype=np.uint32)
+            nxt = np.zeros((bk.size, 2), dtype=np.uint32)
+            self._d2h(heads, dh)
+            self._d2h(nxt, dn)
+            return heads, nxt
+        finally:
+            for b in bufs:
+                lib.phj_dev_free(self._p, b)
 
     # -- schemes
     def join(self, scheme, build, probe, cfg: RadixConfig | None = None, threads=1, bep_budget_mib=512,
