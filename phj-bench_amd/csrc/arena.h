@@ -1,0 +1,208 @@
+// arena.h — the best-fit device memory pool of one context.
+//
+// Memory comes from the backend (hipMalloc on the device) in segments of at
+// least kSegmentBytes; freed blocks coalesce with free neighbours of their
+// segment and are reused by later allocations of any size. Every allocation
+// and every kernel of a context runs on the context's one stream, so a block
+// freed there may be reused at once: stream order puts the new user's work
+// after the old user's. Blocks never move to another pool (another stream).
+// Segments go back to the backend only when an allocation fails, or when more
+// than 60% of the device's memory sits free in this pool, and only after the
+// stream's queued work (which may still use them) has finished.
+//
+// Backend interface (static members): Stream; void *alloc(uint64_t) (null on
+// failure); void free(void *); void sync(Stream); bool mem_info(uint64_t
+// &free, uint64_t &total). The header needs no HIP, so the CPU tests drive the
+// pool with a recording backend.
+#pragma once
+
+#include <stdint.h>
+
+#include <algorithm>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+
+namespace phjd {
+
+// Thrown by the pool: out of memory (or over the pool's limit), or a pointer
+// the pool does not own.
+struct ArenaError : std::runtime_error {
+    enum Kind { kOutOfMemory, kForeignPointer } kind;
+    ArenaError(Kind k, const std::string &m) : std::runtime_error(m), kind(k) {}
+};
+
+template <class Dev> class BasicArena {
+public:
+    using Stream = typename Dev::Stream;
+    static constexpr uint64_t kAlign = 256;
+    static constexpr uint64_t kSegmentBytes = 1ull << 30;
+    static constexpr uint64_t kMinSplit = 1ull << 20;
+
+    ~BasicArena() { release_cached(); }
+
+    void bind(Stream s) {
+        stream_ = s;
+        bound_ = true;
+        uint64_t free_mem = 0, total = 0;
+        if (Dev::mem_info(free_mem, total))
+            cache_limit_ = total / 10 * 6;
+    }
+
+    void *alloc(uint64_t bytes) {
+        const uint64_t sz = (std::max<uint64_t>(bytes, 1) + kAlign - 1) & ~(kAlign - 1);
+        if (limit_ != 0 && current_ + sz > limit_)
+            throw ArenaError(ArenaError::kOutOfMemory, "device allocation of " + std::to_string(sz) +
+                                                           " bytes exceeds the context's memory limit");
+        Block *b = take(sz);
+        if (b == nullptr)
+            b = grow(sz);
+        if (b == nullptr) {
+            release_cached();
+            b = grow(sz);
+        }
+        if (b == nullptr)
+            throw ArenaError(ArenaError::kOutOfMemory, "device allocation of " + std::to_string(sz) + " bytes failed");
+        live_[b->ptr] = b;
+        free_bytes_ -= b->size;
+        current_ += b->size;
+        peak_ = std::max(peak_, current_);
+        return b->ptr;
+    }
+
+    void free(void *p) {
+        if (p == nullptr)
+            return;
+        auto it = live_.find(p);
+        if (it == live_.end())
+            throw ArenaError(ArenaError::kForeignPointer, "pointer not allocated by this context");
+        Block *b = it->second;
+        live_.erase(it);
+        current_ -= b->size;
+        free_bytes_ += b->size;
+        b->free = true;
+        b = coalesce(b);
+        free_.emplace(b->size, b);
+        if (free_bytes_ > cache_limit_)
+            release_cached();
+    }
+
+    // free() for cleanup paths that must not throw (destructors during unwinding).
+    void free_quiet(void *p) noexcept {
+        try {
+            free(p);
+        } catch (...) {
+        }
+    }
+
+    bool owns(const void *p) const { return live_.count(const_cast<void *>(p)) != 0; }
+
+    // Returns segments with no live block to the backend, after the stream's
+    // queued work has finished.
+    void release_cached() {
+        if (bound_)
+            Dev::sync(stream_);
+        for (auto it = free_.begin(); it != free_.end();) {
+            Block *b = it->second;
+            auto seg = segments_.find(b->ptr);
+            if (seg != segments_.end() && b->prev == nullptr && b->next == nullptr && b->size == seg->second) {
+                Dev::free(b->ptr);
+                free_bytes_ -= b->size;
+                segments_.erase(seg);
+                delete b;
+                it = free_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+
+    uint64_t current() const { return current_; }
+    uint64_t peak() const { return peak_; }
+    uint64_t segments() const { return segments_.size(); }
+    void reset_peak() { peak_ = current_; }
+    // Upper bound on the bytes held by live blocks (0: none).
+    void set_limit(uint64_t bytes) { limit_ = bytes; }
+
+private:
+    struct Block {
+        char *ptr;
+        uint64_t size;
+        bool free;
+        Block *prev, *next; // neighbours in the segment, by address
+    };
+
+    void erase_free(Block *b) {
+        auto range = free_.equal_range(b->size);
+        for (auto it = range.first; it != range.second; ++it)
+            if (it->second == b) {
+                free_.erase(it);
+                return;
+            }
+    }
+
+    Block *coalesce(Block *b) {
+        if (b->next && b->next->free) {
+            Block *n = b->next;
+            erase_free(n);
+            b->size += n->size;
+            b->next = n->next;
+            if (n->next)
+                n->next->prev = b;
+            delete n;
+        }
+        if (b->prev && b->prev->free) {
+            Block *p = b->prev;
+            erase_free(p);
+            p->size += b->size;
+            p->next = b->next;
+            if (b->next)
+                b->next->prev = p;
+            delete b;
+            b = p;
+        }
+        return b;
+    }
+
+    // Smallest free block of at least sz, split when the rest is worth keeping.
+    Block *take(uint64_t sz) {
+        auto it = free_.lower_bound(sz);
+        if (it == free_.end())
+            return nullptr;
+        Block *b = it->second;
+        free_.erase(it);
+        if (b->size - sz >= kMinSplit) {
+            Block *r = new Block{b->ptr + sz, b->size - sz, true, b, b->next};
+            if (b->next)
+                b->next->prev = r;
+            b->next = r;
+            b->size = sz;
+            free_.emplace(r->size, r);
+        }
+        b->free = false;
+        return b;
+    }
+
+    Block *grow(uint64_t sz) {
+        const uint64_t seg = std::max(sz, kSegmentBytes);
+        char *p = static_cast<char *>(Dev::alloc(seg));
+        if (p == nullptr)
+            return nullptr;
+        segments_[p] = seg;
+        free_bytes_ += seg;
+        free_.emplace(seg, new Block{p, seg, true, nullptr, nullptr});
+        return take(sz);
+    }
+
+    Stream stream_{};
+    bool bound_ = false;
+    std::multimap<uint64_t, Block *> free_;
+    std::unordered_map<void *, Block *> live_;
+    std::unordered_map<char *, uint64_t> segments_;
+    uint64_t current_ = 0, peak_ = 0, free_bytes_ = 0;
+    uint64_t cache_limit_ = 64ull << 30;
+    uint64_t limit_ = 0;
+};
+
+} // namespace phjd

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/phj_gpu.h"
+#include "arena.h"
 #include "gen.h"
 #include "kernels.h"
 
@@ -48,185 +49,34 @@ struct PhjError : std::runtime_error {
 void invalid(const std::string &m) { throw PhjError(PHJ_ERR_INVALID_ARGUMENT, m); }
 
 // ---------------------------------------------------------------- device memory
-// Best-fit pool of one context. Memory comes from hipMalloc in segments (at
-// least kSegmentBytes); freed blocks coalesce with free neighbours of their
-// segment and are reused by later allocations of any size. Every allocation
-// and every kernel of a context runs on the context's one stream, so a block
-// freed there may be reused at once: stream order puts the new user's work
-// after the old user's. Blocks never move to another stream. Segments go back
-// to the device only when an allocation fails, or when more than 60% of the
-// device's memory sits free in this context.
-class Arena {
-public:
-    ~Arena() { release_cached(); }
-
-    void bind(hipStream_t s) {
-        stream_ = s;
-        size_t free_mem = 0, total = 0;
-        if (hipMemGetInfo(&free_mem, &total) == hipSuccess)
-            cache_limit_ = total / 10 * 6;
-        else
-            (void)hipGetLastError();
-    }
-
-    void *alloc(uint64_t bytes) {
-        const uint64_t sz = (std::max<uint64_t>(bytes, 1) + kAlign - 1) & ~(kAlign - 1);
-        if (limit_ != 0 && current_ + sz > limit_)
-            throw PhjError(PHJ_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(sz) +
-                                                      " bytes exceeds the context's memory limit");
-        Block *b = take(sz);
-        if (b == nullptr)
-            b = grow(sz);
-        if (b == nullptr) {
-            release_cached();
-            b = grow(sz);
-        }
-        if (b == nullptr)
-            throw PhjError(PHJ_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(sz) + " bytes failed");
-        live_[b->ptr] = b;
-        free_bytes_ -= b->size;
-        current_ += b->size;
-        peak_ = std::max(peak_, current_);
-        return b->ptr;
-    }
-
-    void free(void *p) {
-        if (p == nullptr)
-            return;
-        auto it = live_.find(p);
-        if (it == live_.end())
-            throw PhjError(PHJ_ERR_INVALID_ARGUMENT, "pointer not allocated by this context");
-        Block *b = it->second;
-        live_.erase(it);
-        current_ -= b->size;
-        free_bytes_ += b->size;
-        b->free = true;
-        b = coalesce(b);
-        free_.emplace(b->size, b);
-        if (free_bytes_ > cache_limit_)
-            release_cached();
-    }
-
-    // free() for cleanup paths that must not throw (destructors during unwinding).
-    void free_quiet(void *p) noexcept {
-        try {
-            free(p);
-        } catch (...) {
-        }
-    }
-
-    bool owns(const void *p) const { return live_.count(const_cast<void *>(p)) != 0; }
-
-    // Returns segments with no live block to the device (after the stream's
-    // queued work, which may still use them, has finished).
-    void release_cached() {
-        if (stream_ != nullptr)
-            (void)hipStreamSynchronize(stream_);
-        for (auto it = free_.begin(); it != free_.end();) {
-            Block *b = it->second;
-            auto seg = segments_.find(b->ptr);
-            if (seg != segments_.end() && b->prev == nullptr && b->next == nullptr && b->size == seg->second) {
-                (void)hipFree(b->ptr);
-                free_bytes_ -= b->size;
-                segments_.erase(seg);
-                delete b;
-                it = free_.erase(it);
-            } else {
-                ++it;
-            }
-        }
-    }
-
-    uint64_t current() const { return current_; }
-    uint64_t peak() const { return peak_; }
-    void reset_peak() { peak_ = current_; }
-    // Upper bound on the bytes held by live blocks (0: none).
-    void set_limit(uint64_t bytes) { limit_ = bytes; }
-
-private:
-    static constexpr uint64_t kAlign = 256;
-    static constexpr uint64_t kSegmentBytes = 1ull << 30;
-    static constexpr uint64_t kMinSplit = 1ull << 20;
-
-    struct Block {
-        char *ptr;
-        uint64_t size;
-        bool free;
-        Block *prev, *next; // neighbours in the segment, by address
-    };
-
-    void erase_free(Block *b) {
-        auto range = free_.equal_range(b->size);
-        for (auto it = range.first; it != range.second; ++it)
-            if (it->second == b) {
-                free_.erase(it);
-                return;
-            }
-    }
-
-    Block *coalesce(Block *b) {
-        if (b->next && b->next->free) {
-            Block *n = b->next;
-            erase_free(n);
-            b->size += n->size;
-            b->next = n->next;
-            if (n->next)
-                n->next->prev = b;
-            delete n;
-        }
-        if (b->prev && b->prev->free) {
-            Block *p = b->prev;
-            erase_free(p);
-            p->size += b->size;
-            p->next = b->next;
-            if (b->next)
-                b->next->prev = p;
-            delete b;
-            b = p;
-        }
-        return b;
-    }
-
-    // Smallest free block of at least sz, split when the rest is worth keeping.
-    Block *take(uint64_t sz) {
-        auto it = free_.lower_bound(sz);
-        if (it == free_.end())
-            return nullptr;
-        Block *b = it->second;
-        free_.erase(it);
-        if (b->size - sz >= kMinSplit) {
-            Block *r = new Block{b->ptr + sz, b->size - sz, true, b, b->next};
-            if (b->next)
-                b->next->prev = r;
-            b->next = r;
-            b->size = sz;
-            free_.emplace(r->size, r);
-        }
-        b->free = false;
-        return b;
-    }
-
-    Block *grow(uint64_t sz) {
-        const uint64_t seg = std::max(sz, kSegmentBytes);
-        char *p = nullptr;
-        if (hipMalloc(reinterpret_cast<void **>(&p), seg) != hipSuccess) {
+// The context's pool (arena.h) over hipMalloc, with the context stream.
+struct HipBackend {
+    using Stream = hipStream_t;
+    static void *alloc(uint64_t bytes) {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
             return nullptr;
         }
-        segments_[p] = seg;
-        free_bytes_ += seg;
-        free_.emplace(seg, new Block{p, seg, true, nullptr, nullptr});
-        return take(sz);
+        return p;
     }
-
-    hipStream_t stream_ = nullptr;
-    std::multimap<uint64_t, Block *> free_;
-    std::unordered_map<void *, Block *> live_;
-    std::unordered_map<char *, uint64_t> segments_;
-    uint64_t current_ = 0, peak_ = 0, free_bytes_ = 0;
-    uint64_t cache_limit_ = 64ull << 30;
-    uint64_t limit_ = 0;
+    static void free(void *p) { (void)hipFree(p); }
+    static void sync(hipStream_t s) {
+        if (s != nullptr)
+            (void)hipStreamSynchronize(s);
+    }
+    static bool mem_info(uint64_t &free_mem, uint64_t &total) {
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        free_mem = f;
+        total = t;
+        return true;
+    }
 };
+using Arena = BasicArena<HipBackend>;
 
 // ---------------------------------------------------------------- kernel timer
 class EventTimer : public KernelTimer {
@@ -1883,6 +1733,9 @@ template <class F> phj_status guard(phj_ctx *ctx, F &&f) {
     } catch (const PhjError &e) {
         ctx->err = e.what();
         return e.code;
+    } catch (const ArenaError &e) {
+        ctx->err = e.what();
+        return e.kind == ArenaError::kOutOfMemory ? PHJ_ERR_OUT_OF_MEMORY : PHJ_ERR_INVALID_ARGUMENT;
     } catch (const LaunchTooLarge &e) {
         ctx->err = e.what();
         return PHJ_ERR_UNSUPPORTED;
