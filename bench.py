@@ -121,8 +121,7 @@ def skew_bench(reps=2, scale_log2=0):
     """2^22 distinct build keys, each repeated k ~ Zipf(1.2) on [1,64] times and
     shuffled (u8+u16+u64 payload); 2^27 probe rows, 90% drawing a key by
     Zipf(0.8) rank, 10% missing (u64 payload). PHJ-PURE timed; CHJ and PHJ
-    (best-effort, 32 MiB per workgroup x 512 workgroups) must give the same
-    checksum."""
+    (best-effort, M = 32 MiB) must give the closed-form checksum."""
     ctx = G.Context(0)
     spec = G.gen_spec(seed=42, build_rows=1 << (22 - scale_log2), probe_rows=1 << (27 - scale_log2),
                       distribution=G.DIST_ZIPF, match_rate=0.9,
@@ -142,19 +141,25 @@ def skew_bench(reps=2, scale_log2=0):
         res = ctx.runPHJ(build, probe, cfg)
         ms = (time.perf_counter() - t0) * 1e3
         best = ms if best is None else min(best, ms)
-    want = res.checksum()
+    want = ctx.expected_checksum(spec)  # closed form: every hit pairs with every build row of its key
+    sums = {"phj_pure": res.checksum() == want}
     out_rows = res.out_rows
     res.release()
-    sums = {}
     log(f"skew: PHJ-PURE best {best:.2f} ms")
     r = ctx.runCHJ(build, probe)
     sums["chj"] = r.checksum() == want
     log(f"skew: CHJ e2e {r.e2e_wall_ms:.2f} ms")
     r.release()
-    r = ctx.runPhjBep(build, probe, cfg, bep_budget_mib=32 * 512)
+    # M = 32 MiB: the device runs one probe scheduler, so the per-worker budget
+    # is the whole budget (src/PHJBep.cpp:186-190 with one worker)
+    r = ctx.runPhjBep(build, probe, cfg, bep_budget_mib=32)
     sums["phj_bep"] = r.checksum() == want
-    bep = {"evictions": r.bep_evictions, "refinements": r.bep_refinements, "peak_bytes": r.bep_peak_bytes,
-           "e2e_ms": r.e2e_wall_ms}
+    bep = {"budget_mib": r.bep_budget_mib, "evictions": r.bep_evictions, "refinements": r.bep_refinements,
+           "peak_bytes": r.bep_peak_bytes, "e2e_ms": r.e2e_wall_ms,
+           "probe_shuffle_ms": r.probe_shuffle.wall_ms, "probe_ms": r.probe.wall_ms,
+           "eviction_overhead_ms": r.eviction_overhead.wall_ms, "build_ms": r.build.wall_ms}
+    log(f"skew: PHJ (32 MiB) e2e {r.e2e_wall_ms:.1f} ms, {r.bep_refinements} refinements, "
+        f"{r.bep_evictions} evictions, peak {r.bep_peak_bytes >> 20} MiB")
     r.release()
     R, S = build.rows, probe.rows
     build.release()

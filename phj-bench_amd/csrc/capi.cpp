@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1077,13 +1078,6 @@ void probe_resident(phj_ctx *ctx, const phj_resident &r, const phj_relation &p, 
     phj_pure_probe(ctx, r.RP, r.r_cols, S, r.passes, tun, e2e, nullptr, keep, res);
 }
 
-// Adds the number of non-empty partitions to *out (one count over all slices).
-__global__ void k_count_nonempty(const uint32_t *off, uint32_t n, unsigned long long *out) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n && off[g + 1] > off[g])
-        atomicAdd(out, 1ull);
-}
-
 // Builds the tables of every leaf of RP into `img` (buffers go to `keep`) for
 // probes of up to `probe_rows` rows each. False, with nothing kept, when those
 // probes hold a quarter of the largest leaf or more per leaf (each work item
@@ -1133,17 +1127,442 @@ bool build_table_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan 
 }
 
 // ---- PHJ (best-effort partitioning): src/PHJBep.cpp:113-790 ----
-// The build side is shuffled to full leaf depth exactly as PHJ-PURE. The probe
-// side is never materialised whole: it is consumed in slices sized so that one
-// partitioned slice (two pass buffers) fits the global budget; every slice is
-// partitioned to the leaves and joined against the leaves' tables at once (an
-// eviction of every leaf the slice touched), then its buffers are reused.
+// The reference's schedule, run by one device scheduler (the host side of the
+// context's stream) instead of `threads` CPU workers:
+//   - the build side is shuffled to full leaf depth and every leaf's table is
+//     built up front when the build side allows (src/PHJBep.cpp:157-231);
+//   - the probe side is scattered by its first pass bits, one batch at a time,
+//     into pass-1 buckets ("unrefined", src/PHJBep.cpp:574-606). A batch is a
+//     quarter of the unrefined high-water mark, the device's unit of work where
+//     the reference scatters one 10,000-row block;
+//   - after each batch, when unrefined rows reach the high-water mark, the
+//     largest buckets are refined through the remaining passes into leaf
+//     buffers until the low-water mark holds (src/PHJBep.cpp:508-533, :397-444);
+//     all buckets chosen at one trigger go through one segmented pass sequence;
+//   - when buffered leaf rows reach their high-water mark, leaves are evicted
+//     round-robin from a cursor (empty leaves skipped) until their low-water
+//     mark holds: each evicted leaf's buffered probe rows are joined against
+//     its table and dropped (src/PHJBep.cpp:453-505, :535-568); all leaves of
+//     one trigger are joined in one launch sequence;
+//   - at the end every residual bucket is refined and every leaf drained
+//     (src/PHJBep.cpp:633-695).
+// Marks (src/PHJBep.cpp:186-190, one worker): unrefined high water budget/2,
+// low water a half of it; leaf high water 5/8 of the budget, low water 5/8 of
+// that. Rows are buffered in "runs" (one batch's pass-1 output, one refine's
+// leaf output) and read back by gathering their still-live row ranges, so no
+// buffer is ever grown in place. The device keeps at most kBepMaxRuns runs of
+// each kind: beyond that the buckets (leaves) still held by the oldest run
+// are refined (evicted) before the next batch (refine). Counters follow the
+// reference: refinements = buckets refined, evictions = leaves evicted
+// (drain included), peak = buffered leaf rows x row bytes at its largest
+// after a batch, eviction overhead = choosing victims and gathering their
+// rows. bep_build_skip_retries counts failed leaf try-locks between CPU
+// workers; one scheduler takes no locks, so it is always 0.
+constexpr size_t kBepMaxRuns = 8;
+
+struct BepRun {
+    Held bufs;
+    DevRel rel;                 // the run's rows: keys + columns
+    std::vector<uint32_t> off;  // host copy of its group offsets (groups + 1)
+    std::vector<uint8_t> done;  // per group: rows already refined/evicted
+    std::vector<int32_t> qidx;  // leaf runs: bucket -> its index in the refined set, -1 if absent
+    uint64_t live = 0;          // rows not yet consumed
+    explicit BepRun(phj_ctx *c) : bufs(c) {}
+    uint32_t len(uint32_t g) const { return off[g + 1] - off[g]; }
+};
+
+class BepSchedule {
+public:
+    BepSchedule(phj_ctx *ctx, const Partitioned &RP, const DevRel &S, uint32_t p1_bits, const std::vector<uint32_t> &rest,
+                uint64_t budget, const LeafJoinPlan &plan, uint32_t flags, const LeafTableImages *img, OutBuffers &out)
+        : ctx_(ctx), RP_(RP), S_(S), p1_(p1_bits), rest_(rest), plan_(plan), flags_(flags), img_(img), out_(out) {
+        P_ = 1u << p1_;
+        leaves_ = static_cast<uint32_t>(RP.parts);
+        lpp_ = leaves_ / P_;
+        rb_ = 8;
+        for (uint32_t c = 0; c < S.cols.n; ++c)
+            rb_ += type_size(S.cols.types[c]);
+        const uint64_t tile = radix_tile_rows();
+        u_hi_ = std::max<uint64_t>(budget / 2 / rb_, tile);
+        u_lo_ = u_hi_ / 2;
+        l_hi_ = std::max<uint64_t>(budget * 5 / 8 / rb_, 1);
+        l_lo_ = l_hi_ * 5 / 8;
+        batch_ = std::max<uint64_t>((u_hi_ / 4 + tile - 1) / tile * tile, tile);
+        bucket_rows_.assign(P_, 0);
+        leaf_rows_.assign(leaves_, 0);
+    }
+
+    void run() {
+        for (uint64_t a = 0; a < S_.rows; a += batch_) {
+            while (batch_runs_.size() >= kBepMaxRuns)
+                refine(held_by(*batch_runs_.front(), P_, [](uint32_t g) { return g; }));
+            scatter(a, std::min(batch_, S_.rows - a));
+            peak_ = std::max(peak_, global_leaf_ * rb_);
+            // src/PHJBep.cpp:508-533: drain own unrefined to the low-water mark, argmax first
+            if (unrefined_ >= u_hi_) {
+                std::vector<uint32_t> order(P_);
+                for (uint32_t q = 0; q < P_; ++q)
+                    order[q] = q;
+                std::stable_sort(order.begin(), order.end(),
+                                 [&](uint32_t x, uint32_t y) { return bucket_rows_[x] > bucket_rows_[y]; });
+                std::vector<uint32_t> Q;
+                uint64_t left = unrefined_;
+                for (uint32_t q : order) {
+                    if (left < u_lo_ || bucket_rows_[q] == 0)
+                        break;
+                    Q.push_back(q);
+                    left -= bucket_rows_[q];
+                }
+                refine(Q);
+            }
+            evict_to_low_water();
+        }
+        // src/PHJBep.cpp:678-695: refine every residual bucket, then drain every leaf
+        std::vector<uint32_t> Q;
+        uint64_t rows = 0;
+        for (uint32_t q = 0; q < P_; ++q) {
+            if (bucket_rows_[q] == 0)
+                continue;
+            if (!Q.empty() && rows + bucket_rows_[q] > u_hi_) {
+                refine(Q);
+                evict_to_low_water();
+                Q.clear();
+                rows = 0;
+            }
+            Q.push_back(q);
+            rows += bucket_rows_[q];
+        }
+        if (!Q.empty())
+            refine(Q);
+        std::vector<uint32_t> E;
+        rows = 0;
+        for (uint32_t L = 0; L < leaves_; ++L) {
+            if (leaf_rows_[L] == 0)
+                continue;
+            if (!E.empty() && rows + leaf_rows_[L] > std::max(l_hi_, u_hi_)) {
+                evict(E);
+                E.clear();
+                rows = 0;
+            }
+            E.push_back(L);
+            rows += leaf_rows_[L];
+        }
+        if (!E.empty())
+            evict(E);
+    }
+
+    uint64_t out_rows() const { return n_out_; }
+    uint64_t evictions() const { return evictions_; }
+    uint64_t refinements() const { return refinements_; }
+    uint64_t peak_bytes() const { return peak_; }
+    double shuffle_ms() const { return shuffle_ms_; }
+    double probe_ms() const { return probe_ms_; }
+    double overhead_ms() const { return overhead_ms_; }
+    double big_build_ms() const { return big_build_ms_; }
+
+private:
+    // Groups of `run` still holding rows, mapped to bucket or leaf ids.
+    template <class F> std::vector<uint32_t> held_by(const BepRun &run, uint32_t groups, F id) const {
+        std::vector<uint32_t> v;
+        for (uint32_t g = 0; g < groups; ++g)
+            if (!run.done[g] && run.len(g) > 0)
+                v.push_back(id(g));
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        return v;
+    }
+
+    // Pass-1 scatter of probe rows [a, a + n) into a new batch run.
+    void scatter(uint64_t a, uint64_t n) {
+        HIPCHK(hipEventRecord(ev_.a, ctx_->stream));
+        auto run = std::make_unique<BepRun>(ctx_);
+        DevRel slice = S_;
+        slice.rows = n;
+        slice.keys = S_.keys + a;
+        for (uint32_t c = 0; c < S_.cols.n; ++c)
+            slice.cols.ptr[c] = static_cast<const uint8_t *>(S_.cols.ptr[c]) + a * type_size(S_.cols.types[c]);
+        const Partitioned P1 = radix_partition(ctx_, slice, nullptr, 1, 0, {p1_}, run->bufs, "probe", flags_);
+        HIPCHK(hipEventRecord(ev_.b, ctx_->stream));
+        run->rel.rows = n;
+        run->rel.keys = P1.keys;
+        run->rel.cols = as_in(P1.cols);
+        run->off.resize(P_ + 1);
+        HIPCHK(hipMemcpyAsync(run->off.data(), P1.off, (P_ + 1) * 4ull, hipMemcpyDeviceToHost, ctx_->stream));
+        ctx_->sync();
+        shuffle_ms_ += ev_.ms();
+        run->done.assign(P_, 0);
+        run->live = n;
+        for (uint32_t q = 0; q < P_; ++q)
+            bucket_rows_[q] += run->len(q);
+        unrefined_ += n;
+        batch_runs_.push_back(std::move(run));
+    }
+
+    // Rows of group g of each run in `runs` (for which pick() gives g, or -1),
+    // gathered into one relation: groups in the caller's order, each group's
+    // rows from the oldest run first. Returns the staged relation (buffers in
+    // `keep`) and the row count per caller item.
+    template <class Pick>
+    DevRel gather(std::vector<std::unique_ptr<BepRun>> &runs, uint32_t items, Pick pick, Held &keep,
+                  std::vector<uint64_t> &item_rows, KernelTimer *tm) {
+        std::vector<uint64_t> start{0}, row;
+        std::vector<uint32_t> src;
+        item_rows.assign(items, 0);
+        for (uint32_t i = 0; i < items; ++i)
+            for (uint32_t k = 0; k < runs.size(); ++k) {
+                const int64_t g = pick(*runs[k], i);
+                if (g < 0 || runs[k]->done[g] || runs[k]->len(static_cast<uint32_t>(g)) == 0)
+                    continue;
+                const uint32_t len = runs[k]->len(static_cast<uint32_t>(g));
+                src.push_back(k);
+                row.push_back(runs[k]->off[g]);
+                start.push_back(start.back() + len);
+                item_rows[i] += len;
+            }
+        DevRel d;
+        d.rows = start.back();
+        d.cols.n = S_.cols.n;
+        uint64_t *keys = ctx_->alloc<uint64_t>(std::max<uint64_t>(d.rows, 1));
+        keep.push_back(keys);
+        OutColSet cols = alloc_cols(ctx_, S_.cols, d.rows, keep);
+        d.keys = keys;
+        d.cols = as_in(cols);
+        if (d.rows == 0)
+            return d;
+        Scratch scr(ctx_);
+        const uint32_t n_seg = static_cast<uint32_t>(src.size());
+        uint64_t *dstart = scr.get<uint64_t>(n_seg + 1);
+        uint64_t *drow = scr.get<uint64_t>(n_seg);
+        uint32_t *dsrc = scr.get<uint32_t>(n_seg);
+        std::vector<GatherSource> h_srcs(runs.size());
+        for (size_t k = 0; k < runs.size(); ++k) {
+            h_srcs[k].keys = runs[k]->rel.keys;
+            for (uint32_t c = 0; c < S_.cols.n; ++c)
+                h_srcs[k].cols[c] = runs[k]->rel.cols.ptr[c];
+        }
+        GatherSource *dsrcs = scr.get<GatherSource>(h_srcs.size());
+        HIPCHK(hipMemcpyAsync(dstart, start.data(), start.size() * 8, hipMemcpyHostToDevice, ctx_->stream));
+        HIPCHK(hipMemcpyAsync(drow, row.data(), row.size() * 8, hipMemcpyHostToDevice, ctx_->stream));
+        HIPCHK(hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, ctx_->stream));
+        HIPCHK(hipMemcpyAsync(dsrcs, h_srcs.data(), h_srcs.size() * sizeof(GatherSource), hipMemcpyHostToDevice,
+                              ctx_->stream));
+        gather_segments(ctx_->stream, dstart, n_seg, dsrc, drow, dsrcs, d.rows, keys, cols, tm);
+        ctx_->sync(); // the segment lists are freed with `scr`
+        return d;
+    }
+
+    // src/PHJBep.cpp:397-444 refinePartition, for every bucket of Q at once:
+    // the buckets' rows from every batch run, through the remaining passes
+    // (refine_to_leaves mode), into one new leaf run.
+    void refine(const std::vector<uint32_t> &Q) {
+        if (Q.empty())
+            return;
+        while (leaf_runs_.size() >= kBepMaxRuns) {
+            const BepRun &old = *leaf_runs_.front();
+            std::vector<uint32_t> E;
+            for (uint32_t q = 0; q < P_; ++q)
+                if (old.qidx[q] >= 0)
+                    for (uint32_t d = 0; d < lpp_; ++d) {
+                        const uint32_t g = static_cast<uint32_t>(old.qidx[q]) * lpp_ + d;
+                        if (!old.done[g] && old.len(g) > 0)
+                            E.push_back(q * lpp_ + d);
+                    }
+            evict(E);
+        }
+        HIPCHK(hipEventRecord(ev_.a, ctx_->stream));
+        auto leaf = std::make_unique<BepRun>(ctx_);
+        Held staging(ctx_);
+        std::vector<uint64_t> q_rows;
+        const DevRel st = gather(
+            batch_runs_, static_cast<uint32_t>(Q.size()), [&](const BepRun &, uint32_t i) { return int64_t(Q[i]); },
+            staging, q_rows, ctx_->tm());
+        const uint32_t nq = static_cast<uint32_t>(Q.size());
+        if (rest_.empty()) {
+            // one pass in all: the buckets are the leaves (src/RadixPartition.cpp:437-447)
+            leaf->rel = st;
+            for (void *p : staging)
+                leaf->bufs.push_back(p);
+            staging.clear();
+            leaf->off.assign(nq + 1, 0);
+            for (uint32_t i = 0; i < nq; ++i)
+                leaf->off[i + 1] = leaf->off[i] + static_cast<uint32_t>(q_rows[i]);
+        } else {
+            std::vector<uint32_t> seg(nq + 1, 0);
+            for (uint32_t i = 0; i < nq; ++i)
+                seg[i + 1] = seg[i] + static_cast<uint32_t>(q_rows[i]);
+            Scratch scr(ctx_);
+            uint32_t *dseg = scr.get<uint32_t>(nq + 1);
+            HIPCHK(hipMemcpyAsync(dseg, seg.data(), (nq + 1) * 4ull, hipMemcpyHostToDevice, ctx_->stream));
+            const Partitioned LP = radix_partition(ctx_, st, dseg, nq, p1_, rest_, leaf->bufs, "refine", flags_);
+            staging.free_all();
+            leaf->rel.rows = st.rows;
+            leaf->rel.keys = LP.keys;
+            leaf->rel.cols = as_in(LP.cols);
+            leaf->off.resize(static_cast<size_t>(nq) * lpp_ + 1);
+            HIPCHK(hipMemcpyAsync(leaf->off.data(), LP.off, leaf->off.size() * 4, hipMemcpyDeviceToHost, ctx_->stream));
+        }
+        HIPCHK(hipEventRecord(ev_.b, ctx_->stream));
+        ctx_->sync();
+        shuffle_ms_ += ev_.ms();
+        leaf->done.assign(static_cast<size_t>(nq) * lpp_, 0);
+        leaf->qidx.assign(P_, -1);
+        for (uint32_t i = 0; i < nq; ++i) {
+            const uint32_t q = Q[i];
+            leaf->qidx[q] = static_cast<int32_t>(i);
+            for (uint32_t d = 0; d < lpp_; ++d) {
+                const uint32_t cnt = leaf->len(i * lpp_ + d);
+                leaf_rows_[q * lpp_ + d] += cnt;
+                global_leaf_ += cnt;
+            }
+            // the bucket's rows have left every batch run
+            for (auto &r : batch_runs_)
+                if (!r->done[q]) {
+                    r->done[q] = 1;
+                    r->live -= r->len(q);
+                }
+            unrefined_ -= bucket_rows_[q];
+            bucket_rows_[q] = 0;
+        }
+        leaf->live = st.rows;
+        refinements_ += nq;
+        batch_runs_.erase(std::remove_if(batch_runs_.begin(), batch_runs_.end(),
+                                         [](const std::unique_ptr<BepRun> &r) { return r->live == 0; }),
+                          batch_runs_.end());
+        leaf_runs_.push_back(std::move(leaf));
+    }
+
+    // src/PHJBep.cpp:535-568: evict round-robin until below the low-water mark.
+    void evict_to_low_water() {
+        if (global_leaf_ < l_hi_)
+            return;
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<uint32_t> E;
+        uint64_t left = global_leaf_;
+        for (uint32_t scanned = 0; scanned < leaves_ && left >= l_lo_; ++scanned) {
+            const uint32_t L = cursor_;
+            cursor_ = cursor_ + 1 == leaves_ ? 0 : cursor_ + 1;
+            if (leaf_rows_[L] == 0)
+                continue; // the reference's relaxed pre-check skips empty leaves
+            E.push_back(L);
+            left -= leaf_rows_[L];
+        }
+        overhead_ms_ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        std::sort(E.begin(), E.end());
+        evict(E);
+    }
+
+    // src/PHJBep.cpp:453-505 tryEvictOneLeaf, for every leaf of E at once: the
+    // leaves' buffered rows from every leaf run, joined against their tables,
+    // then dropped.
+    void evict(const std::vector<uint32_t> &E) {
+        if (E.empty())
+            return;
+        Held staging(ctx_);
+        std::vector<uint64_t> l_rows;
+        HIPCHK(hipEventRecord(ev_.a, ctx_->stream));
+        const DevRel st = gather(
+            leaf_runs_, static_cast<uint32_t>(E.size()),
+            [&](const BepRun &r, uint32_t i) {
+                const int32_t qi = r.qidx[E[i] / lpp_];
+                return qi < 0 ? int64_t(-1) : int64_t(qi) * lpp_ + E[i] % lpp_;
+            },
+            staging, l_rows, ctx_->tm());
+        HIPCHK(hipEventRecord(ev_.b, ctx_->stream));
+        ctx_->sync();
+        overhead_ms_ += ev_.ms();
+        // the staged rows as a partitioned relation over all leaves
+        std::vector<uint32_t> off(static_cast<size_t>(leaves_) + 1, 0);
+        for (size_t i = 0, L = 0; L < leaves_; ++L) {
+            off[L + 1] = off[L];
+            if (i < E.size() && E[i] == L)
+                off[L + 1] += static_cast<uint32_t>(l_rows[i++]);
+        }
+        Scratch scr(ctx_);
+        uint32_t *doff = scr.get<uint32_t>(off.size());
+        HIPCHK(hipMemcpyAsync(doff, off.data(), off.size() * 4, hipMemcpyHostToDevice, ctx_->stream));
+        Partitioned SP;
+        SP.rows = st.rows;
+        SP.keys = const_cast<uint64_t *>(st.keys);
+        SP.cols.n = st.cols.n;
+        for (uint32_t c = 0; c < st.cols.n; ++c) {
+            SP.cols.types[c] = st.cols.types[c];
+            SP.cols.ptr[c] = const_cast<void *>(st.cols.ptr[c]);
+        }
+        SP.off = doff;
+        SP.parts = leaves_;
+        HIPCHK(hipEventRecord(ev_.a, ctx_->stream));
+        {
+            LeafJoinPlan plan = plan_;
+            plan.two_phase = plan.two_phase || repeated_keys_;
+            LeafJoin lj(ctx_, RP_, SP, plan, repeated_keys_ ? nullptr : img_);
+            uint64_t add;
+            while (true) {
+                if (n_out_ + lj.bound() > out_.cap)
+                    grow_output(ctx_, out_, n_out_, std::max(n_out_ + lj.bound(), 2 * out_.cap));
+                add = lj.emit(out_, n_out_);
+                if (add != LeafJoin::kRetry)
+                    break;
+                repeated_keys_ = true; // later rounds go straight to the two-phase join
+            }
+            n_out_ += add;
+            big_build_ms_ += lj.big_build_ms();
+            probe_ms_ -= lj.big_build_ms();
+        }
+        HIPCHK(hipEventRecord(ev_.b, ctx_->stream));
+        ctx_->sync();
+        probe_ms_ += ev_.ms();
+        for (size_t i = 0; i < E.size(); ++i) {
+            const uint32_t L = E[i];
+            for (auto &r : leaf_runs_) {
+                const int32_t qi = r->qidx[L / lpp_];
+                if (qi < 0)
+                    continue;
+                const uint32_t g = static_cast<uint32_t>(qi) * lpp_ + L % lpp_;
+                if (!r->done[g]) {
+                    r->done[g] = 1;
+                    r->live -= r->len(g);
+                }
+            }
+            global_leaf_ -= leaf_rows_[L];
+            leaf_rows_[L] = 0;
+        }
+        evictions_ += E.size();
+        leaf_runs_.erase(std::remove_if(leaf_runs_.begin(), leaf_runs_.end(),
+                                        [](const std::unique_ptr<BepRun> &r) { return r->live == 0; }),
+                         leaf_runs_.end());
+    }
+
+    phj_ctx *ctx_;
+    const Partitioned &RP_;
+    const DevRel &S_;
+    uint32_t p1_;
+    std::vector<uint32_t> rest_;
+    LeafJoinPlan plan_;
+    uint32_t flags_;
+    const LeafTableImages *img_;
+    OutBuffers &out_;
+    uint32_t P_ = 0, leaves_ = 0, lpp_ = 0;
+    uint64_t rb_ = 0, u_hi_ = 0, u_lo_ = 0, l_hi_ = 0, l_lo_ = 0, batch_ = 0;
+    std::vector<uint64_t> bucket_rows_, leaf_rows_;
+    uint64_t unrefined_ = 0, global_leaf_ = 0;
+    uint32_t cursor_ = 0;
+    std::vector<std::unique_ptr<BepRun>> batch_runs_, leaf_runs_;
+    bool repeated_keys_ = false;
+    uint64_t n_out_ = 0, evictions_ = 0, refinements_ = 0, peak_ = 0;
+    double shuffle_ms_ = 0, probe_ms_ = 0, overhead_ms_ = 0, big_build_ms_ = 0;
+    EventPair ev_;
+};
+
 void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, const uint8_t *pb, uint32_t np,
                  uint64_t budget_bytes, const phj_tuning &tun, phj_join_result *res) {
     validate_rel(&b);
     validate_rel(&p);
     const uint32_t total = std::min(validate_pass_bits(pb, np), kDeviceMaxTotalBits);
-    const std::vector<uint32_t> passes = device_passes(total);
+    // pass 1 keeps the configuration's first pass bits (at most one device pass)
+    const uint32_t p1 = std::min({static_cast<uint32_t>(pb[0]), kDeviceMaxPassBits, total});
+    const std::vector<uint32_t> rest = total > p1 ? device_passes(total - p1) : std::vector<uint32_t>{};
+    std::vector<uint32_t> passes{p1};
+    passes.insert(passes.end(), rest.begin(), rest.end());
     std::memset(res, 0, sizeof(*res));
     Scratch scr(ctx);
     const DevRel R = stage_relation(ctx, scr, b);
@@ -1153,8 +1572,6 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     uint64_t row_bytes = 8;
     for (uint32_t c = 0; c < S.cols.n; ++c)
         row_bytes += type_size(S.cols.types[c]);
-    // Two pass buffers of the slice are live at once.
-    const uint64_t slice_rows = std::max<uint64_t>(budget_bytes / (2 * row_bytes), radix_tile_rows());
 
     EventPair e2e, ebs;
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
@@ -1162,81 +1579,31 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     Held keep(ctx);
     const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
-
-    double shuffle_ms = 0, probe_ms = 0, big_build_ms = 0;
-    // With several small slices every slice would rebuild every leaf table
-    // for a few probe rows per leaf: when all leaves fit LDS and no key
-    // repeats, the tables are built once (the build phase) and every slice
-    // looks up in them in HBM.
+    // Eager leaf tables (src/PHJBep.cpp:157-231): built once in HBM when every
+    // leaf fits LDS, no key repeats and an eviction round holds few probe rows
+    // per leaf; otherwise each round's work items build theirs in LDS.
+    double build_ms = 0;
     LeafTableImages images;
-    const bool use_images = S.rows > slice_rows && build_table_images(ctx, RP, plan, slice_rows, keep, images, big_build_ms);
-    uint64_t refinements = 0, peak_bytes = 0;
-    // evictions: non-empty (slice, leaf) pairs, counted on the device over all
-    // slices; slice timings are read after the last slice (no wait per slice)
-    unsigned long long *nonempty = scr.get<unsigned long long>(1);
-    launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(nonempty), 1, 0);
-    std::vector<std::unique_ptr<EventPair>> shuffle_ev, join_ev;
-    std::vector<double> join_build_ms;
-    const uint32_t p1_parts = 1u << passes[0];
-    OutBuffers out;
-    // Output grows geometrically; each slice's exact row count is known
-    // before its rows are written.
-    out = alloc_output(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1));
-    uint64_t n = 0;
-    for (uint64_t a = 0; a < S.rows; a += slice_rows) {
-        DevRel slice = S;
-        slice.rows = std::min(slice_rows, S.rows - a);
-        slice.keys = S.keys + a;
-        for (uint32_t c = 0; c < S.cols.n; ++c)
-            slice.cols.ptr[c] = static_cast<const uint8_t *>(S.cols.ptr[c]) + a * type_size(S.cols.types[c]);
-        shuffle_ev.push_back(std::make_unique<EventPair>());
-        join_ev.push_back(std::make_unique<EventPair>());
-        EventPair &es = *shuffle_ev.back(), &ej = *join_ev.back();
-        HIPCHK(hipEventRecord(es.a, ctx->stream));
-        std::vector<void *> sk;
-        const Partitioned SP = radix_partition(ctx, slice, nullptr, 1, 0, passes, sk, "probe", radix_flags(tun));
-        HIPCHK(hipEventRecord(es.b, ctx->stream));
-        HIPCHK(hipEventRecord(ej.a, ctx->stream));
-        double bb = 0;
-        {
-            LeafJoin lj(ctx, RP, SP, plan, use_images ? &images : nullptr);
-            uint64_t add;
-            do {
-                if (n + lj.bound() > out.cap)
-                    grow_output(ctx, out, n, std::max(n + lj.bound(), 2 * out.cap));
-            } while ((add = lj.emit(out, n)) == LeafJoin::kRetry);
-            n += add;
-            bb = lj.big_build_ms();
-        }
-        HIPCHK(hipEventRecord(ej.b, ctx->stream));
-        k_count_nonempty<<<static_cast<uint32_t>((SP.parts + 255) / 256), 256, 0, ctx->stream>>>(
-            SP.off, static_cast<uint32_t>(SP.parts), nonempty);
-        refinements += passes.size() > 1 ? p1_parts : 0;
-        peak_bytes = std::max<uint64_t>(peak_bytes, 2 * slice.rows * row_bytes);
-        join_build_ms.push_back(bb);
-        for (void *k : sk)
-            ctx->arena.free(k);
-    }
+    const uint64_t round_rows = std::max<uint64_t>(budget_bytes * 5 / 8 / row_bytes * 3 / 8, 1);
+    const bool use_images = build_table_images(ctx, RP, plan, round_rows, keep, images, build_ms);
+    OutBuffers out = alloc_output(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1));
+    BepSchedule sched(ctx, RP, S, p1, rest, budget_bytes, plan, radix_flags(tun), use_images ? &images : nullptr, out);
+    sched.run();
     HIPCHK(hipEventRecord(e2e.b, ctx->stream));
-    const uint64_t evictions = read_scalar(ctx, reinterpret_cast<const uint64_t *>(nonempty)); // waits for the stream
-    for (size_t i = 0; i < join_ev.size(); ++i) {
-        shuffle_ms += shuffle_ev[i]->ms();
-        probe_ms += join_ev[i]->ms() - join_build_ms[i];
-        big_build_ms += join_build_ms[i];
-    }
+    ctx->sync();
     keep.free_all();
     set_phase(res->build_shuffle, ebs.ms(), R.rows);
-    set_phase(res->build, big_build_ms, R.rows);
-    set_phase(res->probe_shuffle, shuffle_ms, S.rows);
-    set_phase(res->probe, probe_ms, S.rows);
-    set_phase(res->eviction_overhead, 0.0, S.rows);
+    set_phase(res->build, build_ms + sched.big_build_ms(), R.rows);
+    set_phase(res->probe_shuffle, sched.shuffle_ms(), S.rows);
+    set_phase(res->probe, sched.probe_ms(), S.rows);
+    set_phase(res->eviction_overhead, sched.overhead_ms(), S.rows);
     res->e2e_wall_ms = e2e.ms();
     res->bep_budget_mib = budget_bytes >> 20;
-    res->bep_evictions = evictions;
-    res->bep_refinements = refinements;
-    res->bep_peak_bytes = peak_bytes;
+    res->bep_evictions = sched.evictions();
+    res->bep_refinements = sched.refinements();
+    res->bep_peak_bytes = sched.peak_bytes();
     res->bep_build_skip_retries = 0;
-    ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, n);
+    ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, sched.out_rows());
 }
 
 // ---- Batched PHJ-PURE joins ----

@@ -38,6 +38,31 @@ template <class T> __global__ void k_fill(T *__restrict__ p, uint64_t n, T v) {
         p[i] = v;
 }
 
+// Rows gathered by a segment list: output rows [seg_start[g], seg_start[g+1])
+// are rows seg_row[g] .. of source relation seg_src[g] (all sources share one
+// schema). Each thread finds its row's segment by binary search; neighbouring
+// rows of a segment are neighbours in both the source and the output.
+__global__ __launch_bounds__(256) void k_gather_segments(const uint64_t *__restrict__ seg_start, uint32_t n_seg,
+                                                         const uint32_t *__restrict__ seg_src,
+                                                         const uint64_t *__restrict__ seg_row,
+                                                         const GatherSource *__restrict__ srcs, uint32_t n_cols,
+                                                         uint64_t *__restrict__ out_keys, OutColSet out) {
+    const uint64_t total = seg_start[n_seg];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t g = upper_bound_idx(seg_start, n_seg, i);
+        const GatherSource &src = srcs[seg_src[g]];
+        const uint64_t r = seg_row[g] + (i - seg_start[g]);
+        out_keys[i] = src.keys[r];
+        for (uint32_t c = 0; c < n_cols; ++c) {
+            uint64_t lo, hi;
+            const uint32_t t = col_type(out, c);
+            load_elem(src.cols[c], t, r, lo, hi);
+            store_elem(col_ptr(out, c), t, i, lo, hi);
+        }
+    }
+}
+
 // Order-independent checksum: every output row's bytes (key | build cols |
 // probe cols, src/Reference.cpp:22-43) are digested by two chained SplitMix64
 // lanes; lane sums and the row count are accumulated mod 2^64.
@@ -119,6 +144,16 @@ void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v) {
         return;
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 256 * 32));
     k_fill<uint64_t><<<grid, 256, 0, s>>>(p, n, v);
+}
+
+void gather_segments(hipStream_t s, const uint64_t *seg_start, uint32_t n_seg, const uint32_t *seg_src,
+                     const uint64_t *seg_row, const GatherSource *srcs, uint64_t total_rows, uint64_t *out_keys,
+                     const OutColSet &out, KernelTimer *tm) {
+    if (total_rows == 0 || n_seg == 0)
+        return;
+    TimedScope ts(tm, "gather_segments", s);
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((total_rows + 255) / 256, 256 * 32));
+    k_gather_segments<<<grid, 256, 0, s>>>(seg_start, n_seg, seg_src, seg_row, srcs, out.n, out_keys, out);
 }
 
 void checksum_columns(hipStream_t s, uint64_t n, const uint64_t *keys, const ColSet &left, const ColSet &right,

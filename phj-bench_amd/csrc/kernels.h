@@ -229,6 +229,18 @@ void ht_probe_emit(hipStream_t s, const GlobalHT &ht, HtMode mode, const HtRows 
 void ht_find(hipStream_t s, const GlobalHT &ht, const uint64_t *keys, uint64_t n, phjd::Cell16 *unused, uint32_t *heads_rows,
              KernelTimer *tm);
 
+// ---------------------------------------------------------------- row gathers
+// One source relation of gather_segments (device array of these).
+struct GatherSource {
+    const uint64_t *keys;
+    const void *cols[kMaxCols];
+};
+// Output rows [seg_start[g], seg_start[g+1]) (n_seg + 1 entries, device) are
+// rows seg_row[g] .. of srcs[seg_src[g]]; out's column types are the sources'.
+void gather_segments(hipStream_t s, const uint64_t *seg_start, uint32_t n_seg, const uint32_t *seg_src,
+                     const uint64_t *seg_row, const GatherSource *srcs, uint64_t total_rows, uint64_t *out_keys,
+                     const OutColSet &out, KernelTimer *tm);
+
 // ---------------------------------------------------------------- checksum / generators
 void checksum_columns(hipStream_t s, uint64_t n, const uint64_t *keys, const ColSet &left, const ColSet &right,
                       unsigned long long *acc3, KernelTimer *tm);

@@ -112,7 +112,7 @@ def test_batch_config4_mix(ctx):
 @pytest.mark.parametrize("bits", [[16, 16], [13, 12], [9, 8, 8], [16, 9]])
 def test_radix_configs_beyond_device_bits(ctx, bits):
     """25..32 total bits are valid in the reference (src/CLI.cpp:350-355); PHJ
-    and PHJ-PURE join through the device's 24-bit plan with the same rows."""
+    and PHJ-PURE join through the device's 21-bit plan with the same rows."""
     spec = O.make_spec(seed=19, build_rows=20000, probe_rows=90000, distribution=0, match_rate=0.6,
                        build_types=(G.U32,), probe_types=(G.U64, G.U8))
     ob, op = O.gen(spec)

@@ -269,21 +269,20 @@ def test_fused_and_two_phase_leaf_join(ctx, pass_bits, flags):
     res.release()
     res = ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig(pass_bits), bep_budget_mib=1, tuning=dict(flags=flags))
     assert res.checksum() == O.checksum_rows(want)
-    # 1 MiB holds ~29K probe rows per slice, so this join runs ~10 slices with
-    # more probe rows per leaf than build rows: every work item builds its
-    # own table (no prebuilt tables, so no separate build phase).
-    assert res.build.wall_ms == 0
+    # 1 MiB buffers ~29K probe rows: the 300K probe rows are refined and
+    # evicted many times over
+    assert res.bep_refinements > (1 << pass_bits[0]) and res.bep_evictions > (1 << sum(pass_bits))
     res.release()
 
 
 @pytest.mark.parametrize("half_empty", [False, True])
-def test_bep_sparse_slices_read_tables_in_hbm(ctx, half_empty):
-    """PHJ best-effort in slices of ~29K probe rows against 256 leaves of
-    ~1,200 build rows: with far fewer probe rows per leaf than build rows, the
-    slices look up in the prebuilt leaf tables where they lie in HBM. With
-    half_empty, no build key hashes into the upper half of the leaves, so half
-    the probe rows (and possibly the sentinels) land in leaves without a
-    table."""
+def test_bep_sparse_rounds_read_tables_in_hbm(ctx, half_empty):
+    """PHJ best-effort under a 1 MiB budget against 256 leaves of ~1,200 build
+    rows: an eviction round holds far fewer probe rows per leaf than build rows,
+    so the leaf tables are built once up front (the build phase) and every
+    round looks up in them where they lie in HBM. With half_empty, no build key
+    hashes into the upper half of the leaves, so half the probe rows (and
+    possibly the sentinels) land in leaves without a table."""
     rng = np.random.default_rng(23)
     ob, op = unique_with_sentinels(rng, 600000 if half_empty else 300000, 600000)
     if half_empty:
@@ -293,13 +292,49 @@ def test_bep_sparse_slices_read_tables_in_hbm(ctx, half_empty):
     res = ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig([4, 4]), bep_budget_mib=1)
     assert res.build.wall_ms > 0  # tables built once
     assert res.checksum() == want
-    # slices of 2^20 // (2 * 18) probe rows; each slice's rows fill all 256
-    # leaves; the 8 bits run as one device pass, so nothing is refined
-    slices = -(-op.keys.size // ((1 << 20) // 36))
-    assert slices == 21
-    assert res.bep_evictions == slices * 256
-    assert res.bep_refinements == 0
+    # 600K probe rows of 18 bytes through 512 KiB of unrefined buffer and
+    # 640 KiB of leaf buffer: every bucket and leaf is refined and evicted
+    # many times
+    assert res.bep_refinements > 16 * 5 and res.bep_evictions > 256 * 5
+    assert 0 < res.bep_peak_bytes <= (1 << 20)
     res.release()
+
+
+def bep_groups(keys, bits):
+    return np.unique(O.int_hash64(keys) >> np.uint64(64 - bits)).size
+
+
+@pytest.mark.parametrize("pass_bits", [[4, 4], [8, 8], [6], [3, 3, 3]])
+def test_bep_schedule_counters(ctx, pass_bits):
+    """The best-effort schedule (src/PHJBep.cpp:397-695) in its counters: with a
+    budget that buffers every probe row, each non-empty pass-1 bucket is
+    refined once and each non-empty leaf evicted once (the final refine and
+    drain); with a small budget they are refined and evicted repeatedly, the
+    buffered leaf rows stay within the budget, and a smaller budget never
+    evicts less. The rows equal the oracle's every time."""
+    spec = O.make_spec(seed=41, build_rows=40000, probe_rows=400000, distribution=0, match_rate=0.7,
+                       build_types=(G.U32,), probe_types=(G.U64, G.U16))
+    ob, op = O.gen(spec)
+    want = O.checksum_rows(O.serialize_sorted(O.reference_join(ob, op)))
+    total = sum(pass_bits)
+    buckets, leaves = bep_groups(op.keys, pass_bits[0]), bep_groups(op.keys, total)
+    rb = 8 + 8 + 2
+    seen = []
+    for mib in (4096, 4, 1):
+        r = ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig(pass_bits), bep_budget_mib=mib)
+        assert r.checksum() == want
+        assert r.bep_budget_mib == mib and r.bep_build_skip_retries == 0
+        if mib == 4096:
+            assert (r.bep_refinements, r.bep_evictions, r.bep_peak_bytes) == (buckets, leaves, 0)
+        else:
+            assert r.bep_refinements >= buckets and r.bep_evictions > leaves
+            # leaf rows are sampled after each batch, below the leaf high-water
+            # mark plus one refine's rows (at most the unrefined high-water mark)
+            assert 0 < r.bep_peak_bytes <= (mib << 20) * 5 // 8 + (mib << 20) // 2 + rb * 4096
+            assert r.eviction_overhead.wall_ms > 0
+        seen.append(r.bep_evictions)
+        r.release()
+    assert seen == sorted(seen)
 
 
 @pytest.mark.parametrize("chunk", [700, 5000])
