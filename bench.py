@@ -38,27 +38,66 @@ def widths(types):
     return sum(G.TYPE_SIZE[t] for t in types)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_threads():
+    """Threads the CPU baseline uses: the process's CPU share (OMP_NUM_THREADS
+    when set, as on a shared GPU host), else the cores it may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(scale_log2, threads):
-    """The compiled reference's runPHJ (oracle/_ref) on a bounded sample of the
-    same workload, on this host's cores."""
+    """The compiled reference (oracle/_ref) on this host's cores, timed by its
+    own e2e_wall_ms (README.md:144-216), best of 2 after a warm-up:
+    - value: its runPHJ on a bounded sample of the headline workload
+      (2^22 x 2^25 rows, same distribution, schema and pass_bits {8,8});
+    - config1: CHJ and PHJ-PURE on 2^20 x 2^24 in full (distinct uniform u64
+      build keys in [1, 2^63), every probe row matching once, u64 payloads,
+      pass_bits {8}), at `threads` threads and at 1 thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
     if not O.ref_available():
         return None
+
+    def best_ms(scheme, b, p, bits, t):
+        O.ref_run(scheme, b, p, bits, threads=t, want_checksum=False)  # warm-up
+        return min(O.ref_run(scheme, b, p, bits, threads=t, want_checksum=False).e2e_wall_ms for _ in range(2))
+
     rb, rp = 22 - scale_log2, 25 - scale_log2
     spec = O.make_spec(seed=42, build_rows=1 << rb, probe_rows=1 << rp, distribution=0, match_rate=0.5,
                        build_types=(G.U32, G.U64), probe_types=(G.U64, G.U128))
     b, p = O.gen(spec, threads=threads)
-    O.ref_run(3, b, p, (8, 8), threads=threads, want_checksum=False)  # warm-up
-    runs = []
-    for _ in range(2):
-        r = O.ref_run(3, b, p, (8, 8), threads=threads, want_checksum=False)
-        runs.append(r.e2e_wall_ms)
-    ms = min(runs)
-    return {"value": (b.rows + p.rows) / (ms / 1e3), "unit": "tuples/s", "cores": threads, "kind": "reference",
+    ms = best_ms(3, b, p, (8, 8), threads)
+    log(f"CPU baseline: reference runPHJ 2^{rb} x 2^{rp} on {threads} threads: {ms:.1f} ms")
+    del b, p
+    c1 = O.make_spec(seed=42, build_rows=1 << 20, probe_rows=1 << 24, distribution=0, match_rate=1.0,
+                     build_types=(G.U64,), probe_types=(G.U64,))
+    b1, p1 = O.gen(c1, threads=threads)
+    config1 = {"workload": "2^20 x 2^24, distinct uniform u64 build keys, 100% match, u64 payloads, pass_bits {8}",
+               "tuples": b1.rows + p1.rows}
+    for name, scheme in (("chj", 1), ("phj_pure", 3)):
+        for t in sorted({threads, 1}, reverse=True):
+            m = best_ms(scheme, b1, p1, (8,), t)
+            config1[f"{name}_{t}t_ms"] = round(m, 2)
+            config1[f"{name}_{t}t_tuples_per_s"] = (b1.rows + p1.rows) / (m / 1e3)
+            log(f"CPU baseline: reference {name} config 1 on {t} threads: {m:.1f} ms")
+    return {"value": ((1 << rb) + (1 << rp)) / (ms / 1e3),
+            "unit": "tuples/s", "cores": threads, "kind": "reference", "cpu_model": cpu_model(),
             "sample": f"reference runPHJ (oracle/_ref, its own e2e_wall_ms), 2^{rb} x 2^{rp} rows of the same "
-                      f"distribution and schema, pass_bits {{8,8}}, best of 2 after 1 warm-up"}
+                      f"distribution and schema, pass_bits {{8,8}}, best of 2 after 1 warm-up",
+            "config1": config1}
 
 
 def batched_bench(ctx, n_joins, seed=42):
@@ -215,29 +254,8 @@ def serving_bench(n_requests, rates, workers=4, budget_gb=200, scale_log2=0):
                         "of u32/u64/u128 per side, 30% of requests probe one of 64 resident build sides"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale-log2", type=int, default=0, help="shrink both sides by 2^k (testing only)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="also run one plain device copy of this size (HBM counter calibration under a profiler)")
-    ap.add_argument("--tune", default="", help="phj_tuning overrides, e.g. flags=12,join_threads=1024")
-    ap.add_argument("--batch-joins", type=int, default=4096, help="joins of the batched-join measurement (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=400, help="requests per offered rate of the serving "
-                                                                     "measurement (0: skip)")
-    ap.add_argument("--serve-rates", default="250,500,1000,2000,5000,10000,20000")
-    ap.add_argument("--no-skew", action="store_true", help="skip the skewed-join measurement")
-    args = ap.parse_args()
-    tuning = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(",") if kv)}
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs the join on a single MI355X; "
-              "multi-GPU runs are not part of it", file=sys.stderr)
-        sys.exit(2)
-
+def headline(args, tuning):
+    """The headline join (config 2): warm-up, timed steps, checksum, roofline."""
     rb, rp = 27 - args.scale_log2, 30 - args.scale_log2
     R, S = 1 << rb, 1 << rp
     btypes, ptypes = (G.U32, G.U64), (G.U64, G.U128)
@@ -324,6 +342,45 @@ def main():
     build.release()
     probe.release()
     ctx.close()  # each workload below runs in a context of its own, on a device with nothing cached
+    return dict(R=R, S=S, rb=rb, rp=rp, pass_bits=pass_bits, ms_per_step=ms_per_step, tuples_per_s=tuples_per_s,
+                best_e2e=best_e2e, b_alg=b_alg, out_rows=out_rows, parity=parity, phases=phases, ktimes=ktimes,
+                roofline=roofline, n=n)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale-log2", type=int, default=0, help="shrink both sides by 2^k (testing only)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one plain device copy of this size (HBM counter calibration under a profiler)")
+    ap.add_argument("--tune", default="", help="phj_tuning overrides, e.g. flags=12,join_threads=1024")
+    ap.add_argument("--batch-joins", type=int, default=4096, help="joins of the batched-join measurement (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests per offered rate of the serving "
+                                                                     "measurement (0: skip)")
+    ap.add_argument("--serve-rates", default="250,500,1000,2000,5000,10000,20000")
+    ap.add_argument("--no-skew", action="store_true", help="skip the skewed-join measurement")
+    ap.add_argument("--workloads", default="main,batch,skew,serve",
+                    help="comma-separated subset of main (the headline join), batch, skew, serve; a run without "
+                         "main prints value null (for profiling one workload)")
+    args = ap.parse_args()
+    wl = set(w for w in args.workloads.split(",") if w)
+    if "batch" not in wl:
+        args.batch_joins = 0
+    if "serve" not in wl:
+        args.serve_requests = 0
+    if "skew" not in wl:
+        args.no_skew = True
+    tuning = {k: int(v) for k, v in (kv.split("=") for kv in args.tune.split(",") if kv)}
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the join on a single MI355X; "
+              "multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    h = headline(args, tuning) if "main" in wl and args.steps > 0 else None
     batched = None
     if args.batch_joins > 0:
         bctx = G.Context(0)
@@ -337,45 +394,49 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = args.cpu_threads or host_threads()
         try:
             log(f"CPU baseline: compiled reference runPHJ on {threads} threads")
             cpu = cpu_baseline(args.scale_log2, threads)
         except Exception as e:  # reported, not fatal
             log(f"CPU baseline failed: {e}")
 
+    hv = h or {}
     line = {
         "metric": METRIC,
-        "value": tuples_per_s,
+        "value": hv.get("tuples_per_s"),
         "unit": "tuples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": hv.get("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (device-generated, seed 42)",
-        "config": {"workload": f"PHJ-PURE 2^{rb} x 2^{rp}, build u32+u64, probe u64+u128, 50% match, "
-                               f"pass_bits {{8,8}}, LF 0.5, 16-byte cells",
-                   "build_rows": R, "probe_rows": S, "pass_bits": pass_bits, "match_rate": 0.5},
-        "roofline": roofline,
+        "config": None if h is None else {
+            "workload": f"PHJ-PURE 2^{h['rb']} x 2^{h['rp']}, build u32+u64, probe u64+u128, 50% match, "
+                        f"pass_bits {{8,8}}, load factor 0.5; leaf tables in LDS as u64 keys + u16 rows "
+                        f"(~10 B per slot; the 16-byte (key, RowRefCell) cells are the HBM tables' layout)",
+            "build_rows": h["R"], "probe_rows": h["S"], "pass_bits": h["pass_bits"], "match_rate": 0.5},
+        "roofline": hv.get("roofline"),
         "cpu_baseline": cpu,
-        "e2e_ms_best": best_e2e,
-        "e2e_roofline_frac": b_alg / (best_e2e / 1e3) / HBM_PEAK,
-        "step_roofline_frac": b_alg / (ms_per_step / 1e3) / HBM_PEAK,
-        "algorithmic_bytes": b_alg,
-        "out_rows": out_rows,
-        "parity_checksum": parity,
-        "phase_ms_last": dict(zip(("build_shuffle", "probe_shuffle", "build", "probe"), phases[-1])),
-        "kernel_ms_per_step": {k: round(v[0] / n, 4) for k, v in sorted(ktimes.items())},
+        "e2e_ms_best": hv.get("best_e2e"),
+        "e2e_roofline_frac": None if h is None else h["b_alg"] / (h["best_e2e"] / 1e3) / HBM_PEAK,
+        "step_roofline_frac": None if h is None else h["b_alg"] / (h["ms_per_step"] / 1e3) / HBM_PEAK,
+        "algorithmic_bytes": hv.get("b_alg"),
+        "out_rows": hv.get("out_rows"),
+        "parity_checksum": hv.get("parity"),
+        "phase_ms_last": None if h is None else dict(zip(("build_shuffle", "probe_shuffle", "build", "probe"),
+                                                          h["phases"][-1])),
+        "kernel_ms_per_step": None if h is None else {k: round(v[0] / h["n"], 4) for k, v in sorted(h["ktimes"].items())},
         "batched": batched,
         "skew": skew,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    if (not parity or (batched is not None and not batched["parity_per_join"])
+    if ((h is not None and not h["parity"]) or (batched is not None and not batched["parity_per_join"])
             or (serving is not None and not serving["parity_alone_vs_mix"])
             or (skew is not None and not skew["schemes_agree"])):
         sys.exit(1)
