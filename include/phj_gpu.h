@@ -122,6 +122,8 @@ typedef struct phj_tuning {
 #define PHJ_TUNE_TWO_PHASE_JOIN 0x2u  /* leaf join as a count pass plus an exact-offset emit pass */
 #define PHJ_TUNE_CACHED_LOADS 0x4u    /* radix scatter reads its input with cached loads (default: nontemporal) */
 #define PHJ_TUNE_NT_STORES 0x8u       /* radix scatter writes with nontemporal stores */
+#define PHJ_TUNE_HASH_EVERY_PASS 0x10u /* each radix pass's histogram hashes the keys again instead of reading
+                                          the digit byte column the previous pass's scatter wrote */
 
 typedef struct phj_join_desc {
     uint32_t scheme; /* phj_scheme */

@@ -206,6 +206,10 @@ struct phj_ctx {
     // handle -> buffers shared by the results of one batch group (freed with the last)
     std::unordered_map<uint64_t, std::shared_ptr<std::vector<void *>>> shared;
     uint64_t next_handle = 1;
+    // Host-visible words a join reads back without draining the stream: a
+    // value copied there is waited for through `note_event` only.
+    uint32_t *pinned = nullptr;
+    hipEvent_t note_event = nullptr;
 
     KernelTimer *tm() { return timing ? &timer : nullptr; }
     void sync() {
@@ -344,7 +348,24 @@ struct Partitioned {
     OutColSet cols{};
     uint32_t *off = nullptr; // parts + 1
     uint64_t parts = 0;
+    int64_t max_part = -1;   // rows of the largest partition when known on the host, else -1
 };
+
+// Starts reading the largest partition of P back into the context's pinned
+// word; the copy completes in stream order, so work queued after it (the
+// probe side's passes) keeps the device busy while the host later waits for
+// this one value alone (max_part_wait).
+void max_part_start(phj_ctx *ctx, const Partitioned &P, Scratch &scr) {
+    uint32_t *d = scr.get<uint32_t>(1);
+    leaf_max_rows(ctx->stream, P.off, static_cast<uint32_t>(P.parts), d);
+    HIPCHK(hipMemcpyAsync(ctx->pinned, d, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->note_event, ctx->stream));
+}
+
+void max_part_wait(phj_ctx *ctx, Partitioned &P) {
+    HIPCHK(hipEventSynchronize(ctx->note_event));
+    P.max_part = *ctx->pinned;
+}
 
 OutColSet alloc_cols(phj_ctx *ctx, const ColSet &like, uint64_t rows, std::vector<void *> &track) {
     OutColSet o{};
@@ -388,12 +409,24 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
     out.rows = rows;
     uint32_t *cur_seg = seg;
     uint32_t cur_nseg = n_seg;
+    // Between two passes the scatter leaves the next pass's digit of every row
+    // in a byte column, which the next histogram reads instead of the keys.
+    Held dig_prev(ctx);
+    const uint8_t *dig_in = nullptr;
     for (size_t p = 0; p < passes.size(); ++p) {
         const uint32_t bits = passes[p];
         consumed += bits;
         const uint32_t shift = 64 - consumed;
         const uint64_t B = 1ull << bits;
         const uint64_t out_nseg = static_cast<uint64_t>(cur_nseg) * B;
+        PassDigits dig{dig_in, nullptr, 0, 0};
+        Held dig_next(ctx);
+        if (p + 1 < passes.size() && passes[p + 1] <= 8 && rows > 0 && !(flags & kRadixHashEveryPass)) {
+            dig.out = ctx->alloc<uint8_t>(rows);
+            dig_next.push_back(dig.out);
+            dig.next_shift = 64 - consumed - passes[p + 1];
+            dig.next_bits = passes[p + 1];
+        }
         Held bufs(ctx);
         uint64_t *nkeys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
         bufs.push_back(nkeys);
@@ -406,10 +439,13 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
         rs.hist = scr.get<uint32_t>(max_tiles * B);
         rs.block_sums = scr.get<uint32_t>(radix_max_chunks(rows, cur_nseg) * B);
         radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, uniform_bits(shift, bits), nseg, rs,
-                   flags, ctx->tm());
+                   flags, ctx->tm(), dig);
         // previous pass output is dead once this pass has consumed it
         prev_bufs.free_all();
         prev_bufs.swap(bufs);
+        dig_prev.free_all();
+        dig_prev.swap(dig_next);
+        dig_in = dig.out;
         cur_keys = nkeys;
         cur_cols = as_in(ncols);
         cur_seg = nseg;
@@ -566,7 +602,8 @@ LeafJoinPlan plan_of(const phj_tuning &t) {
 // Scatter load/store variant of a tuning record: nontemporal loads unless the
 // caller asks for cached ones (the input is read once per pass).
 uint32_t radix_flags(const phj_tuning &t) {
-    return ((t.flags & PHJ_TUNE_CACHED_LOADS) ? 0u : kRadixNtLoads) | ((t.flags & PHJ_TUNE_NT_STORES) ? kRadixNtStores : 0u);
+    return ((t.flags & PHJ_TUNE_CACHED_LOADS) ? 0u : kRadixNtLoads) | ((t.flags & PHJ_TUNE_NT_STORES) ? kRadixNtStores : 0u) |
+           ((t.flags & PHJ_TUNE_HASH_EVERY_PASS) ? kRadixHashEveryPass : 0u);
 }
 
 // Largest leaf (build rows) the LDS join paths take for a build side whose
@@ -610,6 +647,8 @@ public:
         uint32_t max_leaf;
         if (img) {
             max_leaf = img->max_leaf;
+        } else if (R.max_part >= 0) {
+            max_leaf = static_cast<uint32_t>(R.max_part);
         } else {
             uint32_t *dmax = scr_.get<uint32_t>(1);
             leaf_max_rows(ctx->stream, R.off, n_leaves, dmax);
@@ -894,15 +933,21 @@ void phj_pure_probe(phj_ctx *ctx, const Partitioned &RP, const ColSet &r_cols, c
                     Held &keep, phj_join_result *res) {
     const LeafJoinPlan plan = plan_of(tun);
     EventPair eps, ej;
+    Partitioned R = RP;
+    Scratch scr(ctx);
+    if (R.max_part < 0)
+        max_part_start(ctx, R, scr);
     HIPCHK(hipEventRecord(eps.a, ctx->stream));
     const Partitioned SP = radix_partition(ctx, S, nullptr, 1, 0, passes, keep, "probe", radix_flags(tun));
     HIPCHK(hipEventRecord(eps.b, ctx->stream));
     HIPCHK(hipEventRecord(ej.a, ctx->stream));
+    if (R.max_part < 0)
+        max_part_wait(ctx, R); // the probe passes are queued behind it
     double big_build_ms = 0;
     OutBuffers out;
     uint64_t n = 0;
     {
-        LeafJoin lj(ctx, RP, SP, plan);
+        LeafJoin lj(ctx, R, SP, plan);
         out = alloc_output(ctx, r_cols, S.cols, lj.bound());
         while ((n = lj.emit(out, 0)) == LeafJoin::kRetry) {
             free_output(ctx, out);
@@ -1062,6 +1107,8 @@ void register_build(phj_ctx *ctx, const phj_relation &b, const uint8_t *pb, uint
     for (uint32_t c = 0; c < R.cols.n; ++c)
         w += type_size(R.cols.types[c]);
     r->bytes = R.rows * w + (r->RP.parts + 1) * 4;
+    max_part_start(ctx, r->RP, scr); // known once, so probes from any context skip reading it back
+    max_part_wait(ctx, r->RP);
     ctx->sync();
 }
 
@@ -2136,8 +2183,12 @@ phj_status phj_ctx_create(int device, phj_ctx **out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
         return PHJ_ERR_HIP;
-    if (scratch_address(device, &ctx->scratch) != hipSuccess) {
+    if (scratch_address(device, &ctx->scratch) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&ctx->pinned), 64, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->note_event, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
+        if (ctx->pinned)
+            (void)hipHostFree(ctx->pinned);
         (void)hipStreamDestroy(ctx->stream);
         return PHJ_ERR_HIP;
     }
@@ -2156,6 +2207,8 @@ phj_status phj_ctx_destroy(phj_ctx *ctx) {
             ctx->arena.free(p);
     ctx->owned.clear();
     ctx->arena.release_cached();
+    (void)hipEventDestroy(ctx->note_event);
+    (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PHJ_OK;

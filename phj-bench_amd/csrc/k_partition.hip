@@ -121,8 +121,12 @@ __global__ void k_seg_single(uint32_t *seg_off, uint32_t rows) {
     seg_off[1] = rows;
 }
 
-// Per-tile digit counts, hist[tile * B + d].
-__global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ seg_off,
+// Per-tile digit counts, hist[tile * B + d]. With DIG the pass's digit of
+// every row comes from `digits` (one byte per row, written by the previous
+// pass's scatter) instead of hashing the keys: 1 byte read per row, not 8.
+template <bool DIG>
+__global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restrict__ keys, const uint8_t *__restrict__ digits,
+                                                       const uint32_t *__restrict__ seg_off,
                                                        const uint32_t *__restrict__ seg_first, uint32_t n_seg,
                                                        PassBits pb, uint32_t *__restrict__ hist) {
     constexpr int W = kPartThreads / 64;
@@ -139,17 +143,32 @@ __global__ __launch_bounds__(kPartThreads) void k_hist(const uint64_t *__restric
     const uint32_t wave = threadIdx.x / 64;
     // all loads first: the LDS atomics below would otherwise order each load
     // after the previous row's atomic
-    uint64_t k[kPartRpt];
+    if constexpr (DIG) {
+        uint32_t dg[kPartRpt];
 #pragma unroll
-    for (int r = 0; r < kPartRpt; ++r) {
-        const uint32_t idx = r * kPartThreads + threadIdx.x;
-        k[r] = __builtin_nontemporal_load(keys + L.row0 + min(idx, L.len - 1)); // past the tile: masked below
-    }
+        for (int r = 0; r < kPartRpt; ++r) {
+            const uint32_t idx = r * kPartThreads + threadIdx.x;
+            dg[r] = __builtin_nontemporal_load(digits + L.row0 + min(idx, L.len - 1)); // past the tile: masked below
+        }
 #pragma unroll
-    for (int r = 0; r < kPartRpt; ++r) {
-        const uint32_t idx = r * kPartThreads + threadIdx.x;
-        if (idx < L.len)
-            atomicAdd(&sub[wave][digit_of(int_hash64(k[r]), shift, bits)], 1u);
+        for (int r = 0; r < kPartRpt; ++r) {
+            const uint32_t idx = r * kPartThreads + threadIdx.x;
+            if (idx < L.len)
+                atomicAdd(&sub[wave][dg[r]], 1u);
+        }
+    } else {
+        uint64_t k[kPartRpt];
+#pragma unroll
+        for (int r = 0; r < kPartRpt; ++r) {
+            const uint32_t idx = r * kPartThreads + threadIdx.x;
+            k[r] = __builtin_nontemporal_load(keys + L.row0 + min(idx, L.len - 1)); // past the tile: masked below
+        }
+#pragma unroll
+        for (int r = 0; r < kPartRpt; ++r) {
+            const uint32_t idx = r * kPartThreads + threadIdx.x;
+            if (idx < L.len)
+                atomicAdd(&sub[wave][digit_of(int_hash64(k[r]), shift, bits)], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < B; d += kPartThreads) {
@@ -390,7 +409,11 @@ __device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__res
     write_staged<NTS, T>(sh, v, out, L, slot);
 }
 
-template <bool NTL, bool NTS>
+// With DIG the scatter also writes the next pass's digit of every row
+// (hash bits [next_shift, next_shift + next_bits), at most 8) as a byte
+// column in output order, so the next pass's histogram need not re-read and
+// re-hash the keys.
+template <bool NTL, bool NTS, bool DIG>
 __global__ __launch_bounds__(kScatThreads) __attribute__((amdgpu_waves_per_eu(8)))
 void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
                                                           uint64_t *__restrict__ keys_out, OutColSet cout,
@@ -398,7 +421,9 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
                                                           const uint32_t *__restrict__ seg_first, uint32_t n_seg,
                                                           PassBits pb,
                                                           const uint32_t *__restrict__ hist,
-                                                          const uint32_t *__restrict__ chunk_base) {
+                                                          const uint32_t *__restrict__ chunk_base,
+                                                          uint8_t *__restrict__ digits_out, uint32_t next_shift,
+                                                          uint32_t next_bits) {
     __shared__ ScatterShared sh;
     TileLoc L;
     if (!locate_tile(xcd_share(blockIdx.x, seg_first[n_seg]), seg_off, seg_first, n_seg, L))
@@ -413,6 +438,7 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
     // Rank each row inside its digit with one LDS atomic (rank << kDeviceMaxPassBits | digit).
     uint64_t key[kScatRpt];
     uint32_t slot[kScatRpt];
+    uint8_t next[kScatRpt];
 #pragma unroll
     for (int r = 0; r < kScatRpt; ++r) {
         const uint32_t idx = r * kScatThreads + threadIdx.x;
@@ -422,8 +448,11 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
     for (int r = 0; r < kScatRpt; ++r) {
         const uint32_t idx = r * kScatThreads + threadIdx.x;
         slot[r] = kNoSlot;
+        const uint64_t h = int_hash64(key[r]);
+        if constexpr (DIG)
+            next[r] = static_cast<uint8_t>(digit_of(h, next_shift, next_bits));
         if (idx < L.len) {
-            const uint32_t d = digit_of(int_hash64(key[r]), shift, bits);
+            const uint32_t d = digit_of(h, shift, bits);
             slot[r] = (atomicAdd(&sh.cursor[d], 1u) << kDeviceMaxPassBits) | d;
         }
     }
@@ -451,6 +480,8 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
     __syncthreads();
 
     write_staged<NTS, uint64_t>(sh, key, keys_out, L, slot);
+    if constexpr (DIG)
+        write_staged<NTS, uint8_t>(sh, next, digits_out, L, slot);
     for (uint32_t c = 0; c < cin.n; ++c) {
         switch (col_type(cin, c)) {
         case 0:
@@ -514,7 +545,8 @@ void launch_u32_to_u64(hipStream_t s, const uint32_t *in, uint64_t *out, uint64_
 
 void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
                 const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, const PassBits &pb,
-                uint32_t *out_seg_off, const RadixPassScratch &scr, uint32_t flags, KernelTimer *tm) {
+                uint32_t *out_seg_off, const RadixPassScratch &scr, uint32_t flags, KernelTimer *tm,
+                const PassDigits &dig) {
     const uint32_t bits = pb.bits;
     const uint64_t max_tiles = radix_max_tiles(rows, n_seg);
     const uint64_t max_chunks = radix_max_chunks(rows, n_seg);
@@ -524,7 +556,12 @@ void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint6
     }
     if (rows > 0) {
         TimedScope ts(tm, "radix_hist", s);
-        k_hist<<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, seg_off, scr.seg_tile_first, n_seg, pb, scr.hist);
+        if (dig.in)
+            k_hist<true><<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, dig.in, seg_off, scr.seg_tile_first, n_seg, pb,
+                                                                    scr.hist);
+        else
+            k_hist<false><<<grid8(max_tiles), kPartThreads, 0, s>>>(keys_in, nullptr, seg_off, scr.seg_tile_first, n_seg,
+                                                                     pb, scr.hist);
     }
     {
         TimedScope ts(tm, "radix_scan", s);
@@ -536,10 +573,12 @@ void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint6
     if (rows > 0) {
         TimedScope ts(tm, "radix_scatter", s);
         const bool ntl = flags & kRadixNtLoads, nts = flags & kRadixNtStores;
-        auto k = ntl ? (nts ? k_scatter<true, true> : k_scatter<true, false>)
-                     : (nts ? k_scatter<false, true> : k_scatter<false, false>);
+        auto k = dig.out ? (ntl ? (nts ? k_scatter<true, true, true> : k_scatter<true, false, true>)
+                                : (nts ? k_scatter<false, true, true> : k_scatter<false, false, true>))
+                         : (ntl ? (nts ? k_scatter<true, true, false> : k_scatter<true, false, false>)
+                                : (nts ? k_scatter<false, true, false> : k_scatter<false, false, false>));
         k<<<grid8(max_tiles), kScatThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg, pb,
-                                                     scr.hist, scr.block_sums);
+                                                     scr.hist, scr.block_sums, dig.out, dig.next_shift, dig.next_bits);
     }
 }
 

@@ -73,6 +73,7 @@ uint64_t radix_scan_blocks(uint64_t elems);
 // Scatter variants (phj_tuning.flags bits share these values).
 constexpr uint32_t kRadixNtLoads = 0x4;  // input columns read with nontemporal loads
 constexpr uint32_t kRadixNtStores = 0x8; // output columns written with nontemporal stores
+constexpr uint32_t kRadixHashEveryPass = 0x10; // every histogram hashes the keys (no digit column between passes)
 // Bit window of a pass. Uniform: every segment uses (shift, bits) and input
 // segment s yields output segments s * 2^bits + d. Per segment (sb != null):
 // sb[s] = shift_s | bits_s << 8 with bits_s <= bits, and output segment d of
@@ -84,9 +85,20 @@ struct PassBits {
     uint32_t bits;
 };
 inline PassBits uniform_bits(uint32_t shift, uint32_t bits) { return PassBits{nullptr, nullptr, shift, bits}; }
+// Digit columns between two passes of uniform windows: `in` holds this pass's
+// digit of every input row (one byte per row, from the previous pass), so the
+// histogram reads it instead of the keys; `out` receives the next pass's
+// digit (hash bits [next_shift, next_shift + next_bits), next_bits <= 8) of
+// every output row. Either may be null.
+struct PassDigits {
+    const uint8_t *in;
+    uint8_t *out;
+    uint32_t next_shift, next_bits;
+};
 void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint64_t *keys_out, const OutColSet &cout,
                 const uint32_t *seg_off, uint32_t n_seg, uint64_t rows, const PassBits &pb,
-                uint32_t *out_seg_off, const RadixPassScratch &scr, uint32_t flags, KernelTimer *tm);
+                uint32_t *out_seg_off, const RadixPassScratch &scr, uint32_t flags, KernelTimer *tm,
+                const PassDigits &dig = PassDigits{nullptr, nullptr, 0, 0});
 
 // Exclusive scan of n uint32 (device count: n = *count_ptr * mul when count_ptr != nullptr).
 void scan_u32(hipStream_t s, uint32_t *data, uint64_t max_n, const uint32_t *count_ptr, uint32_t mul,
