@@ -124,6 +124,8 @@ typedef struct phj_tuning {
 #define PHJ_TUNE_NT_STORES 0x8u       /* radix scatter writes with nontemporal stores */
 #define PHJ_TUNE_HASH_EVERY_PASS 0x10u /* each radix pass's histogram hashes the keys again instead of reading
                                           the digit byte column the previous pass's scatter wrote */
+#define PHJ_TUNE_NARROW_PROBE 0x20u    /* PHJ-PURE partitions the probe side as (key, row index) only and gathers
+                                          matched rows' probe columns from the unpartitioned relation */
 
 typedef struct phj_join_desc {
     uint32_t scheme; /* phj_scheme */

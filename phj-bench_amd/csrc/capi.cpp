@@ -371,8 +371,8 @@ OutColSet alloc_cols(phj_ctx *ctx, const ColSet &like, uint64_t rows, std::vecto
     OutColSet o{};
     o.n = like.n;
     for (uint32_t c = 0; c < like.n; ++c) {
-        o.types[c] = like.types[c];
-        o.ptr[c] = ctx->arena.alloc(std::max<uint64_t>(rows * type_size(like.types[c]), 1));
+        o.types[c] = like.types[c] == kTypeRowIndex ? 2u : like.types[c]; // synthesised row indices land as u32
+        o.ptr[c] = ctx->arena.alloc(std::max<uint64_t>(rows * type_size(o.types[c]), 1));
         track.push_back(o.ptr[c]);
     }
     return o;
@@ -695,6 +695,15 @@ public:
     // Output rows emit() may write: exact for the two-phase join, S.rows for
     // the fused one.
     uint64_t bound() const { return bound_; }
+    bool fused() const { return fused_; }
+    // The probe side is narrow (keys and row indices `rowid` into the
+    // unpartitioned columns `src`): matched rows gather their probe columns
+    // from there. Fused join only; a repeated build key makes emit() return
+    // kRetry without a two-phase plan.
+    void use_narrow(const uint32_t *rowid, const ColSet &src) {
+        a_.s_rowid = rowid;
+        a_.s_src = src;
+    }
     double big_build_ms() const { return big_build_ms_; }
 
     // Writes the join's rows from row `base` of `out` (capacity >= base +
@@ -718,6 +727,8 @@ public:
             if (h[1] == 0)
                 return rows;
             fused_ = false;
+            if (a_.s_rowid)
+                return kRetry; // a narrow probe side has no payload to emit from: the caller re-partitions
             plan_two_phase();
             return kRetry;
         }
@@ -937,23 +948,52 @@ void phj_pure_probe(phj_ctx *ctx, const Partitioned &RP, const ColSet &r_cols, c
     Scratch scr(ctx);
     if (R.max_part < 0)
         max_part_start(ctx, R, scr);
-    HIPCHK(hipEventRecord(eps.a, ctx->stream));
-    const Partitioned SP = radix_partition(ctx, S, nullptr, 1, 0, passes, keep, "probe", radix_flags(tun));
-    HIPCHK(hipEventRecord(eps.b, ctx->stream));
-    HIPCHK(hipEventRecord(ej.a, ctx->stream));
-    if (R.max_part < 0)
-        max_part_wait(ctx, R); // the probe passes are queued behind it
     double big_build_ms = 0;
     OutBuffers out;
     uint64_t n = 0;
-    {
+    // PHJ_TUNE_NARROW_PROBE: the probe side is partitioned as (key, row index)
+    // only, 12 bytes a row instead of its full width, and the fused join
+    // gathers the probe columns of matched rows from the unpartitioned
+    // relation. It needs every leaf in LDS and no repeated build key; failing
+    // either, the join runs again with the full-width probe side.
+    bool narrow = (tun.flags & PHJ_TUNE_NARROW_PROBE) != 0 && S.rows > 0;
+    for (bool done = false; !done;) {
+        DevRel Sp = S;
+        Held sk(ctx);
+        if (narrow) {
+            Sp.cols = ColSet{};
+            Sp.cols.n = 1;
+            Sp.cols.types[0] = kTypeRowIndex;
+        }
+        HIPCHK(hipEventRecord(eps.a, ctx->stream));
+        const Partitioned SP = radix_partition(ctx, Sp, nullptr, 1, 0, passes, sk, "probe", radix_flags(tun));
+        HIPCHK(hipEventRecord(eps.b, ctx->stream));
+        HIPCHK(hipEventRecord(ej.a, ctx->stream));
+        if (R.max_part < 0)
+            max_part_wait(ctx, R); // the probe passes are queued behind it
         LeafJoin lj(ctx, R, SP, plan);
+        if (narrow) {
+            if (!lj.fused()) {
+                narrow = false;
+                continue;
+            }
+            lj.use_narrow(static_cast<const uint32_t *>(SP.cols.ptr[0]), S.cols);
+        }
         out = alloc_output(ctx, r_cols, S.cols, lj.bound());
-        while ((n = lj.emit(out, 0)) == LeafJoin::kRetry) {
+        while ((n = lj.emit(out, 0)) == LeafJoin::kRetry && !narrow) {
             free_output(ctx, out);
             out = alloc_output(ctx, r_cols, S.cols, lj.bound());
         }
+        if (n == LeafJoin::kRetry) { // narrow join met a repeated build key
+            free_output(ctx, out);
+            narrow = false;
+            continue;
+        }
         big_build_ms = lj.big_build_ms();
+        for (void *b : sk)
+            keep.push_back(b);
+        sk.clear();
+        done = true;
     }
     HIPCHK(hipEventRecord(ej.b, ctx->stream));
     HIPCHK(hipEventRecord(e2e.b, ctx->stream));

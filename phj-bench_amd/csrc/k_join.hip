@@ -750,7 +750,30 @@ __device__ __forceinline__ void build_fused_table(const LeafJoinArgs &a, const I
 // beyond the sink capacity are counted, not written. A leaf with a repeated
 // key raises the flag and writes nothing: the caller then joins with the
 // two-phase path.
-template <int THREADS, int K, int U>
+// Every column of a set gathered by absolute source rows (64-bit offsets):
+// rows row[u] of each source column to rows dst_row0 + pos[u] of the output.
+template <int U>
+__device__ __forceinline__ void gather_set_abs(const ColSet &in, const OutColSet &out, uint64_t dst_row0,
+                                               const uint32_t (&row)[U], const uint32_t (&pos)[U], const bool (&on)[U]) {
+    for (uint32_t c = 0; c < in.n; ++c) {
+        const uint32_t t = col_type(in, c);
+        const void *src = col_ptr(in, c);
+        void *dst = static_cast<char *>(col_ptr(out, c)) + dst_row0 * type_size(t);
+        uint64_t lo[U], hi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            load_elem(src, t, on[u] ? row[u] : 0u, lo[u], hi[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u])
+                store_elem(dst, t, pos[u], lo[u], hi[u]);
+    }
+}
+
+// NARROW: the partitioned probe side carries only keys and each row's index
+// in the unpartitioned probe relation (a.s_rowid); the probe columns of a
+// matched row are gathered from a.s_src at that index.
+template <int THREADS, int K, int U, bool NARROW>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_leaf_join(LeafJoinArgs a) {
     static_assert(K % U == 0 && K % 2 == 0 && K <= 64, "K slots per lane in rounds of U");
     constexpr int W = THREADS / 64;
@@ -918,7 +941,15 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                     if (on[u])
                         st_at<uint64_t>(okeys, opos[u] * 8u, okey[u]);
                 move_set<U>(a.r_cols, it.r0, a.out.left, rb, ridx, opos, on);
-                move_set<U>(a.s_cols, sb, a.out.right, rb, sidx, opos, on);
+                if constexpr (NARROW) {
+                    uint32_t srow[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        srow[u] = a.s_rowid[sb + (on[u] ? sidx[u] : 0u)];
+                    gather_set_abs<U>(a.s_src, a.out.right, rb, srow, opos, on);
+                } else {
+                    move_set<U>(a.s_cols, sb, a.out.right, rb, sidx, opos, on);
+                }
             }
         }
         __syncthreads(); // slot_off and sr_* are rewritten by the next super-round
@@ -1484,10 +1515,17 @@ void leaf_join_fused(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, Ke
     if (max_work == 0)
         return;
     const size_t lds = fused_table_bytes(a.lds_max_rows, a.load_pct);
+    if (a.s_rowid) {
+        TimedScope ts(tm, "leaf_join_narrow", s);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        k_leaf_join<512, kFusedRowsPerThread, 2, true><<<checked_grid(grid8(max_work), 512, "leaf_join"), 512, lds, s>>>(a);
+        return;
+    }
     TimedScope ts(tm, "leaf_join", s);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_join<512, kFusedRowsPerThread, 2, false>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    k_leaf_join<512, kFusedRowsPerThread, 2><<<checked_grid(grid8(max_work), 512, "leaf_join"), 512, lds, s>>>(a);
+    k_leaf_join<512, kFusedRowsPerThread, 2, false><<<checked_grid(grid8(max_work), 512, "leaf_join"), 512, lds, s>>>(a);
 }
 
 void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool chains, KernelTimer *tm) {

@@ -500,6 +500,15 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
             scatter_column<NTL, NTS, uint64_t>(sh, static_cast<const uint64_t *>(col_ptr(cin, c)),
                                      static_cast<uint64_t *>(col_ptr(cout, c)), L, slot);
             break;
+        case kTypeRowIndex: {
+            // row indices of the input, synthesised (nothing to read)
+            uint32_t v[kScatRpt];
+#pragma unroll
+            for (int r = 0; r < kScatRpt; ++r)
+                v[r] = L.row0 + min(r * kScatThreads + threadIdx.x, L.len - 1);
+            write_staged<NTS, uint32_t>(sh, v, static_cast<uint32_t *>(col_ptr(cout, c)), L, slot);
+            break;
+        }
         default:
             scatter_column<NTL, NTS, ulonglong2>(sh, static_cast<const ulonglong2 *>(col_ptr(cin, c)),
                                        static_cast<ulonglong2 *>(col_ptr(cout, c)), L, slot);

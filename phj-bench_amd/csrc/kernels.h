@@ -74,6 +74,9 @@ uint64_t radix_scan_blocks(uint64_t elems);
 constexpr uint32_t kRadixNtLoads = 0x4;  // input columns read with nontemporal loads
 constexpr uint32_t kRadixNtStores = 0x8; // output columns written with nontemporal stores
 constexpr uint32_t kRadixHashEveryPass = 0x10; // every histogram hashes the keys (no digit column between passes)
+// Column type of a radix pass input that is not stored: each row's index in
+// the pass input (row0 + i), written out as a u32 column (type 2).
+constexpr uint32_t kTypeRowIndex = 5;
 // Bit window of a pass. Uniform: every segment uses (shift, bits) and input
 // segment s yields output segments s * 2^bits + d. Per segment (sb != null):
 // sb[s] = shift_s | bits_s << 8 with bits_s <= bits, and output segment d of
@@ -151,6 +154,11 @@ struct LeafJoinArgs {
     const unsigned char *table_img;
     uint64_t img_stride;
     const uint32_t *img_side;
+    // Narrow probe side (fused join only): s_cols is empty and s_rowid holds
+    // each partitioned probe row's index in the unpartitioned relation, whose
+    // columns are s_src; matched rows gather their probe columns from there.
+    const uint32_t *s_rowid;
+    ColSet s_src;
 };
 // Work items per leaf for the LDS path (0 for empty or oversize leaves),
 // scanned, and the leaf of each item.

@@ -275,6 +275,29 @@ def test_fused_and_two_phase_leaf_join(ctx, pass_bits, flags):
     res.release()
 
 
+@pytest.mark.parametrize("case", ["unique", "repeated_keys", "oversize_leaves"])
+@pytest.mark.parametrize("pass_bits", [[4], [2, 2], [8, 8]])
+def test_narrow_probe_side(ctx, case, pass_bits):
+    """PHJ_TUNE_NARROW_PROBE: the probe side partitioned as (key, row index),
+    probe columns gathered from the unpartitioned relation. Unique build keys
+    (with the 0 and 2^64-1 sentinels) take the narrow path; repeated build keys
+    and leaves beyond the LDS limit fall back to the full-width join. Rows equal
+    the oracle's in every case."""
+    rng = np.random.default_rng(37)
+    if case == "repeated_keys":
+        ob, op = data()["zipf_multi"]
+    else:
+        ob, op = unique_with_sentinels(rng, 5000, 120000)
+    tuning = dict(flags=0x20)
+    if case == "oversize_leaves":
+        tuning["lds_leaf_rows"] = 8
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    res = ctx.runPHJ(host_rel(ob), host_rel(op), G.RadixConfig(pass_bits), tuning=tuning)
+    assert res.out_rows == want.shape[0]
+    assert np.array_equal(sorted_rows_of(res.to_host()), want)
+    res.release()
+
+
 @pytest.mark.parametrize("half_empty", [False, True])
 def test_bep_sparse_rounds_read_tables_in_hbm(ctx, half_empty):
     """PHJ best-effort under a 1 MiB budget against 256 leaves of ~1,200 build
