@@ -100,7 +100,7 @@ def cpu_baseline(scale_log2, threads):
             "config1": config1}
 
 
-def batched_bench(ctx, n_joins, seed=42):
+def batched_bench(ctx, n_joins, seed=42, plain=True):
     """4096 independent uniform joins (|R_i| = floor(2^U[10,20]) distinct keys,
     |S_i| = 8 |R_i|, 75% match, u64 payload each side), back to back in HBM:
     one phj_join_batch call (median of 3) against one runPHJ call per join.
@@ -129,28 +129,48 @@ def batched_bench(ctx, n_joins, seed=42):
     # One untimed run of the whole workload per path first: the context's
     # device pool then holds the memory both paths need (fresh device memory
     # is cleared by the driver when first mapped, which is not join work).
-    for run in (run_batched, run_plain):
+    for run in ((run_batched, run_plain) if plain else (run_batched,)):
         for r in run():
             r.release()
-    t_batches = []
-    for _ in range(3):
-        t, res = timed(run_batched)
-        t_batches.append(t)
-        got = [r.checksum() for r in res]
-        out_rows = sum(r.out_rows for r in res)
-        for r in res:
+
+    def median_batched():
+        ts = []
+        for _ in range(3):
+            t, res = timed(run_batched)
+            ts.append(t)
+            got = [r.checksum() for r in res]
+            rows = sum(r.out_rows for r in res)
+            for r in res:
+                r.release()
+        return sorted(ts)[1], got, rows
+
+    t_default, _, _ = median_batched()
+    # the autotuner (untimed) picks a leaf size per size class; the batch is
+    # then timed with the tuned table
+    t0 = time.perf_counter()
+    table = ctx.batch_autotune(pairs, cfgs)
+    t_tune = time.perf_counter() - t0
+    log(f"batch: autotune {t_tune:.1f} s: " + ", ".join(f"2^{c}:{v[0]}" for c, v in sorted(table.items())))
+    t_batch, got, out_rows = median_batched()
+    if plain:
+        t_plain, res_plain = timed(run_plain)
+        want = [r.checksum() for r in res_plain]
+        for r in res_plain:
             r.release()
-    t_batch = sorted(t_batches)[1]
-    t_plain, plain = timed(run_plain)
-    want = [r.checksum() for r in plain]
-    for r in plain:
-        r.release()
+    else:
+        t_plain, want = float("nan"), [ctx.expected_checksum(sp) for sp in specs]
     build.release()
     probe.release()
     parity = got == want
-    log(f"batch: batched {t_batch * 1e3:.1f} ms, plain {t_plain * 1e3:.1f} ms, per-join parity {parity}")
+    log(f"batch: batched {t_batch * 1e3:.1f} ms (untuned {t_default * 1e3:.1f} ms), plain {t_plain * 1e3:.1f} ms, "
+        f"per-join parity {parity}")
+    # algorithmic bytes: every input row read once, every output row written once
+    b_alg = R * 16 + S * 16 + out_rows * 24
     return {"joins": n_joins, "build_rows": R, "probe_rows": S, "out_rows": out_rows,
-            "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3, "speedup": t_plain / t_batch,
+            "batched_ms": t_batch * 1e3, "batched_untuned_ms": t_default * 1e3, "plain_ms": t_plain * 1e3,
+            "speedup": t_plain / t_batch, "roofline_frac": b_alg / t_batch / HBM_PEAK,
+            "autotune": {f"2^{c}": {"leaf_rows": v[0], "best_ms": round(v[1], 3)} for c, v in sorted(table.items())},
+            "autotune_s": t_tune,
             "batched_tuples_per_s": (R + S) / t_batch, "plain_tuples_per_s": (R + S) / t_plain,
             "parity_per_join": parity,
             "workload": f"{n_joins} joins, |R_i| = floor(2^U[10,20]), |S_i| = 8|R_i|, 75% match, u64 payloads"}
@@ -359,6 +379,8 @@ def main():
                     help="also run one plain device copy of this size (HBM counter calibration under a profiler)")
     ap.add_argument("--tune", default="", help="phj_tuning overrides, e.g. flags=12,join_threads=1024")
     ap.add_argument("--batch-joins", type=int, default=4096, help="joins of the batched-join measurement (0: skip)")
+    ap.add_argument("--batch-no-plain", action="store_true",
+                    help="skip the per-join runPHJ comparison of the batch (profiling the batch alone)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests per offered rate of the serving "
                                                                      "measurement (0: skip)")
     ap.add_argument("--serve-rates", default="250,500,1000,2000,5000,10000,20000")
@@ -384,7 +406,7 @@ def main():
     batched = None
     if args.batch_joins > 0:
         bctx = G.Context(0)
-        batched = batched_bench(bctx, args.batch_joins)
+        batched = batched_bench(bctx, args.batch_joins, plain=not args.batch_no_plain)
         bctx.close()
     skew = skew_bench(scale_log2=args.scale_log2) if not args.no_skew else None
     serving = None

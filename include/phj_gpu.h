@@ -286,6 +286,19 @@ phj_status phj_build_bytes(const phj_resident *build, uint64_t *bytes);
  * one output allocation, freed with the last phj_result_release. Every desc
  * must be PHJ_SCHEME_PHJ_PURE with a valid radix configuration. */
 phj_status phj_join_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *descs, phj_join_result *results);
+/* Batch tuning: no reference counterpart. Joins of a batch aim for leaves of
+ * a number of build rows set per size class, floor(log2(build rows)), in
+ * PHJ_BATCH_CLASSES entries (2,048 rows for every class until tuned). The
+ * autotuner times, for each size class present in descs, a batch of that
+ * class's joins with each candidate leaf size (512, 1,024, 2,048, 4,096;
+ * best of two timed runs after a warm-up), keeps the fastest in the context
+ * for its later phj_join_batch calls, and returns the table and each class's
+ * best time in ms (either output may be null). Outputs of the timing runs
+ * are released. */
+#define PHJ_BATCH_CLASSES 33u
+phj_status phj_batch_autotune(phj_ctx *ctx, uint32_t n, const phj_join_desc *descs, uint32_t *leaf_rows_out,
+                              double *class_ms_out);
+phj_status phj_batch_set_leaf_rows(phj_ctx *ctx, const uint32_t *leaf_rows);
 phj_status phj_result_release(phj_ctx *ctx, phj_join_result *result);
 /* Copy output columns to host arrays sized out_rows * width. */
 phj_status phj_result_copy_to_host(phj_ctx *ctx, const phj_join_result *result, uint64_t *keys, void *const *left,

@@ -194,6 +194,57 @@ hipError_t scratch_address(int device, void **out) {
 }
 } // namespace
 
+constexpr uint32_t kBatchClasses = 33;           // size classes floor(log2(build rows)), 0..32
+constexpr uint32_t kBatchDefaultLeafRows = 2048; // leaf target of every class until tuned
+
+namespace {
+// Pinned host memory for the small arrays a call uploads (offsets, bit
+// windows, segment lists) and the values it reads back, so that no copy goes
+// through pageable memory (which can hold the host until the stream has
+// caught up). A bump allocator over blocks kept for the context's life;
+// reset() only once the stream has finished with everything handed out.
+class PinnedArena {
+public:
+    ~PinnedArena() {
+        for (auto &b : blocks_)
+            (void)hipHostFree(b.p);
+    }
+    template <class T> T *get(uint64_t n) { return static_cast<T *>(bytes(std::max<uint64_t>(n, 1) * sizeof(T))); }
+    void *bytes(uint64_t sz) {
+        sz = (sz + 63) & ~uint64_t{63};
+        while (cur_ < blocks_.size() && off_ + sz > blocks_[cur_].cap) {
+            ++cur_;
+            off_ = 0;
+        }
+        if (cur_ == blocks_.size()) {
+            Block b{nullptr, std::max<uint64_t>(sz, 8ull << 20)};
+            if (hipHostMalloc(reinterpret_cast<void **>(&b.p), b.cap, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                throw std::bad_alloc();
+            }
+            blocks_.push_back(b);
+            off_ = 0;
+        }
+        void *p = blocks_[cur_].p + off_;
+        off_ += sz;
+        return p;
+    }
+    void reset() {
+        cur_ = 0;
+        off_ = 0;
+    }
+
+private:
+    struct Block {
+        char *p;
+        uint64_t cap;
+    };
+    std::vector<Block> blocks_;
+    size_t cur_ = 0;
+    uint64_t off_ = 0;
+};
+} // namespace
+
 struct phj_ctx {
     int device = 0;
     void *scratch = nullptr; // this device's `scratch` (device address)
@@ -210,6 +261,11 @@ struct phj_ctx {
     // value copied there is waited for through `note_event` only.
     uint32_t *pinned = nullptr;
     hipEvent_t note_event = nullptr;
+    PinnedArena pinned_io; // uploads and readbacks of batched joins
+    // Leaf size (build rows) batched joins aim for, by size class
+    // floor(log2(build rows)); set by phj_batch_autotune.
+    uint32_t batch_leaf_rows[kBatchClasses];
+    phj_ctx() { std::fill(std::begin(batch_leaf_rows), std::end(batch_leaf_rows), kBatchDefaultLeafRows); }
 
     KernelTimer *tm() { return timing ? &timer : nullptr; }
     void sync() {
@@ -1694,22 +1750,40 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
 }
 
 // ---- Batched PHJ-PURE joins ----
-// A batch is cut into groups of consecutive joins whose relations lie back to
-// back in memory (so a group is one segmented relation per side, with no
-// copy) and whose input bytes fit kBatchGroupBytes. Each join picks its own
-// leaf bits for leaves of about kBatchLeafRows build rows (its output does not
-// depend on them); a group runs as many device passes as its largest join
-// needs, every pass splitting each join's segments by that join's bits, then
-// one fused leaf join over all leaves of all joins writes each join's rows to
-// its own region of a shared output (bounded by its probe rows, as no build
-// key repeats). A join whose leaves repeat a key or outgrow LDS is rerun alone
-// on the plain path.
-constexpr uint64_t kBatchLeafRows = 2048;
+// A batch is cut into groups of consecutive joins on device relations with one
+// schema, up to kBatchGroupBytes of input. A group whose relations lie back
+// to back is one segmented relation per side as it is; otherwise (joins of
+// at most kBatchStageBytes each) its relations are first gathered back to
+// back. Each join picks its own leaf bits for leaves of about its size
+// class's leaf rows (its output does not depend on them; the table below is
+// set by phj_batch_autotune); a group runs as many device passes as its
+// largest join needs, every pass splitting each join's segments by that
+// join's bits, then one fused leaf join over all leaves of all joins writes
+// each join's rows to its own region of a shared output (bounded by its probe
+// rows, as no build key repeats). A join whose leaves repeat a key or outgrow
+// LDS is rerun alone on the plain path after the batch.
+//
+// Groups are pipelined on the context's one stream: group g+1's partition
+// passes are queued before the host waits for group g's leaf sizes (the one
+// value the leaf-join launch needs, read back into pinned memory), so the
+// device is not left idle while the host plans. Partition buffers go back to
+// the pool as soon as their join is queued (stream order keeps them alive
+// for it); every upload and readback goes through pinned memory.
 constexpr uint64_t kBatchGroupBytes = 8ull << 30;
+constexpr uint64_t kBatchStageBytes = 64ull << 20;
 
-uint32_t batch_bits(uint64_t build_rows) {
+// Size class of a join: floor(log2(build rows)) clamped to the table.
+uint32_t size_class(uint64_t build_rows) {
+    uint32_t c = 0;
+    while (c + 1 < kBatchClasses && (build_rows >> (c + 1)) != 0)
+        ++c;
+    return c;
+}
+
+uint32_t batch_bits(uint64_t build_rows, const uint32_t *leaf_rows) {
+    const uint64_t target = leaf_rows[size_class(build_rows)];
     uint32_t b = 0;
-    while (b < 16 && (build_rows >> b) > kBatchLeafRows)
+    while (b < 16 && (build_rows >> b) > target)
         ++b;
     return b;
 }
@@ -1745,7 +1819,7 @@ bool follows(const phj_relation &a, const phj_relation &b) {
 // Radix partition of a segmented relation (segment j = join j) where join j
 // splits by bits[p][j] in pass p (0 keeps the segment whole). Returns the
 // leaves; leaf_join[l] is the join owning leaf l (a join's leaves are
-// consecutive).
+// consecutive). Bit windows are uploaded from pinned memory.
 Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t *seg_off, uint32_t n_joins,
                                   const std::vector<std::vector<uint32_t>> &bits, std::vector<void *> &keep,
                                   uint32_t flags, std::vector<uint32_t> &leaf_join) {
@@ -1763,8 +1837,8 @@ Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t
     Partitioned out;
     out.rows = rows;
     for (size_t p = 0; p < bits.size(); ++p) {
-        std::vector<uint16_t> sb(cur_nseg);
-        std::vector<uint32_t> first(static_cast<size_t>(cur_nseg) + 1);
+        uint16_t *sb = ctx->pinned_io.get<uint16_t>(cur_nseg);
+        uint32_t *first = ctx->pinned_io.get<uint32_t>(static_cast<uint64_t>(cur_nseg) + 1);
         std::vector<uint32_t> next_join;
         uint32_t maxb = 0;
         uint64_t run = 0;
@@ -1782,10 +1856,10 @@ Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t
         first[cur_nseg] = static_cast<uint32_t>(run);
         for (uint32_t j = 0; j < n_joins; ++j)
             consumed[j] += bits[p][j];
-        uint16_t *dsb = scr.get<uint16_t>(sb.size());
-        uint32_t *dfirst = scr.get<uint32_t>(first.size());
-        HIPCHK(hipMemcpyAsync(dsb, sb.data(), sb.size() * 2, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(dfirst, first.data(), first.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        uint16_t *dsb = scr.get<uint16_t>(cur_nseg);
+        uint32_t *dfirst = scr.get<uint32_t>(static_cast<uint64_t>(cur_nseg) + 1);
+        HIPCHK(hipMemcpyAsync(dsb, sb, cur_nseg * 2ull, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(dfirst, first, (cur_nseg + 1ull) * 4, hipMemcpyHostToDevice, ctx->stream));
         const uint64_t B = 1ull << maxb;
         Held bufs(ctx);
         uint64_t *nkeys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
@@ -1818,38 +1892,108 @@ Partitioned radix_partition_joins(phj_ctx *ctx, const DevRel &in, const uint32_t
     return out;
 }
 
-// One group of a batch: m joins whose relations are back to back.
-void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_result *res) {
-    Scratch scr(ctx);
-    ctx->arena.reset_peak();
-    const phj_tuning tun = d[0].tuning;
+// Largest leaf (build rows) of each join of a group.
+__global__ void k_join_max_leaf(const uint32_t *__restrict__ off, uint32_t n_leaves, const uint32_t *__restrict__ leaf_join,
+                                uint32_t *__restrict__ join_max) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n_leaves)
+        atomicMax(&join_max[leaf_join[l]], off[l + 1] - off[l]);
+}
+
+// One group of a batch in flight.
+struct BatchGroup {
+    const phj_join_desc *d = nullptr;
+    phj_join_result *res = nullptr;
+    uint32_t m = 0;
+    Held keep;                 // staged inputs and partitions, until the leaf join is queued
     DevRel R, S;
-    R.keys = d[0].build.keys;
-    S.keys = d[0].probe.keys;
-    R.cols.n = d[0].build.n_cols;
-    S.cols.n = d[0].probe.n_cols;
-    for (uint32_t c = 0; c < R.cols.n; ++c) {
-        R.cols.types[c] = d[0].build.cols[c].type;
-        R.cols.ptr[c] = d[0].build.cols[c].data;
+    std::vector<uint64_t> base, end; // output rows of join j: [base[j], base[j] + written)
+    uint32_t n_leaves = 0;
+    LeafJoinArgs a{};
+    uint32_t *join_max_h = nullptr; // pinned readbacks
+    unsigned long long *cur_h = nullptr;
+    uint32_t *flag_h = nullptr;
+    uint32_t lds_max = 0;
+    OutBuffers out;
+    std::vector<uint8_t> rerun;
+    EventPair e2e, ebs, eps, ej;
+    explicit BatchGroup(phj_ctx *c) : keep(c) {}
+};
+
+// Relation j of a group (build or probe side) as a gather source.
+GatherSource source_of(const phj_relation &r) {
+    GatherSource g{};
+    g.keys = r.keys;
+    for (uint32_t c = 0; c < r.n_cols; ++c)
+        g.cols[c] = r.cols[c].data;
+    return g;
+}
+
+// The group's side `which` (0 build, 1 probe) as one relation: in place when
+// the joins' relations are back to back, else gathered back to back.
+DevRel group_side(phj_ctx *ctx, BatchGroup &g, int which, bool in_place, uint64_t rows) {
+    const phj_relation &r0 = which ? g.d[0].probe : g.d[0].build;
+    DevRel v;
+    v.rows = rows;
+    v.cols.n = r0.n_cols;
+    for (uint32_t c = 0; c < r0.n_cols; ++c)
+        v.cols.types[c] = r0.cols[c].type;
+    if (in_place) {
+        v.keys = r0.keys;
+        for (uint32_t c = 0; c < r0.n_cols; ++c)
+            v.cols.ptr[c] = r0.cols[c].data;
+        return v;
     }
-    for (uint32_t c = 0; c < S.cols.n; ++c) {
-        S.cols.types[c] = d[0].probe.cols[c].type;
-        S.cols.ptr[c] = d[0].probe.cols[c].data;
+    uint64_t *seg_start = ctx->pinned_io.get<uint64_t>(g.m + 1);
+    uint64_t *seg_row = ctx->pinned_io.get<uint64_t>(g.m);
+    uint32_t *seg_src = ctx->pinned_io.get<uint32_t>(g.m);
+    GatherSource *srcs = ctx->pinned_io.get<GatherSource>(g.m);
+    seg_start[0] = 0;
+    for (uint32_t j = 0; j < g.m; ++j) {
+        const phj_relation &r = which ? g.d[j].probe : g.d[j].build;
+        seg_start[j + 1] = seg_start[j] + r.rows;
+        seg_row[j] = 0;
+        seg_src[j] = j;
+        srcs[j] = source_of(r);
     }
-    std::vector<uint32_t> r_off(m + 1, 0), s_off(m + 1, 0);
-    std::vector<uint64_t> base(m), end(m);
+    Scratch scr(ctx);
+    uint64_t *dstart = scr.get<uint64_t>(g.m + 1);
+    uint64_t *drow = scr.get<uint64_t>(g.m);
+    uint32_t *dsrc = scr.get<uint32_t>(g.m);
+    GatherSource *dsrcs = scr.get<GatherSource>(g.m);
+    HIPCHK(hipMemcpyAsync(dstart, seg_start, (g.m + 1ull) * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(drow, seg_row, g.m * 8ull, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dsrc, seg_src, g.m * 4ull, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dsrcs, srcs, g.m * sizeof(GatherSource), hipMemcpyHostToDevice, ctx->stream));
+    uint64_t *keys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
+    g.keep.push_back(keys);
+    OutColSet cols = alloc_cols(ctx, v.cols, rows, g.keep);
+    gather_segments(ctx->stream, dstart, g.m, dsrc, drow, dsrcs, rows, keys, cols, ctx->tm());
+    v.keys = keys;
+    v.cols = as_in(cols);
+    return v;
+}
+
+// Queues the group's partition passes and the readback of each join's
+// largest leaf (pinned); returns without waiting.
+void group_start(phj_ctx *ctx, BatchGroup &g, bool in_place, const uint32_t *leaf_rows) {
+    const uint32_t m = g.m;
+    HIPCHK(hipEventRecord(g.e2e.a, ctx->stream));
+    uint32_t *r_off = ctx->pinned_io.get<uint32_t>(m + 1);
+    uint32_t *s_off = ctx->pinned_io.get<uint32_t>(m + 1);
+    r_off[0] = s_off[0] = 0;
+    g.base.resize(m);
+    g.end.resize(m);
     std::vector<uint32_t> jbits(m);
     uint32_t n_passes = 0;
     for (uint32_t j = 0; j < m; ++j) {
-        r_off[j + 1] = r_off[j] + static_cast<uint32_t>(d[j].build.rows);
-        s_off[j + 1] = s_off[j] + static_cast<uint32_t>(d[j].probe.rows);
-        base[j] = s_off[j];
-        end[j] = s_off[j + 1];
-        jbits[j] = batch_bits(d[j].build.rows);
+        r_off[j + 1] = r_off[j] + static_cast<uint32_t>(g.d[j].build.rows);
+        s_off[j + 1] = s_off[j] + static_cast<uint32_t>(g.d[j].probe.rows);
+        g.base[j] = s_off[j];
+        g.end[j] = s_off[j + 1];
+        jbits[j] = batch_bits(g.d[j].build.rows, leaf_rows);
         n_passes = std::max(n_passes, (jbits[j] + kDeviceMaxPassBits - 1) / kDeviceMaxPassBits);
     }
-    R.rows = r_off[m];
-    S.rows = s_off[m];
     std::vector<std::vector<uint32_t>> bits(n_passes, std::vector<uint32_t>(m, 0));
     for (uint32_t j = 0; j < m; ++j) {
         uint32_t left = jbits[j];
@@ -1858,26 +2002,27 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
             left -= bits[p][j];
         }
     }
-    uint32_t *dr_off = scr.get<uint32_t>(m + 1);
-    uint32_t *ds_off = scr.get<uint32_t>(m + 1);
-    HIPCHK(hipMemcpyAsync(dr_off, r_off.data(), (m + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ds_off, s_off.data(), (m + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
-
-    EventPair e2e, ebs, eps, ej;
-    HIPCHK(hipEventRecord(e2e.a, ctx->stream));
-    HIPCHK(hipEventRecord(ebs.a, ctx->stream));
-    Held keep(ctx);
+    uint32_t *dr_off = ctx->alloc<uint32_t>(m + 1);
+    g.keep.push_back(dr_off);
+    uint32_t *ds_off = ctx->alloc<uint32_t>(m + 1);
+    g.keep.push_back(ds_off);
+    HIPCHK(hipMemcpyAsync(dr_off, r_off, (m + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ds_off, s_off, (m + 1) * 4ull, hipMemcpyHostToDevice, ctx->stream));
+    g.R = group_side(ctx, g, 0, in_place, r_off[m]);
+    g.S = group_side(ctx, g, 1, in_place, s_off[m]);
+    const phj_tuning tun = g.d[0].tuning;
+    HIPCHK(hipEventRecord(g.ebs.a, ctx->stream));
     std::vector<uint32_t> leaf_join;
-    LeafJoinArgs a{};
-    a.r_cols = R.cols;
-    a.s_cols = S.cols;
+    LeafJoinArgs &a = g.a;
+    a.r_cols = g.R.cols;
+    a.s_cols = g.S.cols;
     if (n_passes > 0) {
-        const Partitioned RP = radix_partition_joins(ctx, R, dr_off, m, bits, keep, radix_flags(tun), leaf_join);
-        HIPCHK(hipEventRecord(ebs.b, ctx->stream));
-        HIPCHK(hipEventRecord(eps.a, ctx->stream));
+        const Partitioned RP = radix_partition_joins(ctx, g.R, dr_off, m, bits, g.keep, radix_flags(tun), leaf_join);
+        HIPCHK(hipEventRecord(g.ebs.b, ctx->stream));
+        HIPCHK(hipEventRecord(g.eps.a, ctx->stream));
         std::vector<uint32_t> lj2;
-        const Partitioned SP = radix_partition_joins(ctx, S, ds_off, m, bits, keep, radix_flags(tun), lj2);
-        HIPCHK(hipEventRecord(eps.b, ctx->stream));
+        const Partitioned SP = radix_partition_joins(ctx, g.S, ds_off, m, bits, g.keep, radix_flags(tun), lj2);
+        HIPCHK(hipEventRecord(g.eps.b, ctx->stream));
         a.r_keys = RP.keys;
         a.r_cols = as_in(RP.cols);
         a.r_off = RP.off;
@@ -1885,96 +2030,116 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
         a.s_cols = as_in(SP.cols);
         a.s_off = SP.off;
     } else {
-        HIPCHK(hipEventRecord(ebs.b, ctx->stream));
-        HIPCHK(hipEventRecord(eps.a, ctx->stream));
-        HIPCHK(hipEventRecord(eps.b, ctx->stream));
+        HIPCHK(hipEventRecord(g.ebs.b, ctx->stream));
+        HIPCHK(hipEventRecord(g.eps.a, ctx->stream));
+        HIPCHK(hipEventRecord(g.eps.b, ctx->stream));
         leaf_join.resize(m);
         for (uint32_t j = 0; j < m; ++j)
             leaf_join[j] = j;
-        a.r_keys = R.keys;
+        a.r_keys = g.R.keys;
         a.r_off = dr_off;
-        a.s_keys = S.keys;
+        a.s_keys = g.S.keys;
         a.s_off = ds_off;
     }
-    const uint32_t n_leaves = static_cast<uint32_t>(leaf_join.size());
-    HIPCHK(hipEventRecord(ej.a, ctx->stream));
-    OutBuffers out = alloc_output(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1));
-    unsigned long long *cur = scr.get<unsigned long long>(m);
-    uint64_t *dend = scr.get<uint64_t>(m);
-    uint32_t *dflag = scr.get<uint32_t>(m);
-    uint32_t *dleaf = scr.get<uint32_t>(n_leaves);
-    HIPCHK(hipMemcpyAsync(cur, base.data(), m * 8ull, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dend, end.data(), m * 8ull, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dleaf, leaf_join.data(), n_leaves * 4ull, hipMemcpyHostToDevice, ctx->stream));
-    launch_fill_u32(ctx->stream, dflag, m, 0);
-    // Leaf sizes decide the LDS table size; joins with a larger leaf rerun alone.
-    std::vector<uint32_t> h_roff(n_leaves + 1);
-    HIPCHK(hipMemcpyAsync(h_roff.data(), a.r_off, (n_leaves + 1) * 4ull, hipMemcpyDeviceToHost, ctx->stream));
-    ctx->sync();
-    std::vector<uint32_t> join_max(m, 0);
+    g.n_leaves = static_cast<uint32_t>(leaf_join.size());
+    uint32_t *lj_h = ctx->pinned_io.get<uint32_t>(g.n_leaves);
+    std::copy(leaf_join.begin(), leaf_join.end(), lj_h);
+    uint32_t *dleaf = ctx->alloc<uint32_t>(g.n_leaves);
+    g.keep.push_back(dleaf);
+    HIPCHK(hipMemcpyAsync(dleaf, lj_h, g.n_leaves * 4ull, hipMemcpyHostToDevice, ctx->stream));
+    a.leaf_join = dleaf;
+    uint32_t *dmax = ctx->alloc<uint32_t>(m);
+    g.keep.push_back(dmax);
+    launch_fill_u32(ctx->stream, dmax, m, 0);
+    if (g.n_leaves)
+        k_join_max_leaf<<<(g.n_leaves + 255) / 256, 256, 0, ctx->stream>>>(a.r_off, g.n_leaves, dleaf, dmax);
+    g.join_max_h = ctx->pinned_io.get<uint32_t>(m);
+    HIPCHK(hipMemcpyAsync(g.join_max_h, dmax, m * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipEventRecord(g.ej.a, ctx->stream)); // the host waits for this point only
+}
+
+// Waits for the group's leaf sizes, then queues its fused leaf join and the
+// readback of every join's output cursor and repeated-key flag.
+void group_join(phj_ctx *ctx, BatchGroup &g) {
+    HIPCHK(hipEventSynchronize(g.ej.a));
+    const uint32_t m = g.m;
     uint32_t max_leaf = 0;
-    for (uint32_t l = 0; l < n_leaves; ++l) {
-        const uint32_t sz = h_roff[l + 1] - h_roff[l];
-        join_max[leaf_join[l]] = std::max(join_max[leaf_join[l]], sz);
-        max_leaf = std::max(max_leaf, sz);
-    }
-    const LeafJoinPlan plan = plan_of(tun);
+    for (uint32_t j = 0; j < m; ++j)
+        max_leaf = std::max(max_leaf, g.join_max_h[j]);
+    const LeafJoinPlan plan = plan_of(g.d[0].tuning);
     uint32_t lds_cap = leaf_join_lds_max_rows(78u * 1024u, plan.load_pct);
     if (max_leaf > lds_cap)
         lds_cap = leaf_join_lds_max_rows(150u * 1024u, plan.load_pct);
-    const uint32_t lds_max = std::max<uint32_t>(std::min(max_leaf, lds_cap), 1);
-    a.n_leaves = n_leaves;
-    a.lds_max_rows = lds_max;
+    g.lds_max = std::max<uint32_t>(std::min(max_leaf, lds_cap), 1);
+    g.rerun.assign(m, 0);
+    for (uint32_t j = 0; j < m; ++j)
+        g.rerun[j] = g.join_max_h[j] > g.lds_max; // leaves beyond LDS: the plain path
+    LeafJoinArgs &a = g.a;
+    g.out = alloc_output(ctx, g.R.cols, g.S.cols, std::max<uint64_t>(g.S.rows, 1));
+    Scratch scr(ctx);
+    unsigned long long *cur = scr.get<unsigned long long>(m);
+    uint64_t *dend = scr.get<uint64_t>(m);
+    uint32_t *dflag = scr.get<uint32_t>(m);
+    uint64_t *base_h = ctx->pinned_io.get<uint64_t>(m);
+    uint64_t *end_h = ctx->pinned_io.get<uint64_t>(m);
+    std::copy(g.base.begin(), g.base.end(), base_h);
+    std::copy(g.end.begin(), g.end.end(), end_h);
+    HIPCHK(hipMemcpyAsync(cur, base_h, m * 8ull, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dend, end_h, m * 8ull, hipMemcpyHostToDevice, ctx->stream));
+    launch_fill_u32(ctx->stream, dflag, m, 0);
+    a.n_leaves = g.n_leaves;
+    a.lds_max_rows = g.lds_max;
     a.load_pct = plan.load_pct;
     a.chunk_rows = leaf_join_fused_chunk();
-    a.leaf_join = dleaf;
     a.join_cursor = cur;
     a.join_end = dend;
     a.join_flag = dflag;
-    a.out = sink_of(out, nullptr);
-    {
-        uint32_t *work_first = scr.get<uint32_t>(n_leaves + 1);
-        uint32_t *scan_tmp = scr.get<uint32_t>(radix_scan_blocks(n_leaves + 1) + 1);
-        const uint64_t max_work = leaf_max_work(S.rows, n_leaves, a.chunk_rows);
-        uint32_t *item_leaf = scr.get<uint32_t>(max_work);
-        a.work_first = work_first;
-        a.item_leaf = item_leaf;
-        leaf_work_plan(ctx->stream, a, work_first, scan_tmp, item_leaf, ctx->tm());
-        leaf_join_fused(ctx->stream, a, max_work, ctx->tm());
-    }
-    HIPCHK(hipEventRecord(ej.b, ctx->stream));
-    HIPCHK(hipEventRecord(e2e.b, ctx->stream));
-    std::vector<uint64_t> h_cur(m);
-    std::vector<uint32_t> h_flag(m);
-    HIPCHK(hipMemcpyAsync(h_cur.data(), cur, m * 8ull, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(h_flag.data(), dflag, m * 4ull, hipMemcpyDeviceToHost, ctx->stream));
-    ctx->sync();
-    keep.free_all();
-    const double t_bs = ebs.ms(), t_ps = eps.ms(), t_j = ej.ms(), t_e2e = e2e.ms();
-    const uint64_t peak = ctx->arena.peak();
-    auto bufs = std::make_shared<std::vector<void *>>(out.bufs);
-    out.bufs.clear();
-    for (uint32_t j = 0; j < m; ++j) {
-        phj_join_result *r = &res[j];
-        if (h_flag[j] != 0 || join_max[j] > lds_max) {
-            run_phj_pure(ctx, d[j].build, d[j].probe, d[j].pass_bits, d[j].n_passes, d[j].tuning, r);
-            continue;
-        }
+    a.out = sink_of(g.out, nullptr);
+    uint32_t *work_first = scr.get<uint32_t>(g.n_leaves + 1);
+    uint32_t *scan_tmp = scr.get<uint32_t>(radix_scan_blocks(g.n_leaves + 1) + 1);
+    const uint64_t max_work = leaf_max_work(g.S.rows, g.n_leaves, a.chunk_rows);
+    uint32_t *item_leaf = scr.get<uint32_t>(max_work);
+    a.work_first = work_first;
+    a.item_leaf = item_leaf;
+    leaf_work_plan(ctx->stream, a, work_first, scan_tmp, item_leaf, ctx->tm());
+    leaf_join_fused(ctx->stream, a, max_work, ctx->tm());
+    HIPCHK(hipEventRecord(g.ej.b, ctx->stream));
+    HIPCHK(hipEventRecord(g.e2e.b, ctx->stream));
+    g.cur_h = ctx->pinned_io.get<unsigned long long>(m);
+    g.flag_h = ctx->pinned_io.get<uint32_t>(m);
+    HIPCHK(hipMemcpyAsync(g.cur_h, cur, m * 8ull, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(g.flag_h, dflag, m * 4ull, hipMemcpyDeviceToHost, ctx->stream));
+    // stream order keeps the partitions alive for the queued join
+    g.keep.free_all();
+}
+
+// After the stream has finished the group: its results, one view of the
+// shared output per join; joins to rerun are marked.
+void group_collect(phj_ctx *ctx, BatchGroup &g, uint64_t peak) {
+    const double t_bs = g.ebs.ms(), t_ps = g.eps.ms(), t_j = g.ej.ms(), t_e2e = g.e2e.ms();
+    auto bufs = std::make_shared<std::vector<void *>>(g.out.bufs);
+    g.out.bufs.clear();
+    for (uint32_t j = 0; j < g.m; ++j) {
+        phj_join_result *r = &g.res[j];
         std::memset(r, 0, sizeof(*r));
+        if (g.flag_h[j] != 0)
+            g.rerun[j] = 1;
+        if (g.rerun[j])
+            continue;
         OutBuffers view;
-        view.keys = out.keys + base[j];
-        view.left = out.left;
-        view.right = out.right;
-        for (uint32_t c = 0; c < out.left.n; ++c)
-            view.left.ptr[c] = static_cast<uint8_t *>(out.left.ptr[c]) + base[j] * type_size(out.left.types[c]);
-        for (uint32_t c = 0; c < out.right.n; ++c)
-            view.right.ptr[c] = static_cast<uint8_t *>(out.right.ptr[c]) + base[j] * type_size(out.right.types[c]);
-        fill_result_cols(r, view, h_cur[j] - base[j]);
+        view.keys = g.out.keys + g.base[j];
+        view.left = g.out.left;
+        view.right = g.out.right;
+        for (uint32_t c = 0; c < g.out.left.n; ++c)
+            view.left.ptr[c] = static_cast<uint8_t *>(g.out.left.ptr[c]) + g.base[j] * type_size(g.out.left.types[c]);
+        for (uint32_t c = 0; c < g.out.right.n; ++c)
+            view.right.ptr[c] = static_cast<uint8_t *>(g.out.right.ptr[c]) + g.base[j] * type_size(g.out.right.types[c]);
+        fill_result_cols(r, view, g.cur_h[j] - g.base[j]);
         // Members of a group report the group's phase times.
-        set_phase(r->build_shuffle, t_bs, R.rows);
-        set_phase(r->probe_shuffle, t_ps, S.rows);
-        set_phase(r->build, 0.0, R.rows);
-        set_phase(r->probe, t_j, S.rows);
+        set_phase(r->build_shuffle, t_bs, g.R.rows);
+        set_phase(r->probe_shuffle, t_ps, g.S.rows);
+        set_phase(r->build, 0.0, g.R.rows);
+        set_phase(r->probe, t_j, g.S.rows);
         r->e2e_wall_ms = t_e2e;
         r->peak_mem_bytes = peak;
         r->handle = ctx->next_handle++;
@@ -1985,10 +2150,9 @@ void run_batch_group(phj_ctx *ctx, const phj_join_desc *d, uint32_t m, phj_join_
             ctx->arena.free(b);
 }
 
-void run_phj_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *d, phj_join_result *res) {
+void run_phj_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *d, phj_join_result *res, const uint32_t *leaf_rows) {
     if (n > 0 && (d == nullptr || res == nullptr))
         invalid("null descriptors/results");
-    bool batchable = true;
     for (uint32_t i = 0; i < n; ++i) {
         if (d[i].scheme != PHJ_SCHEME_PHJ_PURE)
             invalid("batched joins run PHJ-PURE");
@@ -1996,30 +2160,81 @@ void run_phj_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *d, phj_join_re
         validate_rel(&d[i].probe);
         validate_pass_bits(d[i].pass_bits, d[i].n_passes);
         std::memset(&res[i], 0, sizeof(res[i]));
-        // a group reads its relations in place: both sides must be in device memory
-        // (host relations run on the plain path, which stages them)
-        batchable = batchable && d[i].build.on_device && d[i].probe.on_device && same_schema(d[0].build, d[i].build) &&
-                    same_schema(d[0].probe, d[i].probe) && !(d[i].tuning.flags & PHJ_TUNE_TWO_PHASE_JOIN);
     }
-    uint32_t i = 0;
-    while (i < n) {
-        if (!batchable) {
-            run_phj_pure(ctx, d[i].build, d[i].probe, d[i].pass_bits, d[i].n_passes, d[i].tuning, &res[i]);
-            ++i;
+    // A join can be grouped when both sides are device relations (a group
+    // reads them in place or gathers them; host relations take the plain
+    // path, which stages them) and it asks for no other join path.
+    auto groupable = [&](uint32_t i) {
+        return d[i].build.on_device && d[i].probe.on_device && !(d[i].tuning.flags & PHJ_TUNE_TWO_PHASE_JOIN) &&
+               !(d[i].tuning.flags & PHJ_TUNE_NARROW_PROBE) && d[i].build.rows <= 0xFFFFFFF0ull &&
+               d[i].probe.rows <= 0xFFFFFFF0ull;
+    };
+    struct Span {
+        uint32_t i, e;
+        bool in_place;
+    };
+    std::vector<Span> spans;
+    std::vector<uint32_t> plain;
+    auto join_bytes = [&](uint32_t i) { return rel_bytes(d[i].build) + rel_bytes(d[i].probe); };
+    for (uint32_t i = 0; i < n;) {
+        if (!groupable(i)) {
+            plain.push_back(i++);
             continue;
         }
-        uint64_t bytes = rel_bytes(d[i].build) + rel_bytes(d[i].probe);
+        auto fits = [&](uint32_t e) {
+            return e < n && groupable(e) && same_schema(d[i].build, d[e].build) && same_schema(d[i].probe, d[e].probe);
+        };
+        // joins lying back to back form a group as they are
+        uint64_t bytes = join_bytes(i);
         uint32_t e = i + 1;
-        while (e < n && follows(d[e - 1].build, d[e].build) && follows(d[e - 1].probe, d[e].probe)) {
-            const uint64_t add = rel_bytes(d[e].build) + rel_bytes(d[e].probe);
-            if (bytes + add > kBatchGroupBytes)
-                break;
-            bytes += add;
-            ++e;
+        while (fits(e) && follows(d[e - 1].build, d[e].build) && follows(d[e - 1].probe, d[e].probe) &&
+               bytes + join_bytes(e) <= kBatchGroupBytes)
+            bytes += join_bytes(e++);
+        bool in_place = true;
+        if (e == i + 1 && join_bytes(i) <= kBatchStageBytes) {
+            // otherwise a run of small joins is gathered back to back
+            uint32_t k = i + 1;
+            bytes = join_bytes(i);
+            while (fits(k) && join_bytes(k) <= kBatchStageBytes && bytes + join_bytes(k) <= kBatchGroupBytes)
+                bytes += join_bytes(k++);
+            if (k > i + 1) {
+                e = k;
+                in_place = false;
+            }
         }
-        run_batch_group(ctx, d + i, e - i, res + i);
+        spans.push_back(Span{i, e, in_place});
         i = e;
     }
+    ctx->arena.reset_peak();
+    std::vector<std::unique_ptr<BatchGroup>> groups;
+    for (const Span &sp : spans) {
+        auto g = std::make_unique<BatchGroup>(ctx);
+        g->d = d + sp.i;
+        g->res = res + sp.i;
+        g->m = sp.e - sp.i;
+        groups.push_back(std::move(g));
+    }
+    // pipeline: start(g + 1) is queued before the host waits inside join(g)
+    if (!groups.empty())
+        group_start(ctx, *groups[0], spans[0].in_place, leaf_rows);
+    for (size_t g = 0; g < groups.size(); ++g) {
+        if (g + 1 < groups.size())
+            group_start(ctx, *groups[g + 1], spans[g + 1].in_place, leaf_rows);
+        group_join(ctx, *groups[g]);
+    }
+    ctx->sync();
+    const uint64_t peak = ctx->arena.peak();
+    for (auto &g : groups)
+        group_collect(ctx, *g, peak);
+    ctx->pinned_io.reset();
+    for (size_t g = 0; g < groups.size(); ++g)
+        for (uint32_t j = 0; j < groups[g]->m; ++j)
+            if (groups[g]->rerun[j]) {
+                const phj_join_desc &x = groups[g]->d[j];
+                run_phj_pure(ctx, x.build, x.probe, x.pass_bits, x.n_passes, x.tuning, &groups[g]->res[j]);
+            }
+    for (uint32_t i : plain)
+        run_phj_pure(ctx, d[i].build, d[i].probe, d[i].pass_bits, d[i].n_passes, d[i].tuning, &res[i]);
 }
 
 __global__ void k_u64_to_u32(const uint64_t *in, uint32_t *out, uint64_t n) {
@@ -2174,6 +2389,28 @@ ColSet colset_of_result(const phj_join_result *r, bool left) {
         c.ptr[i] = left ? r->left[i] : r->right[i];
     }
     return c;
+}
+
+// Returns a result's device memory: its own blocks, or its share of a batch
+// group's output (freed with the group's last result).
+void release_result(phj_ctx *ctx, phj_join_result &r) {
+    if (r.handle == 0)
+        return;
+    auto it = ctx->owned.find(r.handle);
+    if (it != ctx->owned.end()) {
+        for (void *b : it->second)
+            ctx->arena.free(b);
+        ctx->owned.erase(it);
+    }
+    auto sh = ctx->shared.find(r.handle);
+    if (sh != ctx->shared.end()) {
+        const std::shared_ptr<std::vector<void *>> bufs = sh->second;
+        ctx->shared.erase(sh);
+        if (bufs.use_count() == 1)
+            for (void *b : *bufs)
+                ctx->arena.free(b);
+    }
+    r.handle = 0;
 }
 
 template <class F> phj_status guard(phj_ctx *ctx, F &&f) {
@@ -2576,7 +2813,70 @@ phj_status phj_build_bytes(const phj_resident *build, uint64_t *bytes) {
 }
 
 phj_status phj_join_batch(phj_ctx *ctx, uint32_t n, const phj_join_desc *descs, phj_join_result *results) {
-    return guard(ctx, [&] { run_phj_batch(ctx, n, descs, results); });
+    return guard(ctx, [&] { run_phj_batch(ctx, n, descs, results, ctx->batch_leaf_rows); });
+}
+
+phj_status phj_batch_autotune(phj_ctx *ctx, uint32_t n, const phj_join_desc *descs, uint32_t *leaf_rows_out,
+                              double *class_ms_out) {
+    return guard(ctx, [&] {
+        if (n > 0 && descs == nullptr)
+            invalid("null descriptors");
+        // joins by size class
+        std::vector<std::vector<phj_join_desc>> by_class(kBatchClasses);
+        for (uint32_t i = 0; i < n; ++i)
+            by_class[size_class(descs[i].build.rows)].push_back(descs[i]);
+        const uint32_t candidates[] = {512, 1024, 2048, 4096};
+        std::vector<double> best_ms(kBatchClasses, 0.0);
+        for (uint32_t c = 0; c < kBatchClasses; ++c) {
+            auto &v = by_class[c];
+            if (v.empty())
+                continue;
+            std::vector<phj_join_result> res(v.size());
+            uint32_t table[kBatchClasses];
+            std::copy(std::begin(ctx->batch_leaf_rows), std::end(ctx->batch_leaf_rows), table);
+            double best = 1e300;
+            uint32_t pick = table[c];
+            for (uint32_t cand : candidates) {
+                if ((1ull << c) <= cand && cand != candidates[0])
+                    continue; // joins of this class fit one leaf at every candidate: nothing to tell apart
+                table[c] = cand;
+                double t_min = 1e300;
+                for (int rep = 0; rep < 3; ++rep) { // the first run warms the pool
+                    ctx->sync();
+                    const auto t0 = std::chrono::steady_clock::now();
+                    run_phj_batch(ctx, static_cast<uint32_t>(v.size()), v.data(), res.data(), table);
+                    ctx->sync();
+                    const double ms =
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                    for (auto &r : res)
+                        release_result(ctx, r);
+                    if (rep > 0)
+                        t_min = std::min(t_min, ms);
+                }
+                if (t_min < best) {
+                    best = t_min;
+                    pick = cand;
+                }
+            }
+            ctx->batch_leaf_rows[c] = pick;
+            best_ms[c] = best;
+        }
+        if (leaf_rows_out)
+            std::copy(std::begin(ctx->batch_leaf_rows), std::end(ctx->batch_leaf_rows), leaf_rows_out);
+        if (class_ms_out)
+            std::copy(best_ms.begin(), best_ms.end(), class_ms_out);
+    });
+}
+
+phj_status phj_batch_set_leaf_rows(phj_ctx *ctx, const uint32_t *leaf_rows) {
+    return guard(ctx, [&] {
+        if (leaf_rows == nullptr)
+            invalid("null table");
+        for (uint32_t c = 0; c < kBatchClasses; ++c)
+            if (leaf_rows[c] == 0)
+                invalid("leaf rows must be positive");
+        std::copy(leaf_rows, leaf_rows + kBatchClasses, ctx->batch_leaf_rows);
+    });
 }
 
 phj_status phj_run_chj(phj_ctx *ctx, const phj_relation *build, const phj_relation *probe, uint32_t,
@@ -2612,20 +2912,7 @@ phj_status phj_result_release(phj_ctx *ctx, phj_join_result *result) {
     return guard(ctx, [&] {
         if (result == nullptr || result->handle == 0)
             return;
-        auto it = ctx->owned.find(result->handle);
-        if (it != ctx->owned.end()) {
-            for (void *b : it->second)
-                ctx->arena.free(b);
-            ctx->owned.erase(it);
-        }
-        auto sh = ctx->shared.find(result->handle);
-        if (sh != ctx->shared.end()) {
-            const std::shared_ptr<std::vector<void *>> bufs = sh->second;
-            ctx->shared.erase(sh);
-            if (bufs.use_count() == 1)
-                for (void *b : *bufs)
-                    ctx->arena.free(b);
-        }
+        release_result(ctx, *result);
         std::memset(result, 0, sizeof(*result));
     });
 }

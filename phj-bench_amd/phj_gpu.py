@@ -29,6 +29,7 @@ MAX_COLS = 8
 MAX_PASSES = 8
 INVALID_BLOCK = 0xFFFFFFFF
 PIPELINE_BLOCK_ROWS = 10_000  # src/Types.h:108
+BATCH_CLASSES = 33  # batch tuning size classes, floor(log2(build rows))
 
 # src/Types.h:27-34
 U8, U16, U32, U64, U128 = 0, 1, 2, 3, 4
@@ -182,6 +183,8 @@ _SIGNATURES = {
     "phj_build_release": ([C.c_void_p, C.c_void_p], C.c_int),
     "phj_build_bytes": ([C.c_void_p, C.POINTER(C.c_uint64)], C.c_int),
     "phj_join_batch": ([C.c_void_p, C.c_uint32, C.POINTER(_JoinDesc), C.POINTER(_JoinResult)], C.c_int),
+    "phj_batch_autotune": ([C.c_void_p, C.c_uint32, C.POINTER(_JoinDesc), C.c_void_p, C.c_void_p], C.c_int),
+    "phj_batch_set_leaf_rows": ([C.c_void_p, C.c_void_p], C.c_int),
     "phj_run_chj": ([C.c_void_p, C.POINTER(_Relation), C.POINTER(_Relation), C.c_uint32, C.POINTER(_JoinResult)], C.c_int),
     "phj_run_phj_pure": ([C.c_void_p, C.POINTER(_Relation), C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.c_uint32,
                           C.POINTER(_JoinResult)], C.c_int),
@@ -764,6 +767,27 @@ class Context:
         _raise(self._p, load_library().phj_join_batch(self._p, n, descs, res))
         # each result is a view of its element of the array (which it keeps alive)
         return [JoinResult(self, res[i]) for i in range(n)]
+
+    def batch_autotune(self, pairs, cfgs=None):
+        """phj_batch_autotune: per size class floor(log2(build rows)) present in
+        the pairs, the fastest leaf size (build rows) found by timing; kept by
+        this context for its later join_batch calls. Returns
+        {class: (leaf_rows, best_ms)} for the classes present."""
+        n = len(pairs)
+        if cfgs is None:
+            cfgs = [RadixConfig.for_build_rows(b.rows) for b, _ in pairs]
+        descs = _batch_descs(pairs, cfgs, None)
+        table = np.zeros(BATCH_CLASSES, dtype=np.uint32)
+        ms = np.zeros(BATCH_CLASSES, dtype=np.float64)
+        _raise(self._p, load_library().phj_batch_autotune(self._p, n, descs, table.ctypes.data, ms.ctypes.data))
+        return {c: (int(table[c]), float(ms[c])) for c in range(BATCH_CLASSES) if ms[c] > 0}
+
+    def batch_set_leaf_rows(self, table):
+        """Sets the leaf-size table of join_batch (BATCH_CLASSES entries)."""
+        t = np.ascontiguousarray(table, dtype=np.uint32)
+        if t.shape != (BATCH_CLASSES,):
+            raise ValueError(f"the table has {BATCH_CLASSES} entries")
+        _raise(self._p, load_library().phj_batch_set_leaf_rows(self._p, t.ctypes.data))
 
     def runCHJ(self, build, probe, threads=1, **kw) -> JoinResult:
         return self.join(SCHEME_CHJ, build, probe, None, threads, **kw)

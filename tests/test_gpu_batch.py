@@ -98,3 +98,41 @@ def test_batch_empty_joins(ctx):
         r.release()
     build.release()
     probe.release()
+
+
+def test_batch_gathers_scattered_small_joins(ctx):
+    """Small joins whose relations lie apart (each generated on its own) run
+    as one group gathered back to back; a large one among them runs in place;
+    every join equals its closed form."""
+    specs = [G.gen_spec(seed=40 + i, build_rows=n, probe_rows=5 * n, match_rate=0.6, build_types=(G.U16, G.U64),
+                        probe_types=(G.U128,)) for i, n in enumerate((300, 4000, 1, 20000, 2_000_000, 7000, 0, 900))]
+    rels = [ctx.generate(s) for s in specs]
+    res = ctx.join_batch(rels)
+    for s, r in zip(specs, res):
+        assert r.checksum() == ctx.expected_checksum(s)
+        r.release()
+    for b, p in rels:
+        b.release()
+        p.release()
+
+
+def test_batch_autotune(ctx):
+    """The autotuner picks a leaf size per size class present and the batch
+    keeps its results with the tuned table (and with a hand-set one)."""
+    specs = G.batched_join_specs(seed=17, n_joins=48, lo_log2=10, hi_log2=17, probe_factor=4, match_rate=0.75)
+    build, probe, boff, poff = ctx.generate_batch(specs)
+    pairs = pairs_of(build, probe, boff, poff)
+    table = ctx.batch_autotune(pairs)
+    classes = {int(np.log2(b.rows)) for b, _ in pairs}
+    assert set(table) == classes
+    assert all(rows in (512, 1024, 2048, 4096) and ms > 0 for rows, ms in table.values())
+    for leaf_rows in (None, 600):
+        if leaf_rows is not None:
+            ctx.batch_set_leaf_rows([leaf_rows] * G.BATCH_CLASSES)
+        res = ctx.join_batch(pairs)
+        for s, r in zip(specs, res):
+            assert r.checksum() == ctx.expected_checksum(s)
+            r.release()
+    ctx.batch_set_leaf_rows([2048] * G.BATCH_CLASSES)
+    build.release()
+    probe.release()
