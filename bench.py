@@ -235,7 +235,9 @@ def serving_bench(n_requests, rates, workers=4, budget_gb=200, scale_log2=0):
     """Open-loop Poisson requests (|R| = 2^U[12,24], |S| = U[4,16] |R|, match
     U[0.1,1], 1-3 payload columns of u32/u64/u128 per side; 30% probe one of 64
     resident build sides) through the serving loop at each offered rate. Every
-    20th request's checksum is compared with the same request run alone."""
+    20th request's checksum, and every request kind run alone, is compared with
+    the closed form of its generator spec. Small requests are batched
+    continuously (serve.JoinServer)."""
     import serve
 
     ctx = G.Context(0)
@@ -248,13 +250,17 @@ def serving_bench(n_requests, rates, workers=4, budget_gb=200, scale_log2=0):
     # warm-up: every worker's memory pool grows to its working size
     srv.run(kinds, serve.poisson_schedule(200, 4 * len(kinds), len(kinds), seed=1))
     log("serve: warm-up done")
-    points, parity = [], True
+    exact = [serve.expected(ctx, k) for k in kinds]
+    parity = alone == exact
+    points = []
     for rate in rates:
         sched = serve.poisson_schedule(rate, n_requests, len(kinds), seed=rate, n_new=n_new, resident_share=0.3)
         outs = srv.run(kinds, sched, verify_every=20)
-        parity = parity and all(o.checksum is None or (o.out_rows, o.checksum) == alone[o.kind] for o in outs)
+        parity = parity and all(o.checksum is None or (o.out_rows, o.checksum) == exact[o.kind] for o in outs)
         sm = serve.summarize(outs, kinds)
         sm["offered_per_s"] = rate
+        sm["batched_calls"] = srv.batches
+        sm["batched_requests"] = srv.batched_requests
         points.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in sm.items()})
         log(f"serve {rate}/s: p50 {sm['p50_ms']:.2f} ms p99 {sm['p99_ms']:.2f} ms, "
             f"{sm['completed_per_s']:.0f} req/s, {sm['tuples_per_s']:.3g} tuples/s")
@@ -269,7 +275,8 @@ def serving_bench(n_requests, rates, workers=4, budget_gb=200, scale_log2=0):
     sat = max(points, key=lambda q: q["completed_per_s"])
     return {"workers": workers, "budget_gb": budget_gb, "requests_per_point": n_requests, "points": points,
             "saturation_req_per_s": sat["completed_per_s"], "saturation_tuples_per_s": sat["tuples_per_s"],
-            "parity_alone_vs_mix": parity,
+            "parity_alone_vs_mix": parity, "parity": "every 20th request, and every kind alone, against the "
+                                                   "generator's closed-form checksum",
             "workload": "Poisson arrivals; |R| = 2^U[12,24], |S| = U[4,16]|R|, match U[0.1,1], 1-3 payload columns "
                         "of u32/u64/u128 per side, 30% of requests probe one of 64 resident build sides"}
 

@@ -10,6 +10,10 @@ one HIP stream (Context) each, admit and run requests, so up to W joins are in
 flight on the GPU at once; a worker admits the next request as soon as its
 current one finishes.
 
+Small requests with their own build sides are batched continuously: a worker
+admitting one also admits the queued requests of the same schema that fit
+the budget and joins them in one phj_join_batch call (JoinServer).
+
 Latency is completion minus arrival. The reference has no serving mode; its
 unit of work, runPHJ (src/PHJ.h:31), is what every request runs.
 """
@@ -124,12 +128,37 @@ def run_request(ctx, kind: RequestInput, want_checksum=False):
         r.release()
 
 
-class JoinServer:
-    """W workers (one Context each) behind a FIFO admission queue bounded by HBM bytes."""
+def schema_of(kind: RequestInput):
+    b = kind.build.types if kind.build is not None else None
+    return (tuple(b) if b is not None else None, tuple(kind.probe.types))
 
-    def __init__(self, workers=4, budget_bytes=200 << 30, device=0):
+
+def run_batch(ctx, batch, want_checksum=False):
+    """Requests with their own (small) build sides joined in one
+    phj_join_batch call; returns (out_rows, checksum) per request."""
+    res = ctx.join_batch([(k.build, k.probe) for k in batch], [k.cfg for k in batch])
+    try:
+        return [(r.out_rows, r.checksum() if want_checksum else None) for r in res]
+    finally:
+        for r in res:
+            r.release()
+
+
+class JoinServer:
+    """W workers (one Context each) behind a FIFO admission queue bounded by HBM bytes.
+
+    Continuous batching: a worker that admits a request with its own build
+    side of at most batch_rows build rows also admits every other queued
+    request of that kind (small, same schema) that fits the budget, up to
+    max_batch, and joins them in one phj_join_batch call."""
+
+    def __init__(self, workers=4, budget_bytes=200 << 30, device=0, batch_rows=1 << 18, max_batch=64):
         self.ctxs = [G.Context(device) for _ in range(workers)]
         self.budget = budget_bytes
+        self.batch_rows = batch_rows
+        self.max_batch = max_batch
+        self.batches = 0           # batched calls made in the last run
+        self.batched_requests = 0  # requests they joined
 
     def close(self):
         for c in self.ctxs:
@@ -148,6 +177,9 @@ class JoinServer:
         errors = []
         t0 = time.perf_counter()
 
+        def small(k):
+            return k.resident is None and k.build_rows <= self.batch_rows
+
         def worker(ctx):
             while True:
                 with cond:
@@ -161,18 +193,41 @@ class JoinServer:
                         elif state["done"]:
                             return
                         cond.wait()
-                o = outcomes[i]
-                kind = kinds[o.kind]
-                o.admitted = time.perf_counter() - t0
+                    group = [i]
+                    if small(kinds[outcomes[i].kind]):
+                        sch = schema_of(kinds[outcomes[i].kind])
+                        for j in list(pending):
+                            if len(group) >= self.max_batch:
+                                break
+                            kj = kinds[outcomes[j].kind]
+                            if small(kj) and schema_of(kj) == sch and state["inflight"] + kj.footprint <= self.budget:
+                                pending.remove(j)
+                                state["inflight"] += kj.footprint
+                                group.append(j)
+                now = time.perf_counter() - t0
+                for g in group:
+                    outcomes[g].admitted = now
+                want = [bool(verify_every) and g % verify_every == 0 for g in group]
                 try:
-                    o.out_rows, o.checksum = run_request(ctx, kind, bool(verify_every) and i % verify_every == 0)
+                    if len(group) == 1:
+                        outs = [run_request(ctx, kinds[outcomes[i].kind], want[0])]
+                    else:
+                        outs = run_batch(ctx, [kinds[outcomes[g].kind] for g in group], any(want))
+                        with cond:
+                            self.batches += 1
+                            self.batched_requests += len(group)
+                    for g, w, (rows, cs) in zip(group, want, outs):
+                        outcomes[g].out_rows, outcomes[g].checksum = rows, (cs if w else None)
                 except Exception as e:  # reported by run(); the footprint is still released
                     errors.append((i, repr(e)))
-                o.done = time.perf_counter() - t0
+                done = time.perf_counter() - t0
                 with cond:
-                    state["inflight"] -= kind.footprint
+                    for g in group:
+                        outcomes[g].done = done
+                        state["inflight"] -= kinds[outcomes[g].kind].footprint
                     cond.notify_all()
 
+        self.batches = self.batched_requests = 0
         threads = [threading.Thread(target=worker, args=(c,), daemon=True) for c in self.ctxs]
         for th in threads:
             th.start()

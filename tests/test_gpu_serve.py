@@ -115,3 +115,26 @@ def test_resident_release_refused_while_probing(ctx):
     other.close()
     b.release()
     p.release()
+
+
+def test_serving_batches_queued_small_requests(ctx):
+    """A burst of arrivals queues many small requests: workers admit same-schema
+    small requests together and join them in one phj_join_batch call; every
+    served result still equals its closed form, batched or not."""
+    kinds, residents, n_new = serve.request_pool(ctx, seed=13, n_new=5, n_resident=2, n_resident_probes=2, lo_log2=9,
+                                                 hi_log2=14, probe_lo=2, probe_hi=6)
+    exact = [serve.expected(ctx, k) for k in kinds]
+    srv = serve.JoinServer(workers=2, budget_bytes=1 << 32)
+    sched = serve.poisson_schedule(rate=200000, n_requests=120, n_kinds=len(kinds), seed=5, n_new=n_new,
+                                   resident_share=0.2)
+    outs = srv.run(kinds, sched, verify_every=1)
+    assert srv.batches > 0 and srv.batched_requests >= 2 * srv.batches
+    for o in outs:
+        assert (o.out_rows, o.checksum) == exact[o.kind]
+    srv.close()
+    for k in kinds:
+        for rel in (k.build, k.probe):
+            if rel is not None:
+                rel.release()
+    for r in residents:
+        r.release()
