@@ -343,7 +343,8 @@ def headline(args, tuning):
     passes = 2
     per_step_bytes = {
         "radix_scatter": passes * 2 * (R * wR + S * wS),  # every pass reads and writes each row once
-        "radix_hist": passes * 8 * (R + S),                # every pass reads the keys once
+        # the first pass reads the keys; the second the digit byte column the first scatter wrote
+        "radix_hist": (8 + 1) * (R + S),
         # fused leaf join: both sides' rows read once, output written once
         "leaf_join": R * wR + S * wS + out_rows * wO,
         # two-phase leaf join (repeated build keys or PHJ_TUNE_TWO_PHASE_JOIN)
