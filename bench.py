@@ -100,6 +100,28 @@ def cpu_baseline(scale_log2, threads):
             "config1": config1}
 
 
+def cpu_baseline_child(scale_log2, threads):
+    """cpu_baseline in a child process (it runs the compiled reference, whose
+    faults must not end the benchmark): its JSON, or None with the failure
+    logged. One retry."""
+    for attempt in (1, 2):
+        log(f"CPU baseline: compiled reference on {threads} threads (child process, attempt {attempt})")
+        try:
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--scale-log2",
+                                str(scale_log2), "--cpu-threads", str(threads)], capture_output=True, text=True,
+                               timeout=600)
+        except subprocess.TimeoutExpired:
+            log("CPU baseline: child timed out")
+            continue
+        for line in r.stderr.splitlines():
+            if line.startswith("[bench]"):
+                print(line, file=sys.stderr, flush=True)
+        if r.returncode == 0 and r.stdout.strip():
+            return json.loads(r.stdout.strip().splitlines()[-1])
+        log(f"CPU baseline: child exited with {r.returncode}")
+    return None
+
+
 def batched_bench(ctx, n_joins, seed=42, plain=True):
     """4096 independent uniform joins (|R_i| = floor(2^U[10,20]) distinct keys,
     |S_i| = 8 |R_i|, 75% match, u64 payload each side), back to back in HBM:
@@ -396,7 +418,11 @@ def main():
     ap.add_argument("--workloads", default="main,batch,skew,serve",
                     help="comma-separated subset of main (the headline join), batch, skew, serve; a run without "
                          "main prints value null (for profiling one workload)")
+    ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_only:  # the child process of cpu_baseline_child; touches no GPU
+        print(json.dumps(cpu_baseline(args.scale_log2, args.cpu_threads or host_threads())), flush=True)
+        return
     wl = set(w for w in args.workloads.split(",") if w)
     if "batch" not in wl:
         args.batch_joins = 0
@@ -425,11 +451,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or host_threads()
-        try:
-            log(f"CPU baseline: compiled reference runPHJ on {threads} threads")
-            cpu = cpu_baseline(args.scale_log2, threads)
-        except Exception as e:  # reported, not fatal
-            log(f"CPU baseline failed: {e}")
+        cpu = cpu_baseline_child(args.scale_log2, threads)
 
     hv = h or {}
     line = {
