@@ -231,7 +231,8 @@ def test_join_matches_reference(ctx, dsname, scheme, cfg):
 
 @pytest.mark.parametrize("flags", [dict(flags=1), dict(lds_leaf_rows=8), dict(ht_load_pct=80),
                                    dict(join_threads=256), dict(join_threads=1024), dict(probe_chunk_rows=1000),
-                                   dict(flags=4), dict(flags=8), dict(flags=12)])
+                                   dict(flags=4), dict(flags=8), dict(flags=12), dict(flags=0x10),
+                                   dict(flags=0x12)])
 def test_phj_pure_tuning_paths(ctx, flags):
     ob, op = data()["zipf_multi"]
     want = O.serialize_sorted(O.reference_join(ob, op))
