@@ -665,6 +665,88 @@ __device__ __forceinline__ void move_set(const ColSet &in, uint64_t src_row0, co
     }
 }
 
+// Per-slot emit metadata of the fused join: bit 31 = the slot has an output
+// row, bits 16..30 = its output row relative to the round's base, bits 0..15 =
+// the matched build row within the leaf.
+__device__ __forceinline__ bool meta_on(uint32_t m) { return (m >> 31) != 0; }
+__device__ __forceinline__ uint32_t meta_pos(uint32_t m) { return (m >> 16) & 0x7FFFu; }
+__device__ __forceinline__ uint32_t meta_row(uint32_t m) { return m & 0xFFFFu; }
+
+// One column over all K slots of a lane, G slots at a time: the G loads are
+// issued together, then the G stores (a row at a time would wait for each
+// load before the next is issued, since the compiler cannot move a load above
+// a store that may alias it). PROBE: source rows are the slots' probe rows
+// (slot k of the lane is row k * THREADS + threadIdx.x of the round); else the
+// matched build rows. Slots without a row load row 0 and store nothing.
+template <int THREADS, int K, bool PROBE, class T, int G>
+__device__ __forceinline__ void sweep_load(const void *src, const uint32_t (&meta)[K], uint32_t tid, int h, T (&v)[G]) {
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int k = h * G + u;
+        const uint32_t row = PROBE ? static_cast<uint32_t>(k * THREADS) + tid : meta_row(meta[k]);
+        v[u] = ld_at<T>(src, (meta_on(meta[k]) ? row : 0u) * static_cast<uint32_t>(sizeof(T)));
+    }
+}
+template <int K, class T, int G>
+__device__ __forceinline__ void sweep_store(void *dst, const uint32_t (&meta)[K], int h, const T (&v)[G]) {
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int k = h * G + u;
+        if (meta_on(meta[k]))
+            st_at<T>(dst, meta_pos(meta[k]) * static_cast<uint32_t>(sizeof(T)), v[u]);
+    }
+}
+template <int THREADS, int K, bool PROBE, class T>
+__device__ __forceinline__ void sweep_col(const void *src, void *dst, const uint32_t (&meta)[K], uint32_t tid) {
+    constexpr int G = sizeof(T) >= 8 ? static_cast<int>(32 / sizeof(T)) : 8; // <= 8 VGPRs of values per group
+    static_assert(K % G == 0, "slots in groups");
+    // group h + 1's loads are issued before group h's stores
+    T cur[G];
+    sweep_load<THREADS, K, PROBE, T, G>(src, meta, tid, 0, cur);
+#pragma unroll
+    for (int h = 0; h < K / G; ++h) {
+        T nxt[G];
+        if (h + 1 < K / G)
+            sweep_load<THREADS, K, PROBE, T, G>(src, meta, tid, h + 1, nxt);
+        sweep_store<K, T, G>(dst, meta, h, cur);
+        if (h + 1 < K / G) {
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                cur[u] = nxt[u];
+        }
+    }
+}
+
+// Every column of a set (source rows from src_row0, output rows from dst_row0).
+template <int THREADS, int K, bool PROBE>
+__device__ __forceinline__ void sweep_set(const ColSet &in, uint64_t src_row0, const OutColSet &out, uint64_t dst_row0,
+                                          const uint32_t (&meta)[K]) {
+    for (uint32_t c = 0; c < in.n; ++c) {
+        const uint32_t t = col_type(in, c);
+        const uint64_t sz = type_size(t);
+        const void *src = static_cast<const char *>(col_ptr(in, c)) + src_row0 * sz;
+        void *dst = static_cast<char *>(col_ptr(out, c)) + dst_row0 * sz;
+        // An opaque copy per column: offsets derived from the metadata are
+        // then computed inside the column loop rather than hoisted out of it
+        // for every column type at once (which spills).
+        uint32_t m[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            m[k] = meta[k];
+            asm volatile("" : "+v"(m[k]));
+        }
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        switch (t) {
+        case 0: sweep_col<THREADS, K, PROBE, uint8_t>(src, dst, m, tid); break;
+        case 1: sweep_col<THREADS, K, PROBE, uint16_t>(src, dst, m, tid); break;
+        case 2: sweep_col<THREADS, K, PROBE, uint32_t>(src, dst, m, tid); break;
+        case 3: sweep_col<THREADS, K, PROBE, uint64_t>(src, dst, m, tid); break;
+        default: sweep_col<THREADS, K, PROBE, ulonglong2>(src, dst, m, tid); break;
+        }
+    }
+}
+
 // Leaf table of the fused join: keys[cap] (u64, kEmptyKey = free) and the
 // leaf-local row of each key, rows[cap] (u16); no chains, since the fused
 // join only runs without repeated keys. About 10 bytes per slot, so three
@@ -907,49 +989,63 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
         __syncthreads();
         if (sr_total != 0) {
             const uint64_t rb = sr_base;
+            if constexpr (NARROW) {
 #pragma unroll
-            for (int r = 0; r < K / U; ++r) {
-                if (sb + r * U * THREADS >= it.s1)
-                    continue; // uniform: rounds past the chunk's end
-                uint32_t ridx[U], sidx[U], opos[U];
-                uint64_t okey[U];
-                bool on[U];
+                for (int r = 0; r < K / U; ++r) {
+                    if (sb + r * U * THREADS >= it.s1)
+                        continue; // uniform: rounds past the chunk's end
+                    uint32_t ridx[U], sidx[U], opos[U];
+                    uint64_t okey[U];
+                    bool on[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int k = r * U + u;
-                    const uint32_t c = (code[k / 2] >> (16 * (k % 2))) & 0xFFFFu;
-                    const uint64_t m = __ballot(c != 0);
-                    opos[u] = slot_off[k][wave] + mbcnt(m);
-                    on[u] = c != 0 && rb + opos[u] < out_cap;
-                    sidx[u] = k * THREADS + threadIdx.x;
-                    uint32_t row = 0;
-                    okey[u] = 0;
-                    if (on[u]) {
-                        if (c - 1 == kSideSlot) {
-                            okey[u] = kEmptyKey;
-                            row = side_head;
-                        } else {
-                            okey[u] = tkeys[c - 1];
-                            row = trows[c - 1];
+                    for (int u = 0; u < U; ++u) {
+                        const int k = r * U + u;
+                        const uint32_t c = (code[k / 2] >> (16 * (k % 2))) & 0xFFFFu;
+                        const uint64_t m = __ballot(c != 0);
+                        opos[u] = slot_off[k][wave] + mbcnt(m);
+                        on[u] = c != 0 && rb + opos[u] < out_cap;
+                        sidx[u] = k * THREADS + threadIdx.x;
+                        uint32_t row = 0;
+                        okey[u] = 0;
+                        if (on[u]) {
+                            if (c - 1 == kSideSlot) {
+                                okey[u] = kEmptyKey;
+                                row = side_head;
+                            } else {
+                                okey[u] = tkeys[c - 1];
+                                row = trows[c - 1];
+                            }
                         }
+                        ridx[u] = row;
                     }
-                    ridx[u] = row;
-                }
-                uint64_t *okeys = a.out.keys + rb;
+                    uint64_t *okeys = a.out.keys + rb;
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (on[u])
-                        st_at<uint64_t>(okeys, opos[u] * 8u, okey[u]);
-                move_set<U>(a.r_cols, it.r0, a.out.left, rb, ridx, opos, on);
-                if constexpr (NARROW) {
+                    for (int u = 0; u < U; ++u)
+                        if (on[u])
+                            st_at<uint64_t>(okeys, opos[u] * 8u, okey[u]);
+                    move_set<U>(a.r_cols, it.r0, a.out.left, rb, ridx, opos, on);
                     uint32_t srow[U];
 #pragma unroll
                     for (int u = 0; u < U; ++u)
                         srow[u] = a.s_rowid[sb + (on[u] ? sidx[u] : 0u)];
                     gather_set_abs<U>(a.s_src, a.out.right, rb, srow, opos, on);
-                } else {
-                    move_set<U>(a.s_cols, sb, a.out.right, rb, sidx, opos, on);
                 }
+            } else {
+                // slot k: output row, build row; the output key is the probe
+                // key (equal to the matched build key, the sentinel included)
+                uint32_t meta[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint32_t c = (code[k / 2] >> (16 * (k % 2))) & 0xFFFFu;
+                    const uint64_t m = __ballot(c != 0);
+                    const uint32_t opos = slot_off[k][wave] + mbcnt(m);
+                    const bool on = c != 0 && rb + opos < out_cap;
+                    const uint32_t row = c == 0 ? 0u : (c - 1 == kSideSlot ? side_head : trows[c - 1]);
+                    meta[k] = on ? (0x80000000u | (opos << 16) | row) : 0u;
+                }
+                sweep_col<THREADS, K, true, uint64_t>(a.s_keys + sb, a.out.keys + rb, meta, threadIdx.x);
+                sweep_set<THREADS, K, false>(a.r_cols, it.r0, a.out.left, rb, meta);
+                sweep_set<THREADS, K, true>(a.s_cols, sb, a.out.right, rb, meta);
             }
         }
         __syncthreads(); // slot_off and sr_* are rewritten by the next super-round
