@@ -1314,11 +1314,61 @@ struct BepRun {
     uint32_t len(uint32_t g) const { return off[g + 1] - off[g]; }
 };
 
+// Leaf tables of the build side for the eviction rounds: built once in HBM
+// (single-row cells without repeated keys, group cells with them), or none
+// (each round's work items build theirs in LDS).
+struct BepTables {
+    enum Kind { kLds, kUnique, kGroup } kind = kLds;
+    LeafTableImages unique;
+    GroupImage group{};
+    uint32_t leaf_bits = 0;
+};
+
+// Group images of a partitioned build side: cells, side groups and the
+// grouped build columns, in `keep`. False when a leaf is too large for the
+// LDS build or the plan asks for LDS or HBM leaf tables.
+bool build_group_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan &plan, uint32_t leaf_bits,
+                        uint32_t max_leaf, Held &keep, GroupImage &g, double &build_ms) {
+    constexpr size_t kLdsLimit = 150u * 1024u;
+    if (plan.two_phase || plan.force_global || RP.parts == 0 || max_leaf == 0 ||
+        group_image_lds_bytes(max_leaf, plan.load_pct) > kLdsLimit ||
+        (plan.lds_limit_rows && max_leaf > plan.lds_limit_rows))
+        return false;
+    const uint32_t n_leaves = static_cast<uint32_t>(RP.parts);
+    EventPair ev;
+    HIPCHK(hipEventRecord(ev.a, ctx->stream));
+    const uint64_t stride = leaf_table_bytes(max_leaf, plan.load_pct) / sizeof(Cell16);
+    Cell16 *cells = ctx->alloc<Cell16>(stride * n_leaves);
+    keep.push_back(cells);
+    uint32_t *side = ctx->alloc<uint32_t>(2ull * n_leaves);
+    keep.push_back(side);
+    OutColSet gcols = alloc_cols(ctx, as_in(RP.cols), RP.rows, keep);
+    {
+        Scratch scr(ctx);
+        uint32_t *perm = scr.get<uint32_t>(RP.rows);
+        group_images(ctx->stream, RP.keys, RP.off, n_leaves, max_leaf, plan.load_pct, cells, stride, side, perm,
+                     ctx->tm());
+        gather_perm(ctx->stream, as_in(RP.cols), gcols, perm, RP.rows, ctx->tm());
+    }
+    HIPCHK(hipEventRecord(ev.b, ctx->stream));
+    g.cells = cells;
+    g.stride = stride;
+    g.side = side;
+    g.r_off = RP.off;
+    g.cols = as_in(gcols);
+    g.leaf_bits = leaf_bits;
+    g.load_pct = plan.load_pct;
+    HIPCHK(hipEventSynchronize(ev.b));
+    build_ms += ev.ms();
+    return true;
+}
+
 class BepSchedule {
 public:
     BepSchedule(phj_ctx *ctx, const Partitioned &RP, const DevRel &S, uint32_t p1_bits, const std::vector<uint32_t> &rest,
-                uint64_t budget, const LeafJoinPlan &plan, uint32_t flags, const LeafTableImages *img, OutBuffers &out)
-        : ctx_(ctx), RP_(RP), S_(S), p1_(p1_bits), rest_(rest), plan_(plan), flags_(flags), img_(img), out_(out) {
+                uint64_t budget, const LeafJoinPlan &plan, uint32_t flags, const BepTables &tables, OutBuffers &out)
+        : ctx_(ctx), RP_(RP), S_(S), p1_(p1_bits), rest_(rest), plan_(plan), flags_(flags), tab_(tables), out_(out),
+          keep_(ctx) {
         P_ = 1u << p1_;
         leaves_ = static_cast<uint32_t>(RP.parts);
         lpp_ = leaves_ / P_;
@@ -1333,13 +1383,28 @@ public:
         batch_ = std::max<uint64_t>((u_hi_ / 4 + tile - 1) / tile * tile, tile);
         bucket_rows_.assign(P_, 0);
         leaf_rows_.assign(leaves_, 0);
+        for (auto &b : scat_off_)
+            b = ctx_->pinned_io.get<uint32_t>(P_ + 1);
+        leaf_off_ = ctx_->pinned_io.get<uint32_t>(static_cast<uint64_t>(leaves_) + 1);
+        cursor_host_ = ctx_->pinned_io.get<unsigned long long>(1);
+        if (tab_.kind != BepTables::kLds) {
+            dcursor_ = ctx_->alloc<unsigned long long>(1);
+            keep_.push_back(dcursor_);
+            launch_fill_u64(ctx_->stream, reinterpret_cast<uint64_t *>(dcursor_), 1, 0);
+        }
     }
 
     void run() {
-        for (uint64_t a = 0; a < S_.rows; a += batch_) {
+        const uint64_t nb = (S_.rows + batch_ - 1) / batch_;
+        // the next batch's scatter is queued before the host waits for this
+        // batch's bucket counts, so the device does not idle meanwhile
+        std::unique_ptr<Pending> cur = nb ? launch_scatter(0) : nullptr;
+        for (uint64_t b = 0; b < nb; ++b) {
+            std::unique_ptr<Pending> next = b + 1 < nb ? launch_scatter(b + 1) : nullptr;
             while (batch_runs_.size() >= kBepMaxRuns)
                 refine(held_by(*batch_runs_.front(), P_, [](uint32_t g) { return g; }));
-            scatter(a, std::min(batch_, S_.rows - a));
+            finish_scatter(*cur);
+            cur = std::move(next);
             peak_ = std::max(peak_, global_leaf_ * rb_);
             // src/PHJBep.cpp:508-533: drain own unrefined to the low-water mark, argmax first
             if (unrefined_ >= u_hi_) {
@@ -1392,6 +1457,22 @@ public:
         }
         if (!E.empty())
             evict(E);
+        if (dcursor_) {
+            HIPCHK(hipMemcpyAsync(cursor_host_, dcursor_, 8, hipMemcpyDeviceToHost, ctx_->stream));
+            ctx_->sync();
+            n_out_ = *cursor_host_;
+            if (n_out_ > out_.cap)
+                throw std::runtime_error("best-effort join wrote past its output bound");
+        } else {
+            ctx_->sync();
+        }
+        upload_.reset();
+        for (auto &e : shuffle_ev_)
+            shuffle_ms_ += e->ms();
+        for (auto &e : probe_ev_)
+            probe_ms_ += e->ms();
+        for (auto &e : gather_ev_)
+            overhead_ms_ += e->ms();
     }
 
     uint64_t out_rows() const { return n_out_; }
@@ -1404,6 +1485,23 @@ public:
     double big_build_ms() const { return big_build_ms_; }
 
 private:
+    // A batch scatter in flight: its run and the pinned copy of its offsets.
+    struct Pending {
+        std::unique_ptr<BepRun> run;
+        uint32_t *off;
+        hipEvent_t done = nullptr;
+        ~Pending() {
+            if (done)
+                (void)hipEventDestroy(done);
+        }
+    };
+
+    EventPair &timed(std::vector<std::unique_ptr<EventPair>> &v) {
+        v.push_back(std::make_unique<EventPair>());
+        HIPCHK(hipEventRecord(v.back()->a, ctx_->stream));
+        return *v.back();
+    }
+
     // Groups of `run` still holding rows, mapped to bucket or leaf ids.
     template <class F> std::vector<uint32_t> held_by(const BepRun &run, uint32_t groups, F id) const {
         std::vector<uint32_t> v;
@@ -1415,39 +1513,50 @@ private:
         return v;
     }
 
-    // Pass-1 scatter of probe rows [a, a + n) into a new batch run.
-    void scatter(uint64_t a, uint64_t n) {
-        HIPCHK(hipEventRecord(ev_.a, ctx_->stream));
-        auto run = std::make_unique<BepRun>(ctx_);
+    // Pass-1 scatter of probe rows of batch b into a new batch run; its
+    // bucket offsets are copied back into pinned memory.
+    std::unique_ptr<Pending> launch_scatter(uint64_t b) {
+        const uint64_t a = b * batch_, n = std::min(batch_, S_.rows - a);
+        auto p = std::make_unique<Pending>();
+        p->run = std::make_unique<BepRun>(ctx_);
+        p->off = scat_off_[b % 2];
+        EventPair &ev = timed(shuffle_ev_);
         DevRel slice = S_;
         slice.rows = n;
         slice.keys = S_.keys + a;
         for (uint32_t c = 0; c < S_.cols.n; ++c)
             slice.cols.ptr[c] = static_cast<const uint8_t *>(S_.cols.ptr[c]) + a * type_size(S_.cols.types[c]);
-        const Partitioned P1 = radix_partition(ctx_, slice, nullptr, 1, 0, {p1_}, run->bufs, "probe", flags_);
-        HIPCHK(hipEventRecord(ev_.b, ctx_->stream));
-        run->rel.rows = n;
-        run->rel.keys = P1.keys;
-        run->rel.cols = as_in(P1.cols);
-        run->off.resize(P_ + 1);
-        HIPCHK(hipMemcpyAsync(run->off.data(), P1.off, (P_ + 1) * 4ull, hipMemcpyDeviceToHost, ctx_->stream));
-        ctx_->sync();
-        shuffle_ms_ += ev_.ms();
-        run->done.assign(P_, 0);
-        run->live = n;
+        const Partitioned P1 = radix_partition(ctx_, slice, nullptr, 1, 0, {p1_}, p->run->bufs, "probe", flags_);
+        HIPCHK(hipEventRecord(ev.b, ctx_->stream));
+        p->run->rel.rows = n;
+        p->run->rel.keys = P1.keys;
+        p->run->rel.cols = as_in(P1.cols);
+        HIPCHK(hipMemcpyAsync(p->off, P1.off, (P_ + 1) * 4ull, hipMemcpyDeviceToHost, ctx_->stream));
+        HIPCHK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(p->done, ctx_->stream));
+        return p;
+    }
+
+    void finish_scatter(Pending &p) {
+        HIPCHK(hipEventSynchronize(p.done));
+        BepRun &run = *p.run;
+        run.off.assign(p.off, p.off + P_ + 1);
+        run.done.assign(P_, 0);
+        run.live = run.rel.rows;
         for (uint32_t q = 0; q < P_; ++q)
-            bucket_rows_[q] += run->len(q);
-        unrefined_ += n;
-        batch_runs_.push_back(std::move(run));
+            bucket_rows_[q] += run.len(q);
+        unrefined_ += run.rel.rows;
+        batch_runs_.push_back(std::move(p.run));
     }
 
     // Rows of group g of each run in `runs` (for which pick() gives g, or -1),
     // gathered into one relation: groups in the caller's order, each group's
     // rows from the oldest run first. Returns the staged relation (buffers in
-    // `keep`) and the row count per caller item.
+    // `keep`) and the row count per caller item. The segment list goes up
+    // through pinned memory; nothing waits.
     template <class Pick>
     DevRel gather(std::vector<std::unique_ptr<BepRun>> &runs, uint32_t items, Pick pick, Held &keep,
-                  std::vector<uint64_t> &item_rows, KernelTimer *tm) {
+                  std::vector<uint64_t> &item_rows) {
         std::vector<uint64_t> start{0}, row;
         std::vector<uint32_t> src;
         item_rows.assign(items, 0);
@@ -1462,8 +1571,33 @@ private:
                 start.push_back(start.back() + len);
                 item_rows[i] += len;
             }
+        return stage_segments(runs, start, row, src, keep);
+    }
+
+    // Rows [start[j], start[j + 1]) of the staged relation are rows row[j] ..
+    // of run src[j]. One pinned upload of the lists, one gather launch.
+    DevRel stage_segments(std::vector<std::unique_ptr<BepRun>> &runs, const std::vector<uint64_t> &seg_start,
+                          const std::vector<uint64_t> &seg_row, const std::vector<uint32_t> &seg_src, Held &keep) {
+        const uint64_t n_seg = seg_src.size();
+        // one upload: start (n_seg + 1) | row (n_seg) as u64, src (n_seg) as u32, then the sources
+        const uint64_t bytes_u64 = (2 * n_seg + 1) * 8, bytes_src = (n_seg * 4 + 15) & ~uint64_t{15};
+        const uint64_t bytes = bytes_u64 + bytes_src + runs.size() * sizeof(GatherSource);
+        unsigned char *h = static_cast<unsigned char *>(upload_.bytes(bytes));
+        uint64_t *start = reinterpret_cast<uint64_t *>(h);
+        uint64_t *row = start + n_seg + 1;
+        uint32_t *src = reinterpret_cast<uint32_t *>(h + bytes_u64);
+        GatherSource *srcs = reinterpret_cast<GatherSource *>(h + bytes_u64 + bytes_src);
+        std::memcpy(start, seg_start.data(), (n_seg + 1) * 8);
+        std::memcpy(row, seg_row.data(), n_seg * 8);
+        std::memcpy(src, seg_src.data(), n_seg * 4);
+        for (size_t k = 0; k < runs.size(); ++k) {
+            srcs[k] = GatherSource{};
+            srcs[k].keys = runs[k]->rel.keys;
+            for (uint32_t c = 0; c < S_.cols.n; ++c)
+                srcs[k].cols[c] = runs[k]->rel.cols.ptr[c];
+        }
         DevRel d;
-        d.rows = start.back();
+        d.rows = start[n_seg];
         d.cols.n = S_.cols.n;
         uint64_t *keys = ctx_->alloc<uint64_t>(std::max<uint64_t>(d.rows, 1));
         keep.push_back(keys);
@@ -1472,31 +1606,22 @@ private:
         d.cols = as_in(cols);
         if (d.rows == 0)
             return d;
-        Scratch scr(ctx_);
-        const uint32_t n_seg = static_cast<uint32_t>(src.size());
-        uint64_t *dstart = scr.get<uint64_t>(n_seg + 1);
-        uint64_t *drow = scr.get<uint64_t>(n_seg);
-        uint32_t *dsrc = scr.get<uint32_t>(n_seg);
-        std::vector<GatherSource> h_srcs(runs.size());
-        for (size_t k = 0; k < runs.size(); ++k) {
-            h_srcs[k].keys = runs[k]->rel.keys;
-            for (uint32_t c = 0; c < S_.cols.n; ++c)
-                h_srcs[k].cols[c] = runs[k]->rel.cols.ptr[c];
-        }
-        GatherSource *dsrcs = scr.get<GatherSource>(h_srcs.size());
-        HIPCHK(hipMemcpyAsync(dstart, start.data(), start.size() * 8, hipMemcpyHostToDevice, ctx_->stream));
-        HIPCHK(hipMemcpyAsync(drow, row.data(), row.size() * 8, hipMemcpyHostToDevice, ctx_->stream));
-        HIPCHK(hipMemcpyAsync(dsrc, src.data(), src.size() * 4, hipMemcpyHostToDevice, ctx_->stream));
-        HIPCHK(hipMemcpyAsync(dsrcs, h_srcs.data(), h_srcs.size() * sizeof(GatherSource), hipMemcpyHostToDevice,
-                              ctx_->stream));
-        gather_segments(ctx_->stream, dstart, n_seg, dsrc, drow, dsrcs, d.rows, keys, cols, tm);
-        ctx_->sync(); // the segment lists are freed with `scr`
+        Scratch scr(ctx_); // freed blocks are reused in stream order only
+        unsigned char *dev = scr.get<unsigned char>(bytes);
+        HIPCHK(hipMemcpyAsync(dev, h, bytes, hipMemcpyHostToDevice, ctx_->stream));
+        gather_segments(ctx_->stream, reinterpret_cast<const uint64_t *>(dev), static_cast<uint32_t>(n_seg),
+                        reinterpret_cast<const uint32_t *>(dev + bytes_u64),
+                        reinterpret_cast<const uint64_t *>(dev) + n_seg + 1,
+                        reinterpret_cast<const GatherSource *>(dev + bytes_u64 + bytes_src),
+                        static_cast<uint32_t>(runs.size()), d.rows, keys, cols, ctx_->tm());
         return d;
     }
 
     // src/PHJBep.cpp:397-444 refinePartition, for every bucket of Q at once:
     // the buckets' rows from every batch run, through the remaining passes
-    // (refine_to_leaves mode), into one new leaf run.
+    // (refine_to_leaves mode), into one new leaf run. The host waits for the
+    // new run's leaf offsets (the eviction decisions need them); everything
+    // queued before is then done, so the upload memory is recycled.
     void refine(const std::vector<uint32_t> &Q) {
         if (Q.empty())
             return;
@@ -1512,14 +1637,15 @@ private:
                     }
             evict(E);
         }
-        HIPCHK(hipEventRecord(ev_.a, ctx_->stream));
+        EventPair &ev = timed(shuffle_ev_);
         auto leaf = std::make_unique<BepRun>(ctx_);
         Held staging(ctx_);
         std::vector<uint64_t> q_rows;
         const DevRel st = gather(
             batch_runs_, static_cast<uint32_t>(Q.size()), [&](const BepRun &, uint32_t i) { return int64_t(Q[i]); },
-            staging, q_rows, ctx_->tm());
+            staging, q_rows);
         const uint32_t nq = static_cast<uint32_t>(Q.size());
+        const size_t n_off = rest_.empty() ? nq + 1 : static_cast<size_t>(nq) * lpp_ + 1;
         if (rest_.empty()) {
             // one pass in all: the buckets are the leaves (src/RadixPartition.cpp:437-447)
             leaf->rel = st;
@@ -1529,24 +1655,27 @@ private:
             leaf->off.assign(nq + 1, 0);
             for (uint32_t i = 0; i < nq; ++i)
                 leaf->off[i + 1] = leaf->off[i] + static_cast<uint32_t>(q_rows[i]);
+            HIPCHK(hipEventRecord(ev.b, ctx_->stream));
+            ctx_->sync();
         } else {
-            std::vector<uint32_t> seg(nq + 1, 0);
+            uint32_t *seg = upload_.get<uint32_t>(nq + 1);
+            seg[0] = 0;
             for (uint32_t i = 0; i < nq; ++i)
                 seg[i + 1] = seg[i] + static_cast<uint32_t>(q_rows[i]);
             Scratch scr(ctx_);
             uint32_t *dseg = scr.get<uint32_t>(nq + 1);
-            HIPCHK(hipMemcpyAsync(dseg, seg.data(), (nq + 1) * 4ull, hipMemcpyHostToDevice, ctx_->stream));
+            HIPCHK(hipMemcpyAsync(dseg, seg, (nq + 1) * 4ull, hipMemcpyHostToDevice, ctx_->stream));
             const Partitioned LP = radix_partition(ctx_, st, dseg, nq, p1_, rest_, leaf->bufs, "refine", flags_);
             staging.free_all();
             leaf->rel.rows = st.rows;
             leaf->rel.keys = LP.keys;
             leaf->rel.cols = as_in(LP.cols);
-            leaf->off.resize(static_cast<size_t>(nq) * lpp_ + 1);
-            HIPCHK(hipMemcpyAsync(leaf->off.data(), LP.off, leaf->off.size() * 4, hipMemcpyDeviceToHost, ctx_->stream));
+            HIPCHK(hipEventRecord(ev.b, ctx_->stream));
+            HIPCHK(hipMemcpyAsync(leaf_off_, LP.off, n_off * 4, hipMemcpyDeviceToHost, ctx_->stream));
+            ctx_->sync();
+            leaf->off.assign(leaf_off_, leaf_off_ + n_off);
         }
-        HIPCHK(hipEventRecord(ev_.b, ctx_->stream));
-        ctx_->sync();
-        shuffle_ms_ += ev_.ms();
+        upload_.reset();
         leaf->done.assign(static_cast<size_t>(nq) * lpp_, 0);
         leaf->qidx.assign(P_, -1);
         for (uint32_t i = 0; i < nq; ++i) {
@@ -1589,31 +1718,141 @@ private:
             E.push_back(L);
             left -= leaf_rows_[L];
         }
+        // cursor order is ascending but for one wrap past the last leaf
+        for (size_t i = 1; i < E.size(); ++i)
+            if (E[i] < E[i - 1]) {
+                std::rotate(E.begin(), E.begin() + static_cast<std::ptrdiff_t>(i), E.end());
+                break;
+            }
         overhead_ms_ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        std::sort(E.begin(), E.end());
         evict(E);
     }
 
     // src/PHJBep.cpp:453-505 tryEvictOneLeaf, for every leaf of E at once: the
     // leaves' buffered rows from every leaf run, joined against their tables,
-    // then dropped.
+    // then dropped. Against tables built up front (single-row or group
+    // images) the join reserves its rows on a device cursor and nothing
+    // waits; otherwise the leaves' work items build LDS tables (LeafJoin).
     void evict(const std::vector<uint32_t> &E) {
         if (E.empty())
             return;
         Held staging(ctx_);
-        std::vector<uint64_t> l_rows;
-        HIPCHK(hipEventRecord(ev_.a, ctx_->stream));
-        const DevRel st = gather(
-            leaf_runs_, static_cast<uint32_t>(E.size()),
-            [&](const BepRun &r, uint32_t i) {
-                const int32_t qi = r.qidx[E[i] / lpp_];
-                return qi < 0 ? int64_t(-1) : int64_t(qi) * lpp_ + E[i] % lpp_;
-            },
-            staging, l_rows, ctx_->tm());
-        HIPCHK(hipEventRecord(ev_.b, ctx_->stream));
-        ctx_->sync();
-        overhead_ms_ += ev_.ms();
-        // the staged rows as a partitioned relation over all leaves
+        EventPair &gev = timed(gather_ev_);
+        if (tab_.kind == BepTables::kLds) {
+            std::vector<uint64_t> l_rows;
+            const DevRel st = gather(
+                leaf_runs_, static_cast<uint32_t>(E.size()),
+                [&](const BepRun &r, uint32_t i) {
+                    const int32_t qi = r.qidx[E[i] / lpp_];
+                    return qi < 0 ? int64_t(-1) : int64_t(qi) * lpp_ + E[i] % lpp_;
+                },
+                staging, l_rows);
+            HIPCHK(hipEventRecord(gev.b, ctx_->stream));
+            evict_lds(E, st, l_rows);
+            consume(E);
+        } else {
+            const DevRel st = gather_evicted(E, staging);
+            HIPCHK(hipEventRecord(gev.b, ctx_->stream));
+            join_images(st);
+        }
+        for (const uint32_t L : E) {
+            global_leaf_ -= leaf_rows_[L];
+            leaf_rows_[L] = 0;
+        }
+        evictions_ += E.size();
+        leaf_runs_.erase(std::remove_if(leaf_runs_.begin(), leaf_runs_.end(),
+                                        [](const std::unique_ptr<BepRun> &r) { return r->live == 0; }),
+                         leaf_runs_.end());
+    }
+
+    // Marks the groups of the leaves of E consumed in every leaf run.
+    void consume(const std::vector<uint32_t> &E) {
+        for (const uint32_t L : E)
+            for (auto &r : leaf_runs_) {
+                const int32_t qi = r->qidx[L / lpp_];
+                if (qi < 0)
+                    continue;
+                const uint32_t g = static_cast<uint32_t>(qi) * lpp_ + L % lpp_;
+                if (!r->done[g]) {
+                    r->done[g] = 1;
+                    r->live -= r->len(g);
+                }
+            }
+    }
+
+    // The buffered rows of the leaves of E (ascending) from every leaf run,
+    // consumed as they are listed. The join against tables built up front
+    // takes rows in any order, so each run's groups of one interval of
+    // consecutive leaves (adjacent in the run) merge into few segments.
+    DevRel gather_evicted(const std::vector<uint32_t> &E, Held &keep) {
+        std::vector<uint64_t> start{0}, row;
+        std::vector<uint32_t> src;
+        for (size_t i = 0; i < E.size();) {
+            size_t j = i;
+            while (j + 1 < E.size() && E[j + 1] == E[j] + 1)
+                ++j;
+            const uint32_t a = E[i], b = E[j];
+            for (uint32_t k = 0; k < leaf_runs_.size(); ++k) {
+                BepRun &r = *leaf_runs_[k];
+                for (uint32_t q = a / lpp_; q <= b / lpp_; ++q) {
+                    const int32_t qi = r.qidx[q];
+                    if (qi < 0)
+                        continue;
+                    const uint32_t d0 = q == a / lpp_ ? a % lpp_ : 0, d1 = q == b / lpp_ ? b % lpp_ : lpp_ - 1;
+                    for (uint32_t d = d0; d <= d1; ++d) {
+                        const uint32_t g = static_cast<uint32_t>(qi) * lpp_ + d;
+                        if (r.done[g])
+                            continue;
+                        r.done[g] = 1;
+                        const uint32_t len = r.len(g);
+                        if (len == 0)
+                            continue;
+                        r.live -= len;
+                        const size_t n = src.size();
+                        if (n > 0 && src[n - 1] == k && row[n - 1] + (start[n] - start[n - 1]) == r.off[g]) {
+                            start[n] += len;
+                        } else {
+                            src.push_back(k);
+                            row.push_back(r.off[g]);
+                            start.push_back(start[n] + len);
+                        }
+                    }
+                }
+            }
+            i = j + 1;
+        }
+        return stage_segments(leaf_runs_, start, row, src, keep);
+    }
+
+    // Rows of the leaves of E, staged by gather_evicted, joined against the
+    // tables built up front; the join reserves its rows on the device cursor.
+    void join_images(const DevRel &st) {
+        if (tab_.kind == BepTables::kUnique) {
+            EventPair &ev = timed(probe_ev_);
+            LeafJoinArgs a{};
+            a.r_keys = RP_.keys;
+            a.r_cols = as_in(RP_.cols);
+            a.r_off = RP_.off;
+            a.s_keys = st.keys;
+            a.s_cols = st.cols;
+            a.n_leaves = leaves_;
+            a.load_pct = plan_.load_pct;
+            a.table_img = tab_.unique.img;
+            a.img_stride = tab_.unique.stride;
+            a.img_side = tab_.unique.side;
+            a.out = sink_of(out_, dcursor_);
+            prebuilt_join(ctx_->stream, a, st.rows, tab_.leaf_bits, ctx_->tm());
+            HIPCHK(hipEventRecord(ev.b, ctx_->stream));
+        } else {
+            EventPair &ev = timed(probe_ev_);
+            group_join(ctx_->stream, tab_.group, st.keys, st.cols, st.rows, sink_of(out_, dcursor_), ctx_->tm());
+            HIPCHK(hipEventRecord(ev.b, ctx_->stream));
+        }
+    }
+
+    // The staged rows as a partitioned relation over all leaves, joined leaf
+    // by leaf with LDS tables built by each work item.
+    void evict_lds(const std::vector<uint32_t> &E, const DevRel &st, const std::vector<uint64_t> &l_rows) {
         std::vector<uint32_t> off(static_cast<size_t>(leaves_) + 1, 0);
         for (size_t i = 0, L = 0; L < leaves_; ++L) {
             off[L + 1] = off[L];
@@ -1633,11 +1872,12 @@ private:
         }
         SP.off = doff;
         SP.parts = leaves_;
-        HIPCHK(hipEventRecord(ev_.a, ctx_->stream));
+        EventPair ev;
+        HIPCHK(hipEventRecord(ev.a, ctx_->stream));
         {
             LeafJoinPlan plan = plan_;
             plan.two_phase = plan.two_phase || repeated_keys_;
-            LeafJoin lj(ctx_, RP_, SP, plan, repeated_keys_ ? nullptr : img_);
+            LeafJoin lj(ctx_, RP_, SP, plan, nullptr);
             uint64_t add;
             while (true) {
                 if (n_out_ + lj.bound() > out_.cap)
@@ -1651,28 +1891,9 @@ private:
             big_build_ms_ += lj.big_build_ms();
             probe_ms_ -= lj.big_build_ms();
         }
-        HIPCHK(hipEventRecord(ev_.b, ctx_->stream));
+        HIPCHK(hipEventRecord(ev.b, ctx_->stream));
         ctx_->sync();
-        probe_ms_ += ev_.ms();
-        for (size_t i = 0; i < E.size(); ++i) {
-            const uint32_t L = E[i];
-            for (auto &r : leaf_runs_) {
-                const int32_t qi = r->qidx[L / lpp_];
-                if (qi < 0)
-                    continue;
-                const uint32_t g = static_cast<uint32_t>(qi) * lpp_ + L % lpp_;
-                if (!r->done[g]) {
-                    r->done[g] = 1;
-                    r->live -= r->len(g);
-                }
-            }
-            global_leaf_ -= leaf_rows_[L];
-            leaf_rows_[L] = 0;
-        }
-        evictions_ += E.size();
-        leaf_runs_.erase(std::remove_if(leaf_runs_.begin(), leaf_runs_.end(),
-                                        [](const std::unique_ptr<BepRun> &r) { return r->live == 0; }),
-                         leaf_runs_.end());
+        probe_ms_ += ev.ms();
     }
 
     phj_ctx *ctx_;
@@ -1682,18 +1903,24 @@ private:
     std::vector<uint32_t> rest_;
     LeafJoinPlan plan_;
     uint32_t flags_;
-    const LeafTableImages *img_;
+    const BepTables &tab_;
     OutBuffers &out_;
+    Held keep_;
+    PinnedArena upload_;
+    uint32_t *scat_off_[2] = {nullptr, nullptr};
+    uint32_t *leaf_off_ = nullptr;
+    unsigned long long *cursor_host_ = nullptr;
+    unsigned long long *dcursor_ = nullptr;
     uint32_t P_ = 0, leaves_ = 0, lpp_ = 0;
     uint64_t rb_ = 0, u_hi_ = 0, u_lo_ = 0, l_hi_ = 0, l_lo_ = 0, batch_ = 0;
     std::vector<uint64_t> bucket_rows_, leaf_rows_;
     uint64_t unrefined_ = 0, global_leaf_ = 0;
     uint32_t cursor_ = 0;
     std::vector<std::unique_ptr<BepRun>> batch_runs_, leaf_runs_;
+    std::vector<std::unique_ptr<EventPair>> shuffle_ev_, probe_ev_, gather_ev_;
     bool repeated_keys_ = false;
     uint64_t n_out_ = 0, evictions_ = 0, refinements_ = 0, peak_ = 0;
     double shuffle_ms_ = 0, probe_ms_ = 0, overhead_ms_ = 0, big_build_ms_ = 0;
-    EventPair ev_;
 };
 
 void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, const uint8_t *pb, uint32_t np,
@@ -1722,31 +1949,58 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     Held keep(ctx);
     const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
-    // Eager leaf tables (src/PHJBep.cpp:157-231): built once in HBM when every
-    // leaf fits LDS, no key repeats and an eviction round holds few probe rows
-    // per leaf; otherwise each round's work items build theirs in LDS.
+    // Eager leaf tables (src/PHJBep.cpp:157-231), built once in HBM: single-
+    // row cells when every leaf fits LDS, no key repeats and an eviction round
+    // holds few probe rows per leaf; group cells when keys repeat. Otherwise
+    // each round's work items build theirs in LDS.
     double build_ms = 0;
-    LeafTableImages images;
+    BepTables tables;
+    tables.leaf_bits = total;
     const uint64_t round_rows = std::max<uint64_t>(budget_bytes * 5 / 8 / row_bytes * 3 / 8, 1);
-    const bool use_images = build_table_images(ctx, RP, plan, round_rows, keep, images, build_ms);
-    OutBuffers out = alloc_output(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1));
-    BepSchedule sched(ctx, RP, S, p1, rest, budget_bytes, plan, radix_flags(tun), use_images ? &images : nullptr, out);
-    sched.run();
-    HIPCHK(hipEventRecord(e2e.b, ctx->stream));
-    ctx->sync();
+    uint64_t out_cap = std::max<uint64_t>(S.rows, 1);
+    if (build_table_images(ctx, RP, plan, round_rows, keep, tables.unique, build_ms)) {
+        tables.kind = BepTables::kUnique;
+    } else if (RP.rows > 0 && S.rows > 0) {
+        uint32_t max_leaf = static_cast<uint32_t>(std::max<int64_t>(RP.max_part, 0));
+        if (RP.max_part < 0) {
+            Scratch s2(ctx);
+            uint32_t *dmax = s2.get<uint32_t>(1);
+            leaf_max_rows(ctx->stream, RP.off, static_cast<uint32_t>(RP.parts), dmax);
+            max_leaf = read_scalar(ctx, dmax);
+        }
+        if (build_group_images(ctx, RP, plan, total, max_leaf, keep, tables.group, build_ms)) {
+            tables.kind = BepTables::kGroup;
+            // the output's exact size: every probe row's group length, summed
+            Scratch s2(ctx);
+            unsigned long long *cnt = s2.get<unsigned long long>(1);
+            launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(cnt), 1, 0);
+            group_count(ctx->stream, tables.group, S.keys, S.rows, cnt, ctx->tm());
+            out_cap = std::max<uint64_t>(read_scalar(ctx, cnt), 1);
+        }
+    }
+    OutBuffers out = alloc_output(ctx, R.cols, S.cols, out_cap);
+    uint64_t n_out = 0;
+    {
+        BepSchedule sched(ctx, RP, S, p1, rest, budget_bytes, plan, radix_flags(tun), tables, out);
+        sched.run();
+        HIPCHK(hipEventRecord(e2e.b, ctx->stream));
+        ctx->sync();
+        set_phase(res->build, build_ms + sched.big_build_ms(), R.rows);
+        set_phase(res->probe_shuffle, sched.shuffle_ms(), S.rows);
+        set_phase(res->probe, sched.probe_ms(), S.rows);
+        set_phase(res->eviction_overhead, sched.overhead_ms(), S.rows);
+        res->bep_evictions = sched.evictions();
+        res->bep_refinements = sched.refinements();
+        res->bep_peak_bytes = sched.peak_bytes();
+        n_out = sched.out_rows();
+    }
+    ctx->pinned_io.reset();
     keep.free_all();
     set_phase(res->build_shuffle, ebs.ms(), R.rows);
-    set_phase(res->build, build_ms + sched.big_build_ms(), R.rows);
-    set_phase(res->probe_shuffle, sched.shuffle_ms(), S.rows);
-    set_phase(res->probe, sched.probe_ms(), S.rows);
-    set_phase(res->eviction_overhead, sched.overhead_ms(), S.rows);
     res->e2e_wall_ms = e2e.ms();
     res->bep_budget_mib = budget_bytes >> 20;
-    res->bep_evictions = sched.evictions();
-    res->bep_refinements = sched.refinements();
-    res->bep_peak_bytes = sched.peak_bytes();
     res->bep_build_skip_retries = 0;
-    ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, sched.out_rows());
+    ResultBuilder{ctx, res, R.rows, S.rows}.finish(out, n_out);
 }
 
 // ---- Batched PHJ-PURE joins ----
@@ -1949,12 +2203,16 @@ DevRel group_side(phj_ctx *ctx, BatchGroup &g, int which, bool in_place, uint64_
     uint32_t *seg_src = ctx->pinned_io.get<uint32_t>(g.m);
     GatherSource *srcs = ctx->pinned_io.get<GatherSource>(g.m);
     seg_start[0] = 0;
+    uint32_t n_seg = 0; // non-empty relations only
     for (uint32_t j = 0; j < g.m; ++j) {
         const phj_relation &r = which ? g.d[j].probe : g.d[j].build;
-        seg_start[j + 1] = seg_start[j] + r.rows;
-        seg_row[j] = 0;
-        seg_src[j] = j;
         srcs[j] = source_of(r);
+        if (r.rows == 0)
+            continue;
+        seg_start[n_seg + 1] = seg_start[n_seg] + r.rows;
+        seg_row[n_seg] = 0;
+        seg_src[n_seg] = j;
+        ++n_seg;
     }
     Scratch scr(ctx);
     uint64_t *dstart = scr.get<uint64_t>(g.m + 1);
@@ -1968,7 +2226,7 @@ DevRel group_side(phj_ctx *ctx, BatchGroup &g, int which, bool in_place, uint64_
     uint64_t *keys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
     g.keep.push_back(keys);
     OutColSet cols = alloc_cols(ctx, v.cols, rows, g.keep);
-    gather_segments(ctx->stream, dstart, g.m, dsrc, drow, dsrcs, rows, keys, cols, ctx->tm());
+    gather_segments(ctx->stream, dstart, n_seg, dsrc, drow, dsrcs, g.m, rows, keys, cols, ctx->tm());
     v.keys = keys;
     v.cols = as_in(cols);
     return v;

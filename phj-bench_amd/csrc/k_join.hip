@@ -678,13 +678,23 @@ __device__ __forceinline__ uint32_t meta_row(uint32_t m) { return m & 0xFFFFu; }
 // a store that may alias it). PROBE: source rows are the slots' probe rows
 // (slot k of the lane is row k * THREADS + threadIdx.x of the round); else the
 // matched build rows. Slots without a row load row 0 and store nothing.
-template <int THREADS, int K, bool PROBE, class T, int G>
-__device__ __forceinline__ void sweep_load(const void *src, const uint32_t (&meta)[K], uint32_t tid, int h, T (&v)[G]) {
+// Where a slot's source row comes from: the lane's probe row of the round
+// (slot k is row k * THREADS + threadIdx.x), the build row in the metadata,
+// or a separate array of (64-bit indexed) rows.
+enum SweepRows { kRowProbe, kRowMeta, kRowArray };
+
+template <int THREADS, int K, int MODE, class T, int G>
+__device__ __forceinline__ void sweep_load(const void *src, const uint32_t (&meta)[K], const uint32_t (&rows)[K],
+                                           uint32_t tid, int h, T (&v)[G]) {
 #pragma unroll
     for (int u = 0; u < G; ++u) {
         const int k = h * G + u;
-        const uint32_t row = PROBE ? static_cast<uint32_t>(k * THREADS) + tid : meta_row(meta[k]);
-        v[u] = ld_at<T>(src, (meta_on(meta[k]) ? row : 0u) * static_cast<uint32_t>(sizeof(T)));
+        if constexpr (MODE == kRowArray) {
+            v[u] = static_cast<const T *>(src)[meta_on(meta[k]) ? rows[k] : 0u];
+        } else {
+            const uint32_t row = MODE == kRowProbe ? static_cast<uint32_t>(k * THREADS) + tid : meta_row(meta[k]);
+            v[u] = ld_at<T>(src, (meta_on(meta[k]) ? row : 0u) * static_cast<uint32_t>(sizeof(T)));
+        }
     }
 }
 template <int K, class T, int G>
@@ -696,18 +706,19 @@ __device__ __forceinline__ void sweep_store(void *dst, const uint32_t (&meta)[K]
             st_at<T>(dst, meta_pos(meta[k]) * static_cast<uint32_t>(sizeof(T)), v[u]);
     }
 }
-template <int THREADS, int K, bool PROBE, class T>
-__device__ __forceinline__ void sweep_col(const void *src, void *dst, const uint32_t (&meta)[K], uint32_t tid) {
+template <int THREADS, int K, int MODE, class T>
+__device__ __forceinline__ void sweep_col(const void *src, void *dst, const uint32_t (&meta)[K],
+                                          const uint32_t (&rows)[K], uint32_t tid) {
     constexpr int G = sizeof(T) >= 8 ? static_cast<int>(32 / sizeof(T)) : 8; // <= 8 VGPRs of values per group
     static_assert(K % G == 0, "slots in groups");
     // group h + 1's loads are issued before group h's stores
     T cur[G];
-    sweep_load<THREADS, K, PROBE, T, G>(src, meta, tid, 0, cur);
+    sweep_load<THREADS, K, MODE, T, G>(src, meta, rows, tid, 0, cur);
 #pragma unroll
     for (int h = 0; h < K / G; ++h) {
         T nxt[G];
         if (h + 1 < K / G)
-            sweep_load<THREADS, K, PROBE, T, G>(src, meta, tid, h + 1, nxt);
+            sweep_load<THREADS, K, MODE, T, G>(src, meta, rows, tid, h + 1, nxt);
         sweep_store<K, T, G>(dst, meta, h, cur);
         if (h + 1 < K / G) {
 #pragma unroll
@@ -718,9 +729,9 @@ __device__ __forceinline__ void sweep_col(const void *src, void *dst, const uint
 }
 
 // Every column of a set (source rows from src_row0, output rows from dst_row0).
-template <int THREADS, int K, bool PROBE>
+template <int THREADS, int K, int MODE>
 __device__ __forceinline__ void sweep_set(const ColSet &in, uint64_t src_row0, const OutColSet &out, uint64_t dst_row0,
-                                          const uint32_t (&meta)[K]) {
+                                          const uint32_t (&meta)[K], const uint32_t (&rows)[K]) {
     for (uint32_t c = 0; c < in.n; ++c) {
         const uint32_t t = col_type(in, c);
         const uint64_t sz = type_size(t);
@@ -729,20 +740,23 @@ __device__ __forceinline__ void sweep_set(const ColSet &in, uint64_t src_row0, c
         // An opaque copy per column: offsets derived from the metadata are
         // then computed inside the column loop rather than hoisted out of it
         // for every column type at once (which spills).
-        uint32_t m[K];
+        uint32_t m[K], r[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             m[k] = meta[k];
             asm volatile("" : "+v"(m[k]));
+            r[k] = MODE == kRowArray ? rows[k] : 0u;
+            if constexpr (MODE == kRowArray)
+                asm volatile("" : "+v"(r[k]));
         }
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         switch (t) {
-        case 0: sweep_col<THREADS, K, PROBE, uint8_t>(src, dst, m, tid); break;
-        case 1: sweep_col<THREADS, K, PROBE, uint16_t>(src, dst, m, tid); break;
-        case 2: sweep_col<THREADS, K, PROBE, uint32_t>(src, dst, m, tid); break;
-        case 3: sweep_col<THREADS, K, PROBE, uint64_t>(src, dst, m, tid); break;
-        default: sweep_col<THREADS, K, PROBE, ulonglong2>(src, dst, m, tid); break;
+        case 0: sweep_col<THREADS, K, MODE, uint8_t>(src, dst, m, r, tid); break;
+        case 1: sweep_col<THREADS, K, MODE, uint16_t>(src, dst, m, r, tid); break;
+        case 2: sweep_col<THREADS, K, MODE, uint32_t>(src, dst, m, r, tid); break;
+        case 3: sweep_col<THREADS, K, MODE, uint64_t>(src, dst, m, r, tid); break;
+        default: sweep_col<THREADS, K, MODE, ulonglong2>(src, dst, m, r, tid); break;
         }
     }
 }
@@ -1043,9 +1057,10 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                     const uint32_t row = c == 0 ? 0u : (c - 1 == kSideSlot ? side_head : trows[c - 1]);
                     meta[k] = on ? (0x80000000u | (opos << 16) | row) : 0u;
                 }
-                sweep_col<THREADS, K, true, uint64_t>(a.s_keys + sb, a.out.keys + rb, meta, threadIdx.x);
-                sweep_set<THREADS, K, false>(a.r_cols, it.r0, a.out.left, rb, meta);
-                sweep_set<THREADS, K, true>(a.s_cols, sb, a.out.right, rb, meta);
+                const uint32_t none[K] = {};
+                sweep_col<THREADS, K, kRowProbe, uint64_t>(a.s_keys + sb, a.out.keys + rb, meta, none, threadIdx.x);
+                sweep_set<THREADS, K, kRowMeta>(a.r_cols, it.r0, a.out.left, rb, meta, none);
+                sweep_set<THREADS, K, kRowProbe>(a.s_cols, sb, a.out.right, rb, meta, none);
             }
         }
         __syncthreads(); // slot_off and sr_* are rewritten by the next super-round
@@ -1098,10 +1113,11 @@ __global__ __launch_bounds__(THREADS) void k_leaf_table_image(LeafJoinArgs a, un
 // lookup table is needed; the rows of a leaf are adjacent, so neighbouring
 // lanes read the same table. Lookups advance LB keys in lock step; the work
 // item's output rows are reserved with one atomic add and laid out slot-major
-// as in the fused join; then rows are written in rounds of U slots.
-template <int THREADS, int K, int U>
+// as in the fused join; then every column is written in pipelined sweeps over
+// the lane's slots, as in the fused join.
+template <int THREADS, int K>
 __global__ __launch_bounds__(THREADS) void k_prebuilt_join(LeafJoinArgs a, uint64_t n_rows, uint32_t leaf_bits) {
-    static_assert(K % U == 0 && K <= 64, "K slots per lane in rounds of U");
+    static_assert(K <= 64 && THREADS * K <= 0x8000, "K slots per lane; output rows of an item fit the metadata");
     constexpr int W = THREADS / 64;
     constexpr int LB = 4;
     static_assert(K % LB == 0, "lookup batches");
@@ -1192,36 +1208,20 @@ __global__ __launch_bounds__(THREADS) void k_prebuilt_join(LeafJoinArgs a, uint6
     if (sr_total == 0)
         return; // uniform
     const uint64_t rb = sr_base;
+    // slot k: output row (relative to rb) and matched build row
+    uint32_t meta[K], brow[K];
 #pragma unroll
-    for (int r = 0; r < K / U; ++r) {
-        if (sb + static_cast<uint64_t>(r) * U * THREADS >= s1)
-            break; // uniform
-        uint32_t ridx[U], sidx[U];
-        uint64_t opos[U];
-        bool on[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = r * U + u;
-            const uint64_t m = __ballot(ref[k] != 0);
-            opos[u] = rb + slot_off[k][wave] + mbcnt(m);
-            on[u] = ref[k] != 0 && opos[u] < a.out.cap;
-            ridx[u] = on[u] ? ref[k] - 1 : 0u;
-            sidx[u] = static_cast<uint32_t>(sb) + static_cast<uint32_t>(k * THREADS) + threadIdx.x;
-        }
-        {
-            // the output key is the probe key (equal to the matched build key)
-            uint64_t kv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                kv[u] = a.s_keys[on[u] ? sidx[u] : 0u];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (on[u])
-                    a.out.keys[opos[u]] = kv[u];
-        }
-        gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
-        gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
+    for (int k = 0; k < K; ++k) {
+        const uint64_t m = __ballot(ref[k] != 0);
+        const uint32_t o = slot_off[k][wave] + mbcnt(m);
+        const bool on = ref[k] != 0 && rb + o < a.out.cap;
+        meta[k] = on ? (0x80000000u | (o << 16)) : 0u;
+        brow[k] = on ? ref[k] - 1 : 0u;
     }
+    // the output key is the probe key (equal to the matched build key)
+    sweep_col<THREADS, K, kRowProbe, uint64_t>(a.s_keys + sb, a.out.keys + rb, meta, brow, threadIdx.x);
+    sweep_set<THREADS, K, kRowArray>(a.r_cols, 0, a.out.left, rb, meta, brow);
+    sweep_set<THREADS, K, kRowProbe>(a.s_cols, sb, a.out.right, rb, meta, brow);
 }
 
 // ---------------------------------------------------------------- HBM tables
@@ -1591,7 +1591,8 @@ void prebuilt_join(hipStream_t s, const LeafJoinArgs &a, uint64_t probe_rows, ui
     if (probe_rows == 0)
         return;
     TimedScope ts(tm, "leaf_join", s);
-    k_prebuilt_join<T, K, 2><<<static_cast<uint32_t>((probe_rows + T * K - 1) / (T * K)), T, 0, s>>>(a, probe_rows, leaf_bits);
+    k_prebuilt_join<T, K><<<checked_grid((probe_rows + T * K - 1) / (T * K), T, "prebuilt_join"), T, 0, s>>>(a, probe_rows,
+                                                                                                          leaf_bits);
 }
 
 size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct) { return static_cast<size_t>(table_cap(rows, load_pct)) * sizeof(Cell16); }
@@ -1704,6 +1705,340 @@ void ht_find(hipStream_t s, const GlobalHT &ht, const uint64_t *keys, uint64_t n
     r.rows = n;
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 256 * 64));
     k_ht_find<<<grid, 256, 0, s>>>(make_args(ht, kHtSingle, r), keys, n, heads_rows);
+}
+
+// ------------------------------------------------------------ group images
+
+__host__ __device__ __forceinline__ size_t group_lds(uint32_t max_rows, uint32_t load_pct) {
+    const size_t cap = table_cap(max_rows, load_pct);
+    return cap * 8 + ((cap * 4 + 15) & ~size_t{15});
+}
+
+// One workgroup per leaf: counts the rows of each key in an LDS table, turns
+// the counts into group starts (exclusive scan in slot order), writes the
+// leaf's cells {key, start, count} and places every row in its key's group
+// (perm[start + rank] = row). Empty-key rows form the leaf's last group.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_group_image(const uint64_t *__restrict__ r_keys,
+                                                         const uint32_t *__restrict__ r_off, uint32_t max_rows,
+                                                         uint32_t load_pct, Cell16 *__restrict__ cells,
+                                                         uint64_t stride, uint32_t *__restrict__ side,
+                                                         uint32_t *__restrict__ perm) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t side_cnt, side_cur;
+    __shared__ uint32_t scan_tmp[THREADS / 64 + 1];
+    const uint32_t leaf = blockIdx.x;
+    const uint32_t r0 = r_off[leaf], nr = r_off[leaf + 1] - r0;
+    if (nr == 0 || nr > max_rows) { // uniform; the caller sizes max_rows to the largest leaf
+        if (threadIdx.x == 0) {
+            side[2 * leaf] = r0;
+            side[2 * leaf + 1] = 0;
+        }
+        return;
+    }
+    const uint32_t cap = table_cap(nr, load_pct);
+    uint64_t *tkeys = reinterpret_cast<uint64_t *>(smem);
+    uint32_t *tcnt = reinterpret_cast<uint32_t *>(smem + static_cast<size_t>(cap) * 8);
+    for (uint32_t i = threadIdx.x; i < cap; i += THREADS) {
+        tkeys[i] = kEmptyKey;
+        tcnt[i] = 0;
+    }
+    if (threadIdx.x == 0)
+        side_cnt = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nr; i += THREADS) {
+        const uint64_t k = r_keys[r0 + i];
+        if (k == kEmptyKey) {
+            atomicAdd(&side_cnt, 1u);
+            continue;
+        }
+        uint32_t pos = home_slot(k, cap);
+        while (true) {
+            const unsigned long long old =
+                atomicCAS(reinterpret_cast<unsigned long long *>(&tkeys[pos]),
+                          static_cast<unsigned long long>(kEmptyKey), static_cast<unsigned long long>(k));
+            if (old == kEmptyKey || old == k)
+                break;
+            pos = pos + 1 == cap ? 0u : pos + 1;
+        }
+        atomicAdd(&tcnt[pos], 1u);
+    }
+    __syncthreads();
+    // slots in THREADS chunks of consecutive slots
+    const uint32_t per = (cap + THREADS - 1) / THREADS;
+    const uint32_t c0 = min(cap, threadIdx.x * per), c1 = min(cap, c0 + per);
+    uint32_t sum = 0;
+    for (uint32_t i = c0; i < c1; ++i)
+        sum += tcnt[i];
+    uint32_t total;
+    uint32_t start = block_excl_scan<THREADS>(sum, scan_tmp, &total);
+    Cell16 *dst = cells + static_cast<uint64_t>(leaf) * stride;
+    for (uint32_t i = c0; i < c1; ++i) {
+        const uint32_t n = tcnt[i];
+        Cell16 c;
+        c.key = tkeys[i];
+        c.block_no = r0 + start;
+        c.row_no = n;
+        dst[i] = c;
+        tcnt[i] = start; // the slot's placement cursor from here on
+        start += n;
+    }
+    if (threadIdx.x == 0) {
+        side[2 * leaf] = r0 + total;
+        side[2 * leaf + 1] = side_cnt;
+        side_cur = total;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nr; i += THREADS) {
+        const uint64_t k = r_keys[r0 + i];
+        uint32_t at;
+        if (k == kEmptyKey) {
+            at = atomicAdd(&side_cur, 1u);
+        } else {
+            uint32_t pos = home_slot(k, cap);
+            while (tkeys[pos] != k)
+                pos = pos + 1 == cap ? 0u : pos + 1;
+            at = atomicAdd(&tcnt[pos], 1u);
+        }
+        perm[r0 + at] = r0 + i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_perm(ColSet in, OutColSet out, const uint32_t *__restrict__ perm,
+                                                     uint64_t rows) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < rows; i += stride) {
+        const uint32_t src = perm[i];
+        for (uint32_t c = 0; c < in.n; ++c) {
+            const uint32_t t = col_type(in, c);
+            uint64_t lo, hi;
+            load_elem(col_ptr(in, c), t, src, lo, hi);
+            store_elem(col_ptr(out, c), t, i, lo, hi);
+        }
+    }
+}
+
+// (start, count) of LB keys in the group images, looked up in lock step; a
+// key outside `valid`, a miss, or a leaf without rows gives count 0.
+template <int LB>
+__device__ __forceinline__ void group_find(const GroupImage &g, const uint64_t (&key)[LB], const bool (&valid)[LB],
+                                           uint32_t (&start)[LB], uint32_t (&count)[LB]) {
+    uint32_t pos[LB], cap[LB];
+    uint64_t base[LB];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+        const uint32_t leaf = g.leaf_bits ? static_cast<uint32_t>(int_hash64(key[q]) >> (64 - g.leaf_bits)) : 0u;
+        const uint32_t r0 = g.r_off[leaf];
+        const uint32_t nr = g.r_off[leaf + 1] - r0;
+        cap[q] = table_cap(nr, g.load_pct);
+        base[q] = static_cast<uint64_t>(leaf) * g.stride;
+        pos[q] = home_slot(key[q], cap[q]);
+        start[q] = 0;
+        count[q] = 0;
+        if (valid[q] && nr > 0) {
+            if (key[q] == kEmptyKey) {
+                start[q] = g.side[2 * leaf];
+                count[q] = g.side[2 * leaf + 1];
+            } else {
+                pend |= 1u << q;
+            }
+        }
+    }
+    while (pend) {
+        Cell16 c[LB];
+#pragma unroll
+        for (int q = 0; q < LB; ++q)
+            c[q] = g.cells[base[q] + pos[q]];
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            const bool p = (pend >> q) & 1u;
+            const bool hit = c[q].key == key[q];
+            const bool stop = hit || c[q].key == kEmptyKey;
+            start[q] = (p && hit) ? c[q].block_no : start[q];
+            count[q] = (p && hit) ? c[q].row_no : count[q];
+            pend &= ~((p && stop ? 1u : 0u) << q);
+            pos[q] = (p && !stop) ? (pos[q] + 1 == cap[q] ? 0u : pos[q] + 1) : pos[q];
+        }
+    }
+}
+
+template <int THREADS, int K>
+__global__ __launch_bounds__(THREADS) void k_group_count(GroupImage g, const uint64_t *__restrict__ keys, uint64_t rows,
+                                                         unsigned long long *__restrict__ total) {
+    constexpr int LB = 4;
+    static_assert(K % LB == 0, "lookup batches");
+    const uint64_t sb = static_cast<uint64_t>(blockIdx.x) * THREADS * K;
+    uint64_t sum = 0;
+#pragma unroll
+    for (int gq = 0; gq < K / LB; ++gq) {
+        uint64_t key[LB];
+        bool valid[LB];
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            const uint64_t j = sb + static_cast<uint64_t>((gq * LB + q) * THREADS + threadIdx.x);
+            valid[q] = j < rows;
+            key[q] = keys[min(j, rows - 1)];
+        }
+        uint32_t st[LB], cn[LB];
+        group_find<LB>(g, key, valid, st, cn);
+#pragma unroll
+        for (int q = 0; q < LB; ++q)
+            sum += cn[q];
+    }
+    sum = wave_sum_u64(sum);
+    if (lane_id() == 0 && sum != 0)
+        atomicAdd(total, static_cast<unsigned long long>(sum));
+}
+
+// Rows o of a work item's output, G per lane at a time: entry e (probe row
+// sb + e) and build row b of each; every column moves G rows with the loads
+// issued before the stores.
+template <int G>
+__device__ __forceinline__ void group_move_col(const void *src, uint32_t t, void *dst, uint64_t dst_row0,
+                                               const uint64_t (&srow)[G], const uint32_t (&opos)[G],
+                                               const bool (&on)[G]) {
+    uint64_t lo[G], hi[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+        load_elem(src, t, on[u] ? srow[u] : 0u, lo[u], hi[u]);
+    char *d = static_cast<char *>(dst) + dst_row0 * type_size(t);
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+        if (on[u])
+            store_elem(d, t, opos[u], lo[u], hi[u]);
+}
+
+// Join of a range of probe rows against the group images. A work item of
+// THREADS * K probe rows looks every row up (LB in lock step), scans the
+// rows' match counts in probe-row order in LDS, reserves its output rows with
+// one atomic add, then writes them in order: output row o comes from the
+// entry e whose range [e_off[e], e_off[e] + count) holds o (binary search),
+// build row e_start[e] + (o - e_off[e]) of the grouped columns.
+template <int THREADS, int K, int G>
+__global__ __launch_bounds__(THREADS) void k_group_join(GroupImage g, const uint64_t *__restrict__ keys, ColSet cols,
+                                                        uint64_t rows, OutSink out) {
+    constexpr int LB = 4;
+    constexpr uint32_t N = THREADS * K;
+    static_assert(K % LB == 0, "lookup batches");
+    __shared__ uint32_t e_off[N];
+    __shared__ uint32_t e_start[N];
+    __shared__ uint32_t scan_tmp[THREADS / 64 + 1];
+    __shared__ unsigned long long base_s;
+    const uint64_t sb = static_cast<uint64_t>(blockIdx.x) * N;
+    if (sb >= rows)
+        return; // uniform
+    const uint32_t n_here = static_cast<uint32_t>(min(static_cast<uint64_t>(N), rows - sb));
+#pragma unroll
+    for (int gq = 0; gq < K / LB; ++gq) {
+        uint64_t key[LB];
+        bool valid[LB];
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            const uint32_t e = static_cast<uint32_t>((gq * LB + q) * THREADS) + threadIdx.x;
+            valid[q] = e < n_here;
+            key[q] = keys[sb + min(e, n_here - 1)];
+        }
+        uint32_t st[LB], cn[LB];
+        group_find<LB>(g, key, valid, st, cn);
+#pragma unroll
+        for (int q = 0; q < LB; ++q) {
+            const uint32_t e = static_cast<uint32_t>((gq * LB + q) * THREADS) + threadIdx.x;
+            e_start[e] = st[q];
+            e_off[e] = cn[q];
+        }
+    }
+    __syncthreads();
+    uint32_t loc[K];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        loc[k] = e_off[threadIdx.x * K + k];
+        sum += loc[k];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan<THREADS>(sum, scan_tmp, &total);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        e_off[threadIdx.x * K + k] = off;
+        off += loc[k];
+    }
+    if (total == 0)
+        return; // uniform: the block total
+    if (threadIdx.x == 0)
+        base_s = atomicAdd(out.cursor, static_cast<unsigned long long>(total));
+    __syncthreads();
+    const uint64_t base = base_s;
+    for (uint32_t o0 = 0; o0 < total; o0 += THREADS * G) {
+        uint64_t erow[G], brow[G];
+        uint32_t opos[G];
+        bool on[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const uint32_t o = o0 + static_cast<uint32_t>(u * THREADS) + threadIdx.x;
+            on[u] = o < total && base + o < out.cap;
+            const uint32_t x = on[u] ? o : 0u;
+            const uint32_t e = upper_bound_idx(e_off, N, x);
+            erow[u] = sb + e;
+            brow[u] = static_cast<uint64_t>(e_start[e]) + (x - e_off[e]);
+            opos[u] = o;
+        }
+        {
+            uint64_t kv[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                kv[u] = keys[on[u] ? erow[u] : sb];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (on[u])
+                    out.keys[base + opos[u]] = kv[u];
+        }
+        for (uint32_t c = 0; c < g.cols.n; ++c)
+            group_move_col<G>(col_ptr(g.cols, c), col_type(g.cols, c), col_ptr(out.left, c), base, brow, opos, on);
+        for (uint32_t c = 0; c < cols.n; ++c)
+            group_move_col<G>(col_ptr(cols, c), col_type(cols, c), col_ptr(out.right, c), base, erow, opos, on);
+    }
+}
+
+size_t group_image_lds_bytes(uint32_t max_rows, uint32_t load_pct) { return group_lds(max_rows, load_pct); }
+
+void group_images(hipStream_t s, const uint64_t *r_keys, const uint32_t *r_off, uint32_t n_leaves, uint32_t max_rows,
+                  uint32_t load_pct, Cell16 *cells, uint64_t stride, uint32_t *side, uint32_t *perm, KernelTimer *tm) {
+    if (n_leaves == 0)
+        return;
+    const size_t lds = group_lds(max_rows, load_pct);
+    TimedScope ts(tm, "group_image", s);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_group_image<512>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    k_group_image<512><<<n_leaves, 512, lds, s>>>(r_keys, r_off, max_rows, load_pct, cells, stride, side, perm);
+}
+
+void gather_perm(hipStream_t s, const ColSet &in, const OutColSet &out, const uint32_t *perm, uint64_t rows,
+                 KernelTimer *tm) {
+    if (rows == 0 || in.n == 0)
+        return;
+    TimedScope ts(tm, "group_image", s);
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((rows + 255) / 256, 256 * 64));
+    k_gather_perm<<<grid, 256, 0, s>>>(in, out, perm, rows);
+}
+
+void group_count(hipStream_t s, const GroupImage &g, const uint64_t *keys, uint64_t rows, unsigned long long *total,
+                 KernelTimer *tm) {
+    constexpr int T = 256, K = 8;
+    if (rows == 0)
+        return;
+    TimedScope ts(tm, "group_count", s);
+    k_group_count<T, K><<<checked_grid((rows + T * K - 1) / (T * K), T, "group_count"), T, 0, s>>>(g, keys, rows, total);
+}
+
+void group_join(hipStream_t s, const GroupImage &g, const uint64_t *keys, const ColSet &cols, uint64_t rows,
+                const OutSink &out, KernelTimer *tm) {
+    constexpr int T = 256, K = 8, G = 4;
+    if (rows == 0)
+        return;
+    TimedScope ts(tm, "leaf_join", s);
+    k_group_join<T, K, G><<<checked_grid((rows + T * K - 1) / (T * K), T, "group_join"), T, 0, s>>>(g, keys, cols, rows,
+                                                                                                      out);
 }
 
 } // namespace phjd

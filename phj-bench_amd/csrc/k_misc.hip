@@ -40,26 +40,87 @@ template <class T> __global__ void k_fill(T *__restrict__ p, uint64_t n, T v) {
 
 // Rows gathered by a segment list: output rows [seg_start[g], seg_start[g+1])
 // are rows seg_row[g] .. of source relation seg_src[g] (all sources share one
-// schema). Each thread finds its row's segment by binary search; neighbouring
-// rows of a segment are neighbours in both the source and the output.
-__global__ __launch_bounds__(256) void k_gather_segments(const uint64_t *__restrict__ seg_start, uint32_t n_seg,
-                                                         const uint32_t *__restrict__ seg_src,
-                                                         const uint64_t *__restrict__ seg_row,
-                                                         const GatherSource *__restrict__ srcs, uint32_t n_cols,
-                                                         uint64_t *__restrict__ out_keys, OutColSet out) {
+// schema). A block takes kGatherChunk consecutive output rows: they span a
+// contiguous range of segments, whose starts the block stages in LDS (at most
+// one per row unless segments are empty; then it searches them in place), and
+// each thread finds its rows' segments by binary search there. Every column then moves
+// kGatherRpt rows per thread with all loads issued before the stores.
+constexpr uint32_t kGatherThreads = 256, kGatherRpt = 8, kGatherChunk = 4096, kGatherMaxSrc = 16;
+
+template <int R>
+__device__ __forceinline__ void gather_col(const GatherSource *lsrc, uint32_t c, uint32_t t, void *dst,
+                                           const uint32_t (&src)[R], const uint64_t (&row)[R],
+                                           const uint64_t (&orow)[R], const bool (&on)[R]) {
+    uint64_t lo[R], hi[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+        load_elem(lsrc[src[u]].cols[c], t, row[u], lo[u], hi[u]);
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+        if (on[u])
+            store_elem(dst, t, orow[u], lo[u], hi[u]);
+}
+
+// LSRC: the sources (at most kGatherMaxSrc) are staged in LDS; else read
+// where they lie (one source per segment, as batched joins gather them).
+template <bool LSRC>
+__global__ __launch_bounds__(kGatherThreads) void k_gather_segments(const uint64_t *__restrict__ seg_start,
+                                                                    uint32_t n_seg, const uint32_t *__restrict__ seg_src,
+                                                                    const uint64_t *__restrict__ seg_row,
+                                                                    const GatherSource *__restrict__ srcs, uint32_t n_src,
+                                                                    uint32_t n_cols, uint64_t *__restrict__ out_keys,
+                                                                    OutColSet out) {
+    __shared__ uint64_t lstart[kGatherChunk];
+    __shared__ GatherSource lsrc[kGatherMaxSrc];
+    __shared__ uint32_t g_lo_s, n_here_s;
     const uint64_t total = seg_start[n_seg];
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const uint32_t g = upper_bound_idx(seg_start, n_seg, i);
-        const GatherSource &src = srcs[seg_src[g]];
-        const uint64_t r = seg_row[g] + (i - seg_start[g]);
-        out_keys[i] = src.keys[r];
-        for (uint32_t c = 0; c < n_cols; ++c) {
-            uint64_t lo, hi;
-            const uint32_t t = col_type(out, c);
-            load_elem(src.cols[c], t, r, lo, hi);
-            store_elem(col_ptr(out, c), t, i, lo, hi);
+    const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * kGatherChunk;
+    if (c0 >= total)
+        return; // uniform
+    const uint64_t c1 = min(total, c0 + kGatherChunk);
+    if (threadIdx.x == 0) {
+        const uint32_t a = upper_bound_idx(seg_start, n_seg, c0);
+        const uint32_t b = upper_bound_idx(seg_start, n_seg, c1 - 1);
+        g_lo_s = a;
+        n_here_s = b - a + 1;
+    }
+    if (LSRC && threadIdx.x < n_src)
+        lsrc[threadIdx.x] = srcs[threadIdx.x];
+    __syncthreads();
+    const GatherSource *sp = LSRC ? lsrc : srcs;
+    const uint32_t g_lo = g_lo_s, n_here = n_here_s;
+    // more segments than rows only with empty segments: then search in place
+    const bool staged = n_here <= kGatherChunk;
+    if (staged)
+        for (uint32_t j = threadIdx.x; j < n_here; j += kGatherThreads)
+            lstart[j] = seg_start[g_lo + j];
+    __syncthreads();
+    for (uint64_t base = c0; base < c1; base += kGatherThreads * kGatherRpt) {
+        uint32_t src[kGatherRpt];
+        uint64_t row[kGatherRpt], orow[kGatherRpt];
+        bool on[kGatherRpt];
+#pragma unroll
+        for (int u = 0; u < static_cast<int>(kGatherRpt); ++u) {
+            const uint64_t i = base + static_cast<uint64_t>(u) * kGatherThreads + threadIdx.x;
+            on[u] = i < c1;
+            const uint64_t x = on[u] ? i : c0;
+            const uint32_t g = staged ? upper_bound_idx(lstart, n_here, x) : upper_bound_idx(seg_start + g_lo, n_here, x);
+            src[u] = seg_src[g_lo + g];
+            row[u] = seg_row[g_lo + g] + (x - seg_start[g_lo + g]);
+            orow[u] = i;
         }
+        {
+            uint64_t kv[kGatherRpt];
+#pragma unroll
+            for (int u = 0; u < static_cast<int>(kGatherRpt); ++u)
+                kv[u] = sp[src[u]].keys[row[u]];
+#pragma unroll
+            for (int u = 0; u < static_cast<int>(kGatherRpt); ++u)
+                if (on[u])
+                    out_keys[orow[u]] = kv[u];
+        }
+        for (uint32_t c = 0; c < n_cols; ++c)
+            gather_col<kGatherRpt>(sp, c, col_type(out, c), col_ptr(out, c), src, row, orow, on);
     }
 }
 
@@ -147,13 +208,18 @@ void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v) {
 }
 
 void gather_segments(hipStream_t s, const uint64_t *seg_start, uint32_t n_seg, const uint32_t *seg_src,
-                     const uint64_t *seg_row, const GatherSource *srcs, uint64_t total_rows, uint64_t *out_keys,
-                     const OutColSet &out, KernelTimer *tm) {
+                     const uint64_t *seg_row, const GatherSource *srcs, uint32_t n_src, uint64_t total_rows,
+                     uint64_t *out_keys, const OutColSet &out, KernelTimer *tm) {
     if (total_rows == 0 || n_seg == 0)
         return;
     TimedScope ts(tm, "gather_segments", s);
-    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((total_rows + 255) / 256, 256 * 32));
-    k_gather_segments<<<grid, 256, 0, s>>>(seg_start, n_seg, seg_src, seg_row, srcs, out.n, out_keys, out);
+    const uint32_t grid = checked_grid((total_rows + kGatherChunk - 1) / kGatherChunk, kGatherThreads, "gather_segments");
+    if (n_src <= kGatherMaxSrc)
+        k_gather_segments<true><<<grid, kGatherThreads, 0, s>>>(seg_start, n_seg, seg_src, seg_row, srcs, n_src, out.n,
+                                                                out_keys, out);
+    else
+        k_gather_segments<false><<<grid, kGatherThreads, 0, s>>>(seg_start, n_seg, seg_src, seg_row, srcs, n_src, out.n,
+                                                                 out_keys, out);
 }
 
 void checksum_columns(hipStream_t s, uint64_t n, const uint64_t *keys, const ColSet &left, const ColSet &right,

@@ -192,6 +192,7 @@ size_t leaf_table_bytes(uint32_t rows, uint32_t load_pct);
 void prebuilt_join(hipStream_t s, const LeafJoinArgs &a, uint64_t probe_rows, uint32_t leaf_bits, KernelTimer *tm);
 void leaf_table_images(hipStream_t s, const LeafJoinArgs &a, unsigned char *img, uint64_t stride, uint32_t *side,
                        uint32_t *dup, KernelTimer *tm);
+
 size_t leaf_join_lds_bytes(uint32_t lds_max_rows, uint32_t load_pct);
 uint32_t leaf_join_lds_max_rows(uint32_t lds_bytes, uint32_t load_pct);
 
@@ -205,6 +206,41 @@ struct Cell16 {
     uint32_t row_no;
 };
 static_assert(sizeof(Cell16) == 16, "cell is 16 bytes");
+
+// Group images: every leaf's table of a build side with repeated keys, built
+// once in HBM as 16-byte cells {key, start, count} (the JoinHashTable cell of
+// src/HashTable.h:35-40 with the RowRefCell replaced by the key's row group):
+// the key's build rows are rows [start, start + count) of the grouped build
+// columns, the partitioned build rows reordered so that the rows of one key
+// are adjacent (src/BlockStore.h:56-89 chains, laid out once). The group of
+// the empty-key sentinel of leaf L is {side[2L], side[2L + 1]}.
+struct GroupImage {
+    const Cell16 *cells; // leaf L's table: cells + L * stride, capacity table_cap(rows of L)
+    uint64_t stride;     // cells per leaf
+    const uint32_t *side;
+    const uint32_t *r_off; // leaf offsets of the partitioned build side (n_leaves + 1)
+    ColSet cols;           // grouped build columns
+    uint32_t leaf_bits;
+    uint32_t load_pct;
+};
+// LDS bytes of the image build for leaves of up to max_rows rows.
+size_t group_image_lds_bytes(uint32_t max_rows, uint32_t load_pct);
+// One workgroup per leaf (leaves of at most max_rows rows): cells, side
+// groups, and perm (grouped row -> partitioned build row).
+void group_images(hipStream_t s, const uint64_t *r_keys, const uint32_t *r_off, uint32_t n_leaves, uint32_t max_rows,
+                  uint32_t load_pct, Cell16 *cells, uint64_t stride, uint32_t *side, uint32_t *perm, KernelTimer *tm);
+// Output row i of every column = input row perm[i], for i < rows.
+void gather_perm(hipStream_t s, const ColSet &in, const OutColSet &out, const uint32_t *perm, uint64_t rows,
+                 KernelTimer *tm);
+// Output rows of `rows` probe keys against the images, added to *total.
+void group_count(hipStream_t s, const GroupImage &g, const uint64_t *keys, uint64_t rows, unsigned long long *total,
+                 KernelTimer *tm);
+// Join of probe rows (keys, cols) against the images: each work item of
+// 2,048 probe rows reserves its output rows on out.cursor (rows at or
+// beyond out.cap are counted only) and writes them with the key's rows
+// expanded from the grouped build columns.
+void group_join(hipStream_t s, const GroupImage &g, const uint64_t *keys, const ColSet &cols, uint64_t rows,
+                const OutSink &out, KernelTimer *tm);
 
 struct GlobalHT {
     Cell16 *cells;
@@ -256,10 +292,11 @@ struct GatherSource {
     const void *cols[kMaxCols];
 };
 // Output rows [seg_start[g], seg_start[g+1]) (n_seg + 1 entries, device) are
-// rows seg_row[g] .. of srcs[seg_src[g]]; out's column types are the sources'.
+// rows seg_row[g] .. of srcs[seg_src[g]] (n_src sources, every segment
+// non-empty); out's column types are the sources'.
 void gather_segments(hipStream_t s, const uint64_t *seg_start, uint32_t n_seg, const uint32_t *seg_src,
-                     const uint64_t *seg_row, const GatherSource *srcs, uint64_t total_rows, uint64_t *out_keys,
-                     const OutColSet &out, KernelTimer *tm);
+                     const uint64_t *seg_row, const GatherSource *srcs, uint32_t n_src, uint64_t total_rows,
+                     uint64_t *out_keys, const OutColSet &out, KernelTimer *tm);
 
 // ---------------------------------------------------------------- checksum / generators
 void checksum_columns(hipStream_t s, uint64_t n, const uint64_t *keys, const ColSet &left, const ColSet &right,
