@@ -391,6 +391,12 @@ def headline(args, tuning):
 
     build.release()
     probe.release()
+    if roofline is not None and args.scale_log2 == 0:
+        # the part's measured streaming rates (4 GiB buffers), beside the 8 TB/s peak
+        rates = ctx.stream_bench(4 << 30, reps=5)
+        roofline["measured_stream_GBps"] = {k: round(v / 1e9, 1) for k, v in rates.items()}
+        roofline["frac_of_copy"] = round(roofline["achieved"] / (rates["copy"] / 1e9), 4)
+        log("streaming rates: " + ", ".join(f"{k} {v / 1e9:.0f} GB/s" for k, v in rates.items()))
     ctx.close()  # each workload below runs in a context of its own, on a device with nothing cached
     return dict(R=R, S=S, rb=rb, rp=rp, pass_bits=pass_bits, ms_per_step=ms_per_step, tuples_per_s=tuples_per_s,
                 best_e2e=best_e2e, b_alg=b_alg, out_rows=out_rows, parity=parity, phases=phases, ktimes=ktimes,

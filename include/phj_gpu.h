@@ -355,6 +355,17 @@ phj_status phj_kernel_names(phj_ctx *ctx, char *buf, uint64_t cap);
 /* Plain device copy of `bytes` (src to dst) timed with HIP events; used to
  * calibrate HBM traffic counters. */
 phj_status phj_copy_bench(phj_ctx *ctx, void *dst, const void *src, uint64_t bytes, int reps, double *avg_ms);
+/* Harness: the part's streaming rates, the measured ceiling beside the 8 TB/s
+ * peak. PHJ_STREAM_COPY copies `bytes` from src to dst, PHJ_STREAM_READ reads
+ * src (nontemporal 16-byte loads; dst receives at most one word),
+ * PHJ_STREAM_WRITE writes dst (whole 16-byte words: bytes / 16 of them);
+ * *avg_ms is the mean of `reps` timed passes after one untimed pass. No
+ * reference counterpart. */
+#define PHJ_STREAM_COPY 0u
+#define PHJ_STREAM_READ 1u
+#define PHJ_STREAM_WRITE 2u
+phj_status phj_stream_bench(phj_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint32_t mode, int reps,
+                            double *avg_ms);
 
 #ifdef __cplusplus
 }

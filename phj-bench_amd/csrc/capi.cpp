@@ -3373,4 +3373,23 @@ phj_status phj_copy_bench(phj_ctx *ctx, void *dst, const void *src, uint64_t byt
     });
 }
 
+phj_status phj_stream_bench(phj_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint32_t mode, int reps,
+                            double *avg_ms) {
+    return guard(ctx, [&] {
+        if (reps <= 0 || mode > PHJ_STREAM_WRITE)
+            invalid("reps must be positive and mode one of PHJ_STREAM_*");
+        if ((mode != PHJ_STREAM_WRITE && src == nullptr) || (mode != PHJ_STREAM_READ && dst == nullptr))
+            invalid("null buffer");
+        launch_stream(ctx->stream, static_cast<int>(mode), dst, src, bytes, nullptr);
+        EventPair ev;
+        HIPCHK(hipEventRecord(ev.a, ctx->stream));
+        for (int r = 0; r < reps; ++r)
+            launch_stream(ctx->stream, static_cast<int>(mode), dst, src, bytes, ctx->tm());
+        HIPCHK(hipEventRecord(ev.b, ctx->stream));
+        ctx->sync();
+        if (avg_ms)
+            *avg_ms = ev.ms() / reps;
+    });
+}
+
 } // extern "C"

@@ -181,6 +181,81 @@ void launch_hash_batch(hipStream_t s, const uint64_t *in, uint64_t n, uint64_t *
     k_hash_batch<<<grid, 256, 0, s>>>(in, n, out);
 }
 
+// Streaming rates of the part, for the roofline's measured ceiling: every
+// 16-byte word of a buffer read once (nontemporal loads, eight in flight per
+// thread; the words are folded into one value so the loads are kept), or
+// written once.
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_stream_read(const v4u32 *__restrict__ a, uint64_t n, uint32_t *__restrict__ sink) {
+    constexpr int U = 8;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * U;
+    uint32_t acc = 0;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x * U + threadIdx.x; i < n; i += stride) {
+        v4u32 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t j = i + static_cast<uint64_t>(u) * blockDim.x;
+            v[u] = __builtin_nontemporal_load(a + (j < n ? j : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    if (acc == 0x9E3779B9u) // practically never; keeps the loads
+        sink[0] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_stream_write(v4u32 *__restrict__ b, uint64_t n) {
+    constexpr int U = 4;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * U;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x * U + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t j = i + static_cast<uint64_t>(u) * blockDim.x;
+            if (j < n)
+                b[j] = v4u32{static_cast<uint32_t>(j), 1u, 2u, 3u};
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_stream_copy(const v4u32 *__restrict__ a, v4u32 *__restrict__ b, uint64_t n) {
+    constexpr int U = 4;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * U;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x * U + threadIdx.x; i < n; i += stride) {
+        v4u32 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t j = i + static_cast<uint64_t>(u) * blockDim.x;
+            v[u] = __builtin_nontemporal_load(a + (j < n ? j : 0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t j = i + static_cast<uint64_t>(u) * blockDim.x;
+            if (j < n)
+                __builtin_nontemporal_store(v[u], b + j);
+        }
+    }
+}
+
+void launch_stream(hipStream_t s, int mode, void *dst, const void *src, uint64_t bytes, KernelTimer *tm) {
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0)
+        return;
+    TimedScope ts(tm, mode == 0 ? "stream_copy" : mode == 1 ? "stream_read" : "stream_write", s);
+    if (mode == 0) {
+        // four 16-byte words in flight per thread, nontemporal both ways:
+        // the fastest of the copy forms measured on this part
+        k_stream_copy<<<256 * 64, 256, 0, s>>>(static_cast<const v4u32 *>(src), static_cast<v4u32 *>(dst), n16);
+        return;
+    }
+    const uint32_t grid = 256 * 16;
+    if (mode == 1)
+        k_stream_read<<<grid, 256, 0, s>>>(static_cast<const v4u32 *>(src), n16, static_cast<uint32_t *>(dst));
+    else
+        k_stream_write<<<grid, 256, 0, s>>>(static_cast<v4u32 *>(dst), n16);
+}
+
 void launch_copy(hipStream_t s, void *dst, const void *src, uint64_t bytes, KernelTimer *tm) {
     TimedScope ts(tm, "copy", s);
     const uint64_t n16 = bytes / 16;

@@ -50,6 +50,9 @@ struct TimedScope {
 // ---------------------------------------------------------------- hashing
 void launch_hash_batch(hipStream_t s, const uint64_t *in, uint64_t n, uint64_t *out, KernelTimer *tm);
 void launch_copy(hipStream_t s, void *dst, const void *src, uint64_t bytes, KernelTimer *tm);
+// mode 0: copy src -> dst; 1: read src (dst receives at most one word); 2: write dst.
+// Whole 16-byte words only (bytes / 16 of them).
+void launch_stream(hipStream_t s, int mode, void *dst, const void *src, uint64_t bytes, KernelTimer *tm);
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
 void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
 

@@ -203,6 +203,8 @@ _SIGNATURES = {
     "phj_kernel_times": ([C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)], C.c_int),
     "phj_kernel_names": ([C.c_void_p, C.c_char_p, C.c_uint64], C.c_int),
     "phj_copy_bench": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "phj_stream_bench": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_double)],
+                         C.c_int),
 }
 
 _lib = None
@@ -837,6 +839,24 @@ class Context:
             _raise(self._p, load_library().phj_kernel_times(self._p, name.encode(), C.byref(ms), C.byref(n)))
             out[name] = (float(ms.value), int(n.value))
         return out
+
+    def stream_bench(self, nbytes, reps=5):
+        """Streaming rates of the device, in bytes/s: {"copy": bytes read and
+        written per second copying nbytes, "read": reading, "write": writing}."""
+        lib = load_library()
+        a, b = C.c_void_p(), C.c_void_p()
+        _raise(self._p, lib.phj_dev_alloc(self._p, nbytes, C.byref(a)))
+        _raise(self._p, lib.phj_dev_alloc(self._p, nbytes, C.byref(b)))
+        try:
+            out = {}
+            for name, mode, moved in (("copy", 0, 2 * nbytes), ("read", 1, nbytes), ("write", 2, nbytes)):
+                ms = C.c_double()
+                _raise(self._p, lib.phj_stream_bench(self._p, b, a, nbytes, mode, reps, C.byref(ms)))
+                out[name] = moved / (ms.value * 1e-3)
+            return out
+        finally:
+            lib.phj_dev_free(self._p, a)
+            lib.phj_dev_free(self._p, b)
 
     def copy_bench(self, nbytes, reps=5):
         lib = load_library()

@@ -476,3 +476,12 @@ def test_cpp_adapter_drop_in(scheme, pass_bits, callers):
             n, rs = struct.unpack("<QQ", f.read(16))
             got = np.frombuffer(f.read(), dtype=np.uint8).reshape(n, rs)
     assert np.array_equal(got, want)
+
+
+def test_stream_bench_rates(ctx):
+    """The streaming-rate harness runs each mode and reports plausible rates
+    (above 100 GB/s, below the 8 TB/s peak by any margin of timing noise)."""
+    rates = ctx.stream_bench(256 << 20, reps=3)
+    assert set(rates) == {"copy", "read", "write"}
+    for v in rates.values():
+        assert 1e11 < v < 1e13
