@@ -1045,8 +1045,9 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                     gather_set_abs<U>(a.s_src, a.out.right, rb, srow, opos, on);
                 }
             } else {
-                // slot k: output row, build row; the output key is the probe
-                // key (equal to the matched build key, the sentinel included)
+                // slot k: output row and matched table slot; the output key
+                // is the slot's key (from LDS: no memory read), then the slot
+                // becomes the build row for the column sweeps
                 uint32_t meta[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -1054,11 +1055,23 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                     const uint64_t m = __ballot(c != 0);
                     const uint32_t opos = slot_off[k][wave] + mbcnt(m);
                     const bool on = c != 0 && rb + opos < out_cap;
-                    const uint32_t row = c == 0 ? 0u : (c - 1 == kSideSlot ? side_head : trows[c - 1]);
-                    meta[k] = on ? (0x80000000u | (opos << 16) | row) : 0u;
+                    meta[k] = on ? (0x80000000u | (opos << 16) | (c - 1)) : 0u;
+                }
+                uint64_t *okeys = a.out.keys + rb;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (meta_on(meta[k])) {
+                        const uint32_t slot = meta_row(meta[k]);
+                        st_at<uint64_t>(okeys, meta_pos(meta[k]) * 8u, slot == kSideSlot ? kEmptyKey : tkeys[slot]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint32_t slot = meta_row(meta[k]);
+                    const uint32_t row = slot == kSideSlot ? side_head : trows[meta_on(meta[k]) ? slot : 0u];
+                    meta[k] = (meta[k] & 0xFFFF0000u) | row;
                 }
                 const uint32_t none[K] = {};
-                sweep_col<THREADS, K, kRowProbe, uint64_t>(a.s_keys + sb, a.out.keys + rb, meta, none, threadIdx.x);
                 sweep_set<THREADS, K, kRowMeta>(a.r_cols, it.r0, a.out.left, rb, meta, none);
                 sweep_set<THREADS, K, kRowProbe>(a.s_cols, sb, a.out.right, rb, meta, none);
             }
