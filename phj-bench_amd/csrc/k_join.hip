@@ -665,19 +665,14 @@ __device__ __forceinline__ void move_set(const ColSet &in, uint64_t src_row0, co
     }
 }
 
-// Per-slot emit metadata of the fused join: bit 31 = the slot has an output
-// row, bits 16..30 = its output row relative to the round's base, bits 0..15 =
-// the matched build row within the leaf.
+// Per-slot emit metadata of the fused and prebuilt joins: bit 31 = the slot
+// has an output row, bits 16..30 = its output row relative to the round's
+// base, bits 0..15 = the matched build row within the leaf (or, before the
+// fused join resolves it, the matched table slot).
 __device__ __forceinline__ bool meta_on(uint32_t m) { return (m >> 31) != 0; }
 __device__ __forceinline__ uint32_t meta_pos(uint32_t m) { return (m >> 16) & 0x7FFFu; }
 __device__ __forceinline__ uint32_t meta_row(uint32_t m) { return m & 0xFFFFu; }
 
-// One column over all K slots of a lane, G slots at a time: the G loads are
-// issued together, then the G stores (a row at a time would wait for each
-// load before the next is issued, since the compiler cannot move a load above
-// a store that may alias it). PROBE: source rows are the slots' probe rows
-// (slot k of the lane is row k * THREADS + threadIdx.x of the round); else the
-// matched build rows. Slots without a row load row 0 and store nothing.
 // Where a slot's source row comes from: the lane's probe row of the round
 // (slot k is row k * THREADS + threadIdx.x), the build row in the metadata,
 // or a separate array of (64-bit indexed) rows.
@@ -706,6 +701,11 @@ __device__ __forceinline__ void sweep_store(void *dst, const uint32_t (&meta)[K]
             st_at<T>(dst, meta_pos(meta[k]) * static_cast<uint32_t>(sizeof(T)), v[u]);
     }
 }
+// One column over all K slots of a lane, G slots at a time: the G loads are
+// issued together, then the G stores, and group h + 1's loads go out before
+// group h's stores (a row at a time would wait for each load before the next
+// is issued, since the compiler cannot move a load above a store that may
+// alias it). Slots without a row load row 0 and store nothing.
 template <int THREADS, int K, int MODE, class T>
 __device__ __forceinline__ void sweep_col(const void *src, void *dst, const uint32_t (&meta)[K],
                                           const uint32_t (&rows)[K], uint32_t tid) {
