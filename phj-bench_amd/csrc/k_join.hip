@@ -1931,7 +1931,7 @@ __device__ __forceinline__ void group_move_col(const void *src, uint32_t t, void
 template <int THREADS, int K, int G>
 __global__ __launch_bounds__(THREADS) void k_group_join(GroupImage g, const uint64_t *__restrict__ keys, ColSet cols,
                                                         uint64_t rows, OutSink out) {
-    constexpr int LB = 4;
+    constexpr int LB = K < 4 ? K : 4;
     constexpr uint32_t N = THREADS * K;
     static_assert(K % LB == 0, "lookup batches");
     __shared__ uint32_t e_off[N];
@@ -2046,7 +2046,7 @@ void group_count(hipStream_t s, const GroupImage &g, const uint64_t *keys, uint6
 
 void group_join(hipStream_t s, const GroupImage &g, const uint64_t *keys, const ColSet &cols, uint64_t rows,
                 const OutSink &out, KernelTimer *tm) {
-    constexpr int T = 256, K = 8, G = 4;
+    constexpr int T = 256, K = 2, G = 8; // short items: each probe row may expand to many output rows
     if (rows == 0)
         return;
     TimedScope ts(tm, "leaf_join", s);

@@ -239,7 +239,7 @@ void gather_perm(hipStream_t s, const ColSet &in, const OutColSet &out, const ui
 void group_count(hipStream_t s, const GroupImage &g, const uint64_t *keys, uint64_t rows, unsigned long long *total,
                  KernelTimer *tm);
 // Join of probe rows (keys, cols) against the images: each work item of
-// 2,048 probe rows reserves its output rows on out.cursor (rows at or
+// 512 probe rows reserves its output rows on out.cursor (rows at or
 // beyond out.cap are counted only) and writes them with the key's rows
 // expanded from the grouped build columns.
 void group_join(hipStream_t s, const GroupImage &g, const uint64_t *keys, const ColSet &cols, uint64_t rows,
