@@ -361,6 +361,28 @@ def test_bep_schedule_counters(ctx, pass_bits):
     assert seen == sorted(seen)
 
 
+@pytest.mark.parametrize("dsname", ["zipf_multi", "sentinel_dups", "no_payload", "wide_payload"])
+@pytest.mark.parametrize("pass_bits", [[4, 4], [6], [8, 8]])
+def test_bep_group_images_many_rounds(ctx, dsname, pass_bits):
+    """PHJ best-effort against group images (repeated build keys: every key's
+    rows adjacent, cells {key, start, count}) under a budget small enough that
+    the probe rows are refined and evicted in many rounds, each round joined
+    against the images built once. Keys include 0 and the all-ones sentinel
+    (sentinel_dups); rows equal the oracle's, sorted byte for byte."""
+    ob, op = data()[dsname]
+    # the probe side repeated to 120K rows: 1 MiB of buffers holds a fraction of it
+    reps = -(-120_000 // op.keys.size)
+    tile = lambda a: np.tile(a, (reps,) + (1,) * (a.ndim - 1))[:120_000]
+    op = O.Relation(tile(op.keys), [tile(c) for c in op.cols], op.types)
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    res = ctx.runPhjBep(host_rel(ob), host_rel(op), G.RadixConfig(pass_bits), bep_budget_mib=1)
+    assert res.out_rows == want.shape[0]
+    assert np.array_equal(sorted_rows_of(res.to_host()), want)
+    assert res.build.wall_ms > 0  # the images were built up front
+    assert res.bep_evictions >= bep_groups(op.keys, sum(pass_bits))
+    res.release()
+
+
 @pytest.mark.parametrize("chunk", [700, 5000])
 def test_chains_across_work_items(ctx, chunk):
     """Repeated build keys in leaves of thousands of rows (tables built by many
