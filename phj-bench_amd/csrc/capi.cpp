@@ -1365,12 +1365,17 @@ bool build_group_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan 
 
 class BepSchedule {
 public:
+    // RP: the build side partitioned for the tables (the schedule's leaves
+    // for the LDS join; possibly finer for tables built up front).
     BepSchedule(phj_ctx *ctx, const Partitioned &RP, const DevRel &S, uint32_t p1_bits, const std::vector<uint32_t> &rest,
                 uint64_t budget, const LeafJoinPlan &plan, uint32_t flags, const BepTables &tables, OutBuffers &out)
         : ctx_(ctx), RP_(RP), S_(S), p1_(p1_bits), rest_(rest), plan_(plan), flags_(flags), tab_(tables), out_(out),
           keep_(ctx) {
         P_ = 1u << p1_;
-        leaves_ = static_cast<uint32_t>(RP.parts);
+        uint32_t total = p1_;
+        for (uint32_t b : rest_)
+            total += b;
+        leaves_ = 1u << total;
         lpp_ = leaves_ / P_;
         rb_ = 8;
         for (uint32_t c = 0; c < S.cols.n; ++c)
@@ -1467,12 +1472,7 @@ public:
             ctx_->sync();
         }
         upload_.reset();
-        for (auto &e : shuffle_ev_)
-            shuffle_ms_ += e->ms();
-        for (auto &e : probe_ev_)
-            probe_ms_ += e->ms();
-        for (auto &e : gather_ev_)
-            overhead_ms_ += e->ms();
+        fold_events();
     }
 
     uint64_t out_rows() const { return n_out_; }
@@ -1497,9 +1497,29 @@ private:
     };
 
     EventPair &timed(std::vector<std::unique_ptr<EventPair>> &v) {
-        v.push_back(std::make_unique<EventPair>());
+        if (free_ev_.empty()) {
+            v.push_back(std::make_unique<EventPair>());
+        } else {
+            v.push_back(std::move(free_ev_.back()));
+            free_ev_.pop_back();
+        }
         HIPCHK(hipEventRecord(v.back()->a, ctx_->stream));
         return *v.back();
+    }
+
+    // After a wait that covers everything queued so far: the phase times of
+    // the recorded event pairs are added up and the pairs reused.
+    void fold_events() {
+        auto fold = [&](std::vector<std::unique_ptr<EventPair>> &v, double &acc) {
+            for (auto &e : v) {
+                acc += e->ms();
+                free_ev_.push_back(std::move(e));
+            }
+            v.clear();
+        };
+        fold(shuffle_ev_, shuffle_ms_);
+        fold(probe_ev_, probe_ms_);
+        fold(gather_ev_, overhead_ms_);
     }
 
     // Groups of `run` still holding rows, mapped to bucket or leaf ids.
@@ -1676,6 +1696,7 @@ private:
             leaf->off.assign(leaf_off_, leaf_off_ + n_off);
         }
         upload_.reset();
+        fold_events();
         leaf->done.assign(static_cast<size_t>(nq) * lpp_, 0);
         leaf->qidx.assign(P_, -1);
         for (uint32_t i = 0; i < nq; ++i) {
@@ -1835,7 +1856,7 @@ private:
             a.r_off = RP_.off;
             a.s_keys = st.keys;
             a.s_cols = st.cols;
-            a.n_leaves = leaves_;
+            a.n_leaves = static_cast<uint32_t>(RP_.parts);
             a.load_pct = plan_.load_pct;
             a.table_img = tab_.unique.img;
             a.img_stride = tab_.unique.stride;
@@ -1917,7 +1938,7 @@ private:
     uint64_t unrefined_ = 0, global_leaf_ = 0;
     uint32_t cursor_ = 0;
     std::vector<std::unique_ptr<BepRun>> batch_runs_, leaf_runs_;
-    std::vector<std::unique_ptr<EventPair>> shuffle_ev_, probe_ev_, gather_ev_;
+    std::vector<std::unique_ptr<EventPair>> shuffle_ev_, probe_ev_, gather_ev_, free_ev_;
     bool repeated_keys_ = false;
     uint64_t n_out_ = 0, evictions_ = 0, refinements_ = 0, peak_ = 0;
     double shuffle_ms_ = 0, probe_ms_ = 0, overhead_ms_ = 0, big_build_ms_ = 0;
@@ -1947,15 +1968,24 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
     HIPCHK(hipEventRecord(e2e.a, ctx->stream));
     HIPCHK(hipEventRecord(ebs.a, ctx->stream));
     Held keep(ctx);
-    const Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
+    // The tables' leaves: the configuration's, or finer when its leaves would
+    // average more than an LDS leaf's rows (as PHJ-PURE refines them). The
+    // schedule's buckets and leaves, and so its counters, stay the
+    // configuration's; a probe row finds its table from the top bits of its
+    // hash, which the finer leaves refine.
+    const uint32_t join_bits = std::max(total, join_total_bits(total, b.rows, tun));
+    Held fine(ctx);
+    Partitioned RP = radix_partition(ctx, R, nullptr, 1, 0, join_bits > total ? device_passes(join_bits) : passes,
+                                     join_bits > total ? fine : keep, "build", radix_flags(tun));
     HIPCHK(hipEventRecord(ebs.b, ctx->stream));
     // Eager leaf tables (src/PHJBep.cpp:157-231), built once in HBM: single-
     // row cells when every leaf fits LDS, no key repeats and an eviction round
     // holds few probe rows per leaf; group cells when keys repeat. Otherwise
-    // each round's work items build theirs in LDS.
+    // each round's work items build theirs in LDS (on the configuration's
+    // leaves).
     double build_ms = 0;
     BepTables tables;
-    tables.leaf_bits = total;
+    tables.leaf_bits = join_bits;
     const uint64_t round_rows = std::max<uint64_t>(budget_bytes * 5 / 8 / row_bytes * 3 / 8, 1);
     uint64_t out_cap = std::max<uint64_t>(S.rows, 1);
     if (build_table_images(ctx, RP, plan, round_rows, keep, tables.unique, build_ms)) {
@@ -1968,7 +1998,7 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
             leaf_max_rows(ctx->stream, RP.off, static_cast<uint32_t>(RP.parts), dmax);
             max_leaf = read_scalar(ctx, dmax);
         }
-        if (build_group_images(ctx, RP, plan, total, max_leaf, keep, tables.group, build_ms)) {
+        if (build_group_images(ctx, RP, plan, join_bits, max_leaf, keep, tables.group, build_ms)) {
             tables.kind = BepTables::kGroup;
             // the output's exact size: every probe row's group length, summed
             Scratch s2(ctx);
@@ -1976,6 +2006,18 @@ void run_phj_bep(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, con
             launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(cnt), 1, 0);
             group_count(ctx->stream, tables.group, S.keys, S.rows, cnt, ctx->tm());
             out_cap = std::max<uint64_t>(read_scalar(ctx, cnt), 1);
+        }
+    }
+    if (join_bits > total) {
+        if (tables.kind == BepTables::kLds) {
+            // no tables up front: the rounds join leaf by leaf on the configuration's leaves
+            fine.free_all();
+            HIPCHK(hipEventRecord(ebs.a, ctx->stream));
+            RP = radix_partition(ctx, R, nullptr, 1, 0, passes, keep, "build", radix_flags(tun));
+            HIPCHK(hipEventRecord(ebs.b, ctx->stream));
+        } else {
+            keep.insert(keep.end(), fine.begin(), fine.end());
+            fine.clear();
         }
     }
     OutBuffers out = alloc_output(ctx, R.cols, S.cols, out_cap);
