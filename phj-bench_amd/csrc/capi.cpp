@@ -1273,11 +1273,15 @@ bool build_table_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan 
 // The reference's schedule, run by one device scheduler (the host side of the
 // context's stream) instead of `threads` CPU workers:
 //   - the build side is shuffled to full leaf depth and every leaf's table is
-//     built up front when the build side allows (src/PHJBep.cpp:157-231);
+//     built up front in HBM when the build side allows (src/PHJBep.cpp:157-231):
+//     single-row cells without repeated keys, group cells with them (the
+//     key's rows adjacent), on leaves refined past the configuration's when
+//     those would not fit an LDS build;
 //   - the probe side is scattered by its first pass bits, one batch at a time,
 //     into pass-1 buckets ("unrefined", src/PHJBep.cpp:574-606). A batch is a
 //     quarter of the unrefined high-water mark, the device's unit of work where
-//     the reference scatters one 10,000-row block;
+//     the reference scatters one 10,000-row block; the next batch's scatter is
+//     queued before the host waits for this one's bucket counts;
 //   - after each batch, when unrefined rows reach the high-water mark, the
 //     largest buckets are refined through the remaining passes into leaf
 //     buffers until the low-water mark holds (src/PHJBep.cpp:508-533, :397-444);
@@ -1286,7 +1290,8 @@ bool build_table_images(phj_ctx *ctx, const Partitioned &RP, const LeafJoinPlan 
 //     round-robin from a cursor (empty leaves skipped) until their low-water
 //     mark holds: each evicted leaf's buffered probe rows are joined against
 //     its table and dropped (src/PHJBep.cpp:453-505, :535-568); all leaves of
-//     one trigger are joined in one launch sequence;
+//     one trigger are joined in one launch sequence, which against tables
+//     built up front reserves its output on a device cursor and never waits;
 //   - at the end every residual bucket is refined and every leaf drained
 //     (src/PHJBep.cpp:633-695).
 // Marks (src/PHJBep.cpp:186-190, one worker): unrefined high water budget/2,
