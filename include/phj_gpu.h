@@ -10,7 +10,9 @@
  *                                              bep_budget_mib)
  *   phj_join             scheme dispatch by SchemeChoice                      src/Main.cpp:54-109, src/CLI.h:15-22
  *   phj_int_hash64_batch intHash64Batch(in, n, out)                           src/Hash.h:33-58
- *   phj_radix_shuffle    radixShuffle(input, cfg, threads)                    src/RadixPartition.h:187-199
+ *   phj_radix_shuffle    radixShuffle(input, cfg, threads)                    src/RadixPartition.h:195-199
+ *   phj_radix_shuffle_project radixShuffle(input, out_schema, in_col_index,  src/RadixPartition.h:187-193
+ *                        cfg, threads)
  *   phj_refine_to_leaves refineToLeaves(input, schema, pass_bits, leaves)     src/RadixPartition.h:393-399
  *   phj_hashtable_build_find  JoinHashTable::reserve/insert + batchFind,      src/HashTable.h:72-138,
  *                        with BlockStore next-chain threading                 src/JoinOps.h:205-228, src/BlockStore.h:86-89
@@ -126,6 +128,8 @@ typedef struct phj_tuning {
                                           the digit byte column the previous pass's scatter wrote */
 #define PHJ_TUNE_NARROW_PROBE 0x20u    /* PHJ-PURE partitions the probe side as (key, row index) only and gathers
                                           matched rows' probe columns from the unpartitioned relation */
+#define PHJ_TUNE_ATOMIC_RANK 0x40u     /* radix scatter ranks rows with LDS atomics; partitions then keep no input
+                                          order (default: stable wave-ballot ranking, rows in input order) */
 
 typedef struct phj_join_desc {
     uint32_t scheme; /* phj_scheme */
@@ -224,6 +228,14 @@ phj_status phj_int_hash64_batch(phj_ctx *ctx, const uint64_t *in, uint64_t n, ui
 /* ---- radix partition: src/RadixPartition.h:187-199, :393-399 ---- */
 phj_status phj_radix_shuffle(phj_ctx *ctx, const phj_relation *input, const uint8_t *pass_bits, uint32_t n_passes,
                              phj_partitioned *out);
+/* radixShuffle(input, out_schema, in_col_index, cfg, threads),
+ * src/RadixPartition.h:187-193: output column c is input column
+ * in_col_index[c], of type out_types[c] (which must equal that column's type;
+ * an index past the input's columns or a type that differs is
+ * PHJ_ERR_INVALID_ARGUMENT). Columns may be dropped, reordered or repeated. */
+phj_status phj_radix_shuffle_project(phj_ctx *ctx, const phj_relation *input, const uint32_t *out_types,
+                                     const uint32_t *in_col_index, uint32_t n_out_cols, const uint8_t *pass_bits,
+                                     uint32_t n_passes, phj_partitioned *out);
 /* Restripe a shuffle made with pass_bits[0] only into leaves through passes
  * 1..n_passes-1 (refine_to_leaves mode); out has prod(2^pass_bits) parts. */
 phj_status phj_refine_to_leaves(phj_ctx *ctx, const phj_partitioned *pass1, const uint8_t *pass_bits, uint32_t n_passes,

@@ -198,6 +198,38 @@ def int_hash64_py(x: int) -> int:
     return x
 
 
+# ---------------------------------------------------------------- radix partition
+def leaf_ids(keys, total_bits: int):
+    """Partition of every key after `total_bits` bits of radix passes: the top
+    bits of intHash64 (src/RadixPartition.cpp:546-577, each pass taking the
+    next bits below the ones already consumed)."""
+    if total_bits == 0:
+        return np.zeros(np.asarray(keys).shape[0], dtype=np.int64)
+    return (int_hash64(keys) >> np.uint64(64 - total_bits)).astype(np.int64)
+
+
+def radix_shuffle(rel: Relation, pass_bits, out_types=None, in_col_index=None):
+    """radixShuffle(input, out_schema, in_col_index, cfg, threads)
+    (src/RadixPartition.cpp:546-577): returns (partitioned relation, offsets).
+
+    Order: scatterBatch appends each batch's rows to their partition in row
+    order (src/RadixPartition.cpp:316-393); pass 1 gives each thread a
+    contiguous run of blocks and concatenates the per-thread chains in thread
+    order (:144-205), and every later pass re-scatters one input partition's
+    chain on one thread (:209-250). Every partition therefore holds its rows in
+    input order: a stable sort of the rows by partition id."""
+    total = int(sum(pass_bits))
+    if in_col_index is None:
+        in_col_index = list(range(len(rel.cols)))
+        out_types = list(rel.types)
+    ids = leaf_ids(rel.keys, total)
+    order = np.argsort(ids, kind="stable")
+    cols = [rel.cols[i][order] for i in in_col_index]
+    off = np.zeros((1 << total) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(np.bincount(ids, minlength=1 << total))
+    return Relation(rel.keys[order], cols, list(out_types)), off
+
+
 # ---------------------------------------------------------------- join
 def reference_join_pairs(build: Relation, probe: Relation):
     L = lib()
@@ -312,6 +344,31 @@ def ref_int_hash64(x):
     out = np.empty_like(x)
     ref().ref_int_hash64_batch(_ptr(x), C.c_uint64(x.size), _ptr(out))
     return out
+
+
+def ref_radix_shuffle(rel: Relation, pass_bits, out_types=None, in_col_index=None, threads=1):
+    """The compiled reference's radixShuffle, flattened: (relation, offsets)."""
+    if in_col_index is None:
+        in_col_index = list(range(len(rel.cols)))
+        out_types = list(rel.types)
+    total = int(sum(pass_bits))
+    if total > 20:
+        raise NotImplementedError("this wrapper sizes part_rows for at most 2^20 partitions")
+    parts = 1 << total
+    keys = np.zeros(max(rel.rows, 1), dtype=np.uint64)
+    cols = [empty_column(t, max(rel.rows, 1)) for t in out_types]
+    rows = np.zeros(parts, dtype=np.uint64)
+    pb = (C.c_uint8 * len(pass_bits))(*pass_bits)
+    idx = (C.c_uint64 * max(1, len(in_col_index)))(*in_col_index)
+    rc = ref().ref_radix_shuffle(C.byref(rel.orc()), _types_arr(out_types), idx, C.c_uint32(len(out_types)), pb,
+                                 C.c_uint32(len(pass_bits)), C.c_uint64(threads), _ptr(keys), _cols_arr(cols),
+                                 _ptr(rows))
+    if rc != 0:
+        raise ValueError(f"reference radixShuffle rejected the configuration (rc={rc})")
+    off = np.zeros(parts + 1, dtype=np.int64)
+    off[1:] = np.cumsum(rows.astype(np.int64))
+    n = rel.rows
+    return Relation(keys[:n], [c[:n] for c in cols], list(out_types)), off
 
 
 def ref_reference_checksum(build: Relation, probe: Relation):

@@ -17,6 +17,7 @@
 #include "Hash.h"
 #include "PHJ.h"
 #include "PHJBep.h"
+#include "RadixPartition.h"
 #include "Reference.h"
 #include "oracle.h"
 
@@ -115,6 +116,43 @@ int ref_run(int scheme, const orc_relation *build, const orc_relation *probe, co
                 digestOutput(r.output, &res->checksum);
         } else {
             return 1;
+        }
+    } catch (const std::exception &) {
+        return 2;
+    }
+    return 0;
+}
+
+// radixShuffle(input, out_schema, in_col_index, cfg, threads)
+// (src/RadixPartition.h:187-193) with its partitions flattened in partition
+// order: out_keys/out_cols receive every partition's chain blocks in chain
+// order and part_rows[p] its row count (2^total entries). Returns 0, or 2 when
+// the reference throws (std::invalid_argument).
+int ref_radix_shuffle(const orc_relation *in, const uint32_t *out_types, const uint64_t *in_col_index, uint32_t n_out,
+                      const uint8_t *pass_bits, uint32_t n_passes, uint64_t threads, uint64_t *out_keys,
+                      void *const *out_cols, uint64_t *part_rows) {
+    try {
+        const phj::BlockStream s = toStream(*in);
+        phj::PayloadSchema schema;
+        std::vector<size_t> index;
+        for (uint32_t c = 0; c < n_out; ++c) {
+            schema.types.push_back(static_cast<phj::PayloadType>(out_types[c]));
+            index.push_back(in_col_index[c]);
+        }
+        phj::RadixConfig cfg;
+        cfg.pass_bits.assign(pass_bits, pass_bits + n_passes);
+        const phj::PartitionedShuffleOutput out = phj::radixShuffle(s, schema, index, cfg, threads);
+        size_t off = 0;
+        for (size_t p = 0; p < out.partitions; ++p) {
+            part_rows[p] = out.partition_rows[p];
+            for (const auto &blk : out.chains[p].blocks) {
+                std::memcpy(out_keys + off, blk.keys.data(), blk.rows * 8);
+                for (uint32_t c = 0; c < n_out; ++c) {
+                    const size_t sz = phj::payloadTypeSize(schema.types[c]);
+                    std::memcpy(static_cast<uint8_t *>(out_cols[c]) + off * sz, blk.payloads[c].raw(), blk.rows * sz);
+                }
+                off += blk.rows;
+            }
         }
     } catch (const std::exception &) {
         return 2;

@@ -659,7 +659,8 @@ LeafJoinPlan plan_of(const phj_tuning &t) {
 // caller asks for cached ones (the input is read once per pass).
 uint32_t radix_flags(const phj_tuning &t) {
     return ((t.flags & PHJ_TUNE_CACHED_LOADS) ? 0u : kRadixNtLoads) | ((t.flags & PHJ_TUNE_NT_STORES) ? kRadixNtStores : 0u) |
-           ((t.flags & PHJ_TUNE_HASH_EVERY_PASS) ? kRadixHashEveryPass : 0u);
+           ((t.flags & PHJ_TUNE_HASH_EVERY_PASS) ? kRadixHashEveryPass : 0u) |
+           ((t.flags & PHJ_TUNE_ATOMIC_RANK) ? kRadixAtomicRank : 0u);
 }
 
 // Largest leaf (build rows) the LDS join paths take for a build side whose
@@ -2876,6 +2877,46 @@ phj_status phj_radix_shuffle(phj_ctx *ctx, const phj_relation *input, const uint
         const uint32_t total = validate_pass_bits(pass_bits, n_passes);
         Scratch scr(ctx);
         const DevRel R = stage_relation(ctx, scr, *input);
+        Held keep(ctx);
+        const Partitioned P = radix_partition(ctx, R, nullptr, 1, 0, device_passes(total), keep, "shuffle");
+        fill_partitioned(ctx, P, keep, out);
+    });
+}
+
+// radixShuffle(input, out_schema, in_col_index, cfg, threads)
+// (src/RadixPartition.h:187-193, RadixPartition.cpp:546-566): the scatter
+// moves input column in_col_index[c] into output column c. The column map is
+// applied to the relation's column pointers, so the device passes are those
+// of phj_radix_shuffle; a column named twice is moved twice.
+phj_status phj_radix_shuffle_project(phj_ctx *ctx, const phj_relation *input, const uint32_t *out_types,
+                                     const uint32_t *in_col_index, uint32_t n_out_cols, const uint8_t *pass_bits,
+                                     uint32_t n_passes, phj_partitioned *out) {
+    return guard(ctx, [&] {
+        validate_rel(input);
+        if (out == nullptr)
+            invalid("null output");
+        const uint32_t total = validate_pass_bits(pass_bits, n_passes);
+        if (n_out_cols > PHJ_MAX_COLS)
+            invalid("at most 8 payload columns");
+        if (n_out_cols > 0 && (out_types == nullptr || in_col_index == nullptr))
+            invalid("radixShuffle: null out_schema or in_col_index");
+        phj_relation proj = *input;
+        proj.n_cols = n_out_cols;
+        for (uint32_t c = 0; c < n_out_cols; ++c) {
+            const uint32_t i = in_col_index[c];
+            if (i >= input->n_cols)
+                invalid("radixShuffle: in_col_index[" + std::to_string(c) + "] = " + std::to_string(i) +
+                        " names no input column");
+            // The reference moves the input column's bytes at the output
+            // type's width; widths that differ would read past or split its
+            // values, so the types must agree.
+            if (out_types[c] != input->cols[i].type)
+                invalid("radixShuffle: out_schema type of column " + std::to_string(c) +
+                        " differs from the type of input column " + std::to_string(i));
+            proj.cols[c] = input->cols[i];
+        }
+        Scratch scr(ctx);
+        const DevRel R = stage_relation(ctx, scr, proj);
         Held keep(ctx);
         const Partitioned P = radix_partition(ctx, R, nullptr, 1, 0, device_passes(total), keep, "shuffle");
         fill_partitioned(ctx, P, keep, out);

@@ -16,8 +16,18 @@
 //   k_colscan_chunk per (segment, chunk of tiles): exclusive column scan of the
 //                   tile rows, chunk totals
 //   k_colscan_seg   per segment: chunk bases, digit bases, output segment offsets
-//   k_scatter       LDS-atomic ranking inside the tile, then each column is
-//                   staged in LDS in digit order and leaves as contiguous runs
+//   k_scatter       stable ranking inside the tile (wave ballots per digit bit,
+//                   mbcnt, per-wave digit counts), then each column is staged
+//                   in LDS in digit order and leaves as contiguous runs
+//
+// Every pass is stable: inside a tile rows keep their input order within a
+// digit, tiles take their digit bases in tile order and segments in segment
+// order, so partition p holds its rows in input order. The reference's
+// radixShuffle yields the same order at any thread count: each thread
+// scatters a contiguous run of blocks in order and the per-partition chains
+// are concatenated in thread order (src/RadixPartition.cpp:144-205, :316-393).
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace phjd {
@@ -361,14 +371,34 @@ template <bool NT, class T> __device__ __forceinline__ void st(T *p, const T &v)
     }
 }
 
+constexpr int kScatWaves = kScatThreads / 64;
+constexpr int kWaveRows = kTile / kScatWaves; // rows of a tile ranked by one wave
+
 struct ScatterShared {
-    uint64_t stage[kTile];
+    union {
+        uint64_t stage[kTile];
+        // per-wave digit counts while ranking, then each wave's base per digit
+        uint16_t wcount[kScatWaves][kMaxDigits];
+    };
     uint16_t slot_digit[kTile];
     uint32_t cursor[kMaxDigits];
     uint32_t lstart[kMaxDigits];
     uint32_t gbase[kMaxDigits];
     uint32_t scan_tmp[kScatThreads / 64 + 1];
 };
+static_assert(sizeof(uint16_t) * kScatWaves * kMaxDigits <= sizeof(uint64_t) * kTile, "wave counts fit the stage");
+static_assert(kWaveRows < 65536, "a wave's digit count fits 16 bits");
+
+// Row of the tile held by thread tid in its r-th register. ATOM (atomic
+// ranking): rows r * kScatThreads + tid. Otherwise wave w holds rows
+// [w * kWaveRows, (w + 1) * kWaveRows), 64 consecutive rows per register.
+// Either way one wave's load touches 64 consecutive rows.
+template <bool ATOM> __device__ __forceinline__ uint32_t tile_row(int r, uint32_t tid) {
+    if constexpr (ATOM)
+        return r * kScatThreads + tid;
+    else
+        return (tid >> 6) * kWaveRows + r * 64 + (tid & 63u);
+}
 
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
@@ -397,23 +427,54 @@ __device__ __forceinline__ void write_staged(ScatterShared &sh, const T (&v)[kSc
     }
 }
 
-template <bool NTL, bool NTS, class T>
+template <bool NTL, bool NTS, bool ATOM, class T>
 __device__ __forceinline__ void scatter_column(ScatterShared &sh, const T *__restrict__ in, T *__restrict__ out,
                                                const TileLoc &L, const uint32_t (&slot)[kScatRpt]) {
+    // the row indices are recomputed per column from an opaque copy of the
+    // thread id: hoisted out of the column loop they would stay live in
+    // registers across every column and spill
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     T v[kScatRpt];
 #pragma unroll
     for (int r = 0; r < kScatRpt; ++r) {
-        const uint32_t idx = r * kScatThreads + threadIdx.x;
+        const uint32_t idx = tile_row<ATOM>(r, tid);
         v[r] = ld<NTL>(in + L.row0 + min(idx, L.len - 1)); // rows past the tile carry no slot
     }
     write_staged<NTS, T>(sh, v, out, L, slot);
+}
+
+// Stable rank of each of a wave's rows among the tile's rows of its digit
+// that precede it in the wave's range: lanes holding the same digit find each
+// other with one ballot per digit bit, the rank inside a 64-row step is the
+// count of such lanes below (mbcnt), and the wave's running count per digit
+// (its own LDS row, so no atomics) carries across its steps. The lowest lane
+// of each digit advances the count. Returns rank << kDeviceMaxPassBits | digit.
+__device__ __forceinline__ uint32_t wave_rank(uint16_t *__restrict__ wcnt, uint32_t d, bool valid, uint32_t bits) {
+    uint64_t same = __ballot(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < kDeviceMaxPassBits; ++b) {
+        if (b < bits) { // uniform over the workgroup
+            const bool one = (d >> b) & 1u;
+            const uint64_t m = __ballot(one);
+            same &= one ? m : ~m;
+        }
+    }
+    const uint32_t below = mbcnt(same);
+    const uint32_t base = wcnt[d];
+    if (valid && below == 0)
+        wcnt[d] = static_cast<uint16_t>(base + __popcll(same));
+    return ((base + below) << kDeviceMaxPassBits) | d;
 }
 
 // With DIG the scatter also writes the next pass's digit of every row
 // (hash bits [next_shift, next_shift + next_bits), at most 8) as a byte
 // column in output order, so the next pass's histogram need not re-read and
 // re-hash the keys.
-template <bool NTL, bool NTS, bool DIG>
+//
+// ATOM ranks rows with one LDS atomic each instead (not stable; kept as a
+// tuning variant for comparison).
+template <bool NTL, bool NTS, bool DIG, bool ATOM>
 __global__ __launch_bounds__(kScatThreads) __attribute__((amdgpu_waves_per_eu(8)))
 void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
                                                           uint64_t *__restrict__ keys_out, OutColSet cout,
@@ -431,32 +492,51 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
     const uint32_t B = 1u << pb.bits;
     uint32_t shift, bits;
     seg_window(pb, L.seg, shift, bits);
-    for (uint32_t i = threadIdx.x; i < B; i += kScatThreads)
-        sh.cursor[i] = 0;
+    if constexpr (ATOM) {
+        for (uint32_t i = threadIdx.x; i < B; i += kScatThreads)
+            sh.cursor[i] = 0;
+    } else {
+        uint32_t *w = reinterpret_cast<uint32_t *>(&sh.wcount[0][0]);
+        for (uint32_t i = threadIdx.x; i < kScatWaves * kMaxDigits / 2; i += kScatThreads)
+            w[i] = 0;
+    }
     __syncthreads();
 
-    // Rank each row inside its digit with one LDS atomic (rank << kDeviceMaxPassBits | digit).
+    // Rank each row inside its digit (rank << kDeviceMaxPassBits | digit).
     uint64_t key[kScatRpt];
     uint32_t slot[kScatRpt];
-    uint8_t next[kScatRpt];
 #pragma unroll
     for (int r = 0; r < kScatRpt; ++r) {
-        const uint32_t idx = r * kScatThreads + threadIdx.x;
+        const uint32_t idx = tile_row<ATOM>(r, threadIdx.x);
         key[r] = ld<NTL>(keys_in + L.row0 + min(idx, L.len - 1)); // past the tile: masked below
     }
+    const uint32_t wave = threadIdx.x >> 6;
 #pragma unroll
     for (int r = 0; r < kScatRpt; ++r) {
-        const uint32_t idx = r * kScatThreads + threadIdx.x;
-        slot[r] = kNoSlot;
-        const uint64_t h = int_hash64(key[r]);
-        if constexpr (DIG)
-            next[r] = static_cast<uint8_t>(digit_of(h, next_shift, next_bits));
-        if (idx < L.len) {
-            const uint32_t d = digit_of(h, shift, bits);
-            slot[r] = (atomicAdd(&sh.cursor[d], 1u) << kDeviceMaxPassBits) | d;
+        const uint32_t idx = tile_row<ATOM>(r, threadIdx.x);
+        const uint32_t d = digit_of(int_hash64(key[r]), shift, bits);
+        if constexpr (ATOM) {
+            slot[r] = idx < L.len ? (atomicAdd(&sh.cursor[d], 1u) << kDeviceMaxPassBits) | d : kNoSlot;
+        } else {
+            const uint32_t s = wave_rank(sh.wcount[wave], d, idx < L.len, bits);
+            slot[r] = idx < L.len ? s : kNoSlot;
         }
     }
     __syncthreads();
+    if constexpr (!ATOM) {
+        // wave counts -> each wave's base inside its digit; digit totals
+        if (threadIdx.x < B) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int w = 0; w < kScatWaves; ++w) {
+                const uint32_t c = sh.wcount[w][threadIdx.x];
+                sh.wcount[w][threadIdx.x] = static_cast<uint16_t>(acc);
+                acc += c;
+            }
+            sh.cursor[threadIdx.x] = acc;
+        }
+        __syncthreads();
+    }
     {
         const uint32_t c = threadIdx.x < B ? sh.cursor[threadIdx.x] : 0u;
         uint32_t total;
@@ -472,32 +552,41 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
     for (int r = 0; r < kScatRpt; ++r) {
         if (slot[r] != kNoSlot) {
             const uint32_t d = slot[r] & (kMaxDigits - 1u);
-            const uint32_t sl = sh.lstart[d] + (slot[r] >> kDeviceMaxPassBits);
+            uint32_t sl = sh.lstart[d] + (slot[r] >> kDeviceMaxPassBits);
+            if constexpr (!ATOM)
+                sl += sh.wcount[wave][d];
             sh.slot_digit[sl] = static_cast<uint16_t>(d);
             slot[r] = sl;
         }
     }
-    __syncthreads();
+    __syncthreads(); // the wave bases share LDS with the staging area below
 
     write_staged<NTS, uint64_t>(sh, key, keys_out, L, slot);
-    if constexpr (DIG)
+    if constexpr (DIG) {
+        // the next pass's digits, hashed again here rather than held in
+        // registers through the ranking
+        uint8_t next[kScatRpt];
+#pragma unroll
+        for (int r = 0; r < kScatRpt; ++r)
+            next[r] = static_cast<uint8_t>(digit_of(int_hash64(key[r]), next_shift, next_bits));
         write_staged<NTS, uint8_t>(sh, next, digits_out, L, slot);
+    }
     for (uint32_t c = 0; c < cin.n; ++c) {
         switch (col_type(cin, c)) {
         case 0:
-            scatter_column<NTL, NTS, uint8_t>(sh, static_cast<const uint8_t *>(col_ptr(cin, c)), static_cast<uint8_t *>(col_ptr(cout, c)),
+            scatter_column<NTL, NTS, ATOM, uint8_t>(sh, static_cast<const uint8_t *>(col_ptr(cin, c)), static_cast<uint8_t *>(col_ptr(cout, c)),
                                     L, slot);
             break;
         case 1:
-            scatter_column<NTL, NTS, uint16_t>(sh, static_cast<const uint16_t *>(col_ptr(cin, c)),
+            scatter_column<NTL, NTS, ATOM, uint16_t>(sh, static_cast<const uint16_t *>(col_ptr(cin, c)),
                                      static_cast<uint16_t *>(col_ptr(cout, c)), L, slot);
             break;
         case 2:
-            scatter_column<NTL, NTS, uint32_t>(sh, static_cast<const uint32_t *>(col_ptr(cin, c)),
+            scatter_column<NTL, NTS, ATOM, uint32_t>(sh, static_cast<const uint32_t *>(col_ptr(cin, c)),
                                      static_cast<uint32_t *>(col_ptr(cout, c)), L, slot);
             break;
         case 3:
-            scatter_column<NTL, NTS, uint64_t>(sh, static_cast<const uint64_t *>(col_ptr(cin, c)),
+            scatter_column<NTL, NTS, ATOM, uint64_t>(sh, static_cast<const uint64_t *>(col_ptr(cin, c)),
                                      static_cast<uint64_t *>(col_ptr(cout, c)), L, slot);
             break;
         case kTypeRowIndex: {
@@ -505,12 +594,12 @@ void k_scatter(const uint64_t *__restrict__ keys_in, ColSet cin,
             uint32_t v[kScatRpt];
 #pragma unroll
             for (int r = 0; r < kScatRpt; ++r)
-                v[r] = L.row0 + min(r * kScatThreads + threadIdx.x, L.len - 1);
+                v[r] = L.row0 + min(tile_row<ATOM>(r, threadIdx.x), L.len - 1);
             write_staged<NTS, uint32_t>(sh, v, static_cast<uint32_t *>(col_ptr(cout, c)), L, slot);
             break;
         }
         default:
-            scatter_column<NTL, NTS, ulonglong2>(sh, static_cast<const ulonglong2 *>(col_ptr(cin, c)),
+            scatter_column<NTL, NTS, ATOM, ulonglong2>(sh, static_cast<const ulonglong2 *>(col_ptr(cin, c)),
                                        static_cast<ulonglong2 *>(col_ptr(cout, c)), L, slot);
             break;
         }
@@ -582,10 +671,14 @@ void radix_pass(hipStream_t s, const uint64_t *keys_in, const ColSet &cin, uint6
     if (rows > 0) {
         TimedScope ts(tm, "radix_scatter", s);
         const bool ntl = flags & kRadixNtLoads, nts = flags & kRadixNtStores;
-        auto k = dig.out ? (ntl ? (nts ? k_scatter<true, true, true> : k_scatter<true, false, true>)
-                                : (nts ? k_scatter<false, true, true> : k_scatter<false, false, true>))
-                         : (ntl ? (nts ? k_scatter<true, true, false> : k_scatter<true, false, false>)
-                                : (nts ? k_scatter<false, true, false> : k_scatter<false, false, false>));
+        auto pick = [&](auto atom) {
+            constexpr bool A = decltype(atom)::value;
+            return dig.out ? (ntl ? (nts ? k_scatter<true, true, true, A> : k_scatter<true, false, true, A>)
+                                  : (nts ? k_scatter<false, true, true, A> : k_scatter<false, false, true, A>))
+                           : (ntl ? (nts ? k_scatter<true, true, false, A> : k_scatter<true, false, false, A>)
+                                  : (nts ? k_scatter<false, true, false, A> : k_scatter<false, false, false, A>));
+        };
+        auto k = (flags & kRadixAtomicRank) ? pick(std::true_type{}) : pick(std::false_type{});
         k<<<grid8(max_tiles), kScatThreads, 0, s>>>(keys_in, cin, keys_out, cout, seg_off, scr.seg_tile_first, n_seg, pb,
                                                      scr.hist, scr.block_sums, dig.out, dig.next_shift, dig.next_bits);
     }

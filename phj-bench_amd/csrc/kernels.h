@@ -77,6 +77,7 @@ uint64_t radix_scan_blocks(uint64_t elems);
 constexpr uint32_t kRadixNtLoads = 0x4;  // input columns read with nontemporal loads
 constexpr uint32_t kRadixNtStores = 0x8; // output columns written with nontemporal stores
 constexpr uint32_t kRadixHashEveryPass = 0x10; // every histogram hashes the keys (no digit column between passes)
+constexpr uint32_t kRadixAtomicRank = 0x40; // scatter ranks rows by LDS atomics (not stable) instead of wave ballots
 // Column type of a radix pass input that is not stored: each row's index in
 // the pass input (row0 + i), written out as a u32 column (type 2).
 constexpr uint32_t kTypeRowIndex = 5;

@@ -6,7 +6,8 @@ The names and argument meanings follow the reference's scheme functions:
     runPHJ(build, probe, cfg, threads)        (PHJ-PURE)     src/PHJ.h:31
     runPhjBep(build, probe, cfg, threads, bep_budget_mib)    src/PHJBep.h:97-98
     intHash64Batch(keys)                                     src/Hash.h:33-58
-    radixShuffle(input, cfg, threads)                        src/RadixPartition.h:187-199
+    radixShuffle(input, [out_schema, in_col_index,] cfg, threads)
+                                                             src/RadixPartition.h:187-199
     refineToLeaves(pass1, pass_bits)                         src/RadixPartition.h:393-399
 
 Errors map as in the reference: invalid radix configurations and schemas
@@ -168,6 +169,8 @@ _SIGNATURES = {
     "phj_set_mem_limit": ([C.c_void_p, C.c_uint64], C.c_int),
     "phj_int_hash64_batch": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int], C.c_int),
     "phj_radix_shuffle": ([C.c_void_p, C.POINTER(_Relation), C.c_void_p, C.c_uint32, C.POINTER(_Partitioned)], C.c_int),
+    "phj_radix_shuffle_project": ([C.c_void_p, C.POINTER(_Relation), C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                   C.c_uint32, C.POINTER(_Partitioned)], C.c_int),
     "phj_refine_to_leaves": ([C.c_void_p, C.POINTER(_Partitioned), C.c_void_p, C.c_uint32, C.POINTER(_Partitioned)],
                              C.c_int),
     "phj_partitioned_release": ([C.c_void_p, C.POINTER(_Partitioned)], C.c_int),
@@ -672,11 +675,26 @@ class Context:
         _raise(self._p, load_library().phj_int_hash64_batch(self._p, keys.ctypes.data, keys.size, out.ctypes.data, 0))
         return out
 
-    def radixShuffle(self, rel, cfg: RadixConfig, threads=1) -> Partitioned:
+    def radixShuffle(self, rel, cfg: RadixConfig, threads=1, out_schema=None, in_col_index=None) -> Partitioned:
+        """radixShuffle (src/RadixPartition.h:187-199). With out_schema and
+        in_col_index, output column c is input column in_col_index[c] (whose
+        type must be out_schema[c]); without them the columns are kept as they
+        are. Each partition holds its rows in input order."""
         out = _Partitioned()
         pb, n = _pass_array(cfg.pass_bits)
         r = rel._c()
-        _raise(self._p, load_library().phj_radix_shuffle(self._p, C.byref(r), pb, n, C.byref(out)))
+        if out_schema is None and in_col_index is None:
+            _raise(self._p, load_library().phj_radix_shuffle(self._p, C.byref(r), pb, n, C.byref(out)))
+            return Partitioned(self, out)
+        out_schema, in_col_index = list(out_schema or []), list(in_col_index or [])
+        if len(out_schema) != len(in_col_index):
+            raise ValueError("radixShuffle: out_schema and in_col_index size mismatch")  # src/RadixPartition.cpp:559-560
+        if any(i < 0 for i in in_col_index):
+            raise ValueError("radixShuffle: negative in_col_index")
+        types = (C.c_uint32 * max(1, len(out_schema)))(*out_schema)
+        index = (C.c_uint32 * max(1, len(in_col_index)))(*in_col_index)
+        _raise(self._p, load_library().phj_radix_shuffle_project(self._p, C.byref(r), types, index, len(out_schema), pb,
+                                                                 n, C.byref(out)))
         return Partitioned(self, out)
 
     def refineToLeaves(self, pass1: Partitioned, pass_bits) -> Partitioned:

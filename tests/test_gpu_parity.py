@@ -100,6 +100,55 @@ def test_radix_shuffle_partitions(ctx, pass_bits, n):
     assert np.array_equal(np.diff(off.astype(np.int64)), counts)
     # the multiset of rows is preserved
     assert np.array_equal(rows_multiset(out), rows_multiset(host_rel(rel)))
+    # and every partition holds its rows in input order, row for row as the
+    # oracle (pinned to the compiled reference's radixShuffle) places them
+    want, woff = O.radix_shuffle(rel, pass_bits)
+    assert np.array_equal(off.astype(np.int64), woff)
+    assert np.array_equal(out.keys, want.keys)
+    for a, b in zip(out.cols, want.cols):
+        assert np.array_equal(a, b)
+    part.release()
+
+
+@pytest.mark.parametrize("index", [[], [2], [3, 0], [1, 1, 2], [2, 3, 1, 0]])
+@pytest.mark.parametrize("pass_bits", [[8], [5, 5]])
+def test_radix_shuffle_projection(ctx, index, pass_bits):
+    """radixShuffle(input, out_schema, in_col_index, cfg) (src/RadixPartition.h:187-193):
+    columns dropped, reordered and repeated, row for row as the oracle."""
+    rng = np.random.default_rng(len(index) * 10 + len(pass_bits))
+    rel = random_relation(rng, 30_001, [G.U8, G.U64, G.U128, G.U32])
+    types = [rel.types[i] for i in index]
+    part = ctx.radixShuffle(host_rel(rel), G.RadixConfig(pass_bits), out_schema=types, in_col_index=index)
+    out, off = part.to_host()
+    want, woff = O.radix_shuffle(rel, pass_bits, types, index)
+    assert out.types == types
+    assert np.array_equal(off.astype(np.int64), woff) and np.array_equal(out.keys, want.keys)
+    for a, b in zip(out.cols, want.cols):
+        assert np.array_equal(a, b)
+    part.release()
+
+
+def test_radix_shuffle_projection_rejects_bad_maps(ctx):
+    rel = host_rel(random_relation(np.random.default_rng(2), 10, [G.U64, G.U16]))
+    for types, index in (([G.U64], [2]), ([G.U32], [0]), ([G.U64, G.U16], [0]), ([G.U64] * 9, [0] * 9)):
+        with pytest.raises(ValueError):
+            ctx.radixShuffle(rel, G.RadixConfig([8]), out_schema=types, in_col_index=index)
+
+
+@pytest.mark.parametrize("pass_bits", [[9], [8, 8], [1]])
+def test_radix_shuffle_stable_under_hot_keys(ctx, pass_bits):
+    """Most rows share a few keys (one digit takes most of every tile): rows
+    keep their input order inside each partition."""
+    rng = np.random.default_rng(99)
+    n = 200_003
+    keys = np.where(rng.random(n) < 0.9, rng.choice(np.array([7, 2**64 - 1, 0], dtype=np.uint64), n),
+                    rng.integers(0, 2**64, n, dtype=np.uint64)).astype(np.uint64)
+    rel = O.Relation(keys, [np.arange(n, dtype=np.uint32)], [G.U32])
+    part = ctx.radixShuffle(host_rel(rel), G.RadixConfig(pass_bits))
+    out, off = part.to_host()
+    want, woff = O.radix_shuffle(rel, pass_bits)
+    assert np.array_equal(off.astype(np.int64), woff)
+    assert np.array_equal(out.cols[0], want.cols[0])  # the row numbers: input order in every partition
     part.release()
 
 
@@ -136,6 +185,11 @@ def test_refine_to_leaves(ctx, pass_bits):
         assert np.array_equal(a, b)
     # and the whole relation's multiset
     assert np.array_equal(rows_multiset(out), rows_multiset(host_rel(rel)))
+    # stable passes: the leaves hold the rows exactly as one radixShuffle over all passes
+    want, woff = O.radix_shuffle(rel, pass_bits)
+    assert np.array_equal(off.astype(np.int64), woff) and np.array_equal(out.keys, want.keys)
+    for a, b in zip(out.cols, want.cols):
+        assert np.array_equal(a, b)
     for x in (p1, leaves):
         x.release()
 
@@ -232,7 +286,7 @@ def test_join_matches_reference(ctx, dsname, scheme, cfg):
 @pytest.mark.parametrize("flags", [dict(flags=1), dict(lds_leaf_rows=8), dict(ht_load_pct=80),
                                    dict(join_threads=256), dict(join_threads=1024), dict(probe_chunk_rows=1000),
                                    dict(flags=4), dict(flags=8), dict(flags=12), dict(flags=0x10),
-                                   dict(flags=0x12)])
+                                   dict(flags=0x12), dict(flags=0x40), dict(flags=0x50)])
 def test_phj_pure_tuning_paths(ctx, flags):
     ob, op = data()["zipf_multi"]
     want = O.serialize_sorted(O.reference_join(ob, op))

@@ -212,3 +212,79 @@ def test_oracle_matches_compiled_reference(dist):
     assert np.array_equal(rows, O.ref_reference_rows(b, p, rows.shape[1], rows.shape[0]))
     for scheme, pb in ((1, (8,)), (3, (4, 4)), (2, (4, 4))):
         assert O.ref_run(scheme, b, p, pb, threads=2, bep_budget_mib=1).checksum.tuple() == want
+
+
+# ---------------------------------------------------------------- radixShuffle (src/RadixPartition.cpp:546-577)
+def stable_passes_py(keys, pass_bits):
+    """Pass-by-pass bucketing in pure Python on a small input: each pass splits
+    every bucket of the previous pass by the next hash bits, appending rows in
+    order. Returns the row order and the bucket sizes."""
+    buckets = [list(range(len(keys)))]
+    consumed = 0
+    for bits in pass_bits:
+        shift = 64 - consumed - bits
+        nxt = []
+        for b in buckets:
+            sub = [[] for _ in range(1 << bits)]
+            for i in b:
+                sub[(O.int_hash64_py(int(keys[i])) >> shift) & ((1 << bits) - 1)].append(i)
+            nxt.extend(sub)
+        buckets = nxt
+        consumed += bits
+    return [i for b in buckets for i in b], [len(b) for b in buckets]
+
+
+@pytest.mark.parametrize("pass_bits", [[1], [3], [2, 3], [2, 2, 2], [5, 1]])
+def test_radix_shuffle_matches_pass_by_pass_bucketing(pass_bits):
+    rel, _ = small_case(sum(pass_bits), 700, 1, 50, [1, 4], [3])
+    got, off = O.radix_shuffle(rel, pass_bits)
+    order, sizes = stable_passes_py(rel.keys, pass_bits)
+    assert np.array_equal(got.keys, rel.keys[order])
+    assert np.array_equal(got.cols[1], rel.cols[1][order])
+    assert np.array_equal(np.diff(off), sizes)
+
+
+def rel_for_shuffle(seed, n):
+    rng = np.random.default_rng(seed)
+    keys = rng.integers(0, 2**64, n, dtype=np.uint64)
+    keys[: n // 4] = keys[n // 2: n // 2 + n // 4]  # repeated keys stay in input order
+    types = [0, 3, 4, 2]
+    cols = [rng.integers(0, 2**64, (n, 2), dtype=np.uint64) if t == 4 else
+            rng.integers(0, 256 ** O.TYPE_SIZE[t] - 1, n, dtype=O.NP_DTYPE[t], endpoint=True) for t in types]
+    return O.Relation(keys, cols, types)
+
+
+@needs_ref
+@pytest.mark.parametrize("n", [0, 1, 9_999, 10_001, 45_000])
+@pytest.mark.parametrize("pass_bits", [[8], [4, 4], [3, 3, 3], [7, 6]])
+@pytest.mark.parametrize("threads", [1, 3])
+def test_radix_shuffle_matches_compiled_reference(n, pass_bits, threads):
+    """Row for row, with the reference's 10,000-row blocks spread over threads."""
+    rel = rel_for_shuffle(n + len(pass_bits), n)
+    want, woff = O.ref_radix_shuffle(rel, pass_bits, threads=threads)
+    got, off = O.radix_shuffle(rel, pass_bits)
+    assert np.array_equal(off, woff)
+    assert np.array_equal(got.keys, want.keys)
+    for a, b in zip(got.cols, want.cols):
+        assert np.array_equal(a, b)
+
+
+@needs_ref
+@pytest.mark.parametrize("index", [[], [2], [3, 0], [1, 1, 2], [2, 3, 1, 0]])
+def test_radix_shuffle_projection_matches_compiled_reference(index):
+    rel = rel_for_shuffle(11, 23_456)
+    types = [rel.types[i] for i in index]
+    want, woff = O.ref_radix_shuffle(rel, [5, 5], types, index, threads=2)
+    got, off = O.radix_shuffle(rel, [5, 5], types, index)
+    assert np.array_equal(off, woff) and np.array_equal(got.keys, want.keys)
+    assert got.types == types == want.types
+    for a, b in zip(got.cols, want.cols):
+        assert np.array_equal(a, b)
+
+
+@needs_ref
+def test_reference_radix_shuffle_rejects_bad_configs():
+    rel = rel_for_shuffle(1, 100)
+    for bad in ([], [0]):  # src/RadixPartition.cpp:552-556
+        with pytest.raises(ValueError):
+            O.ref_radix_shuffle(rel, bad)
