@@ -410,12 +410,21 @@ __device__ __forceinline__ void round_layout(const uint16_t (&h)[U], const uint3
     L.total = running + chain_total;
 }
 
+// Emit kernels' workgroup size and probe rows per thread per round (both
+// emit kernels derive the same round layout from them). 512 x 4 keeps a
+// round at 2,048 probe rows with eight waves per workgroup: with the round's
+// probe rows staged in LDS (64 KB per workgroup on the skewed workload), two
+// workgroups give a SIMD four waves, where 256 x 8 gave it two.
+constexpr int kEmitThreads = 512;
+constexpr int kEmitSlots = 4;
 // Multi-match rows are written by the whole workgroup, kEmitVec rows per
 // thread per step, at consecutive output positions.
 constexpr int kEmitVec = 4;
 // LDS for the staged build columns of the multi-match emit (beyond it they
 // are gathered from HBM).
 constexpr size_t kEmitRcolLds = 64 * 1024;
+// LDS for a round's probe keys and leading probe columns (multi-match emit).
+constexpr size_t kEmitScolLds = 32 * 1024;
 
 // LDS of the emit kernels for the item's groups: perm and glen, nr each.
 __host__ __device__ constexpr size_t emit_chain_lds(uint32_t lds_max_rows) {
@@ -426,6 +435,20 @@ __host__ __device__ constexpr size_t emit_chain_lds(uint32_t lds_max_rows) {
 // lds_max_rows elements of its width, 16-byte aligned.
 __host__ __device__ constexpr size_t emit_rcol_bytes(uint32_t lds_max_rows, uint32_t width) {
     return (static_cast<size_t>(lds_max_rows) * width + 15) & ~size_t{15};
+}
+
+// LDS of every column of a set staged for `rows` rows.
+inline size_t emit_rcols_bytes(const ColSet &cs, uint32_t rows) {
+    size_t b = 0;
+    for (uint32_t c = 0; c < cs.n; ++c)
+        b += emit_rcol_bytes(rows, type_size(cs.types[c]));
+    return b;
+}
+__device__ __forceinline__ size_t emit_rcols_bytes_dev(const ColSet &cs, uint32_t rows) {
+    size_t b = 0;
+    for (uint32_t c = 0; c < cs.n; ++c)
+        b += emit_rcol_bytes(rows, type_size(col_type(cs, c)));
+    return b;
 }
 
 // Copies rows [r0, r0 + nr) of every build column to LDS (workgroup-wide).
@@ -463,13 +486,41 @@ __device__ __forceinline__ void stage_rcols(const ColSet &rc, uint32_t r0, uint3
     }
 }
 
-// gather_set with the build columns read from their LDS copy (idx leaf-local).
+// Copies rows [r0, r0 + n) of the probe keys and of the first ncols probe
+// columns to LDS (workgroup-wide): the keys first (rows u64), then column c
+// at emit_rcol_bytes(rows, width) steps.
+template <int THREADS>
+__device__ __forceinline__ void stage_scols(const uint64_t *keys, const ColSet &sc, uint32_t ncols, uint32_t r0, uint32_t n,
+                                            uint32_t rows, unsigned char *dst) {
+    for (uint32_t i = threadIdx.x; i < n; i += THREADS)
+        reinterpret_cast<uint64_t *>(dst)[i] = keys[r0 + i];
+    ColSet head = sc;
+    head.n = ncols;
+    stage_rcols<THREADS>(head, r0, n, rows, dst + emit_rcol_bytes(rows, 8));
+}
+
+// gather_set with the first `nlds` columns read from their LDS copy (idx
+// relative to the copy) and the others from memory (idx + row0).
 template <int U>
 __device__ __forceinline__ void gather_set_lds(const unsigned char *base, uint32_t lds_max_rows, const ColSet &in,
                                                const OutColSet &out, const uint32_t (&idx)[U], const uint64_t (&pos)[U],
-                                               const bool (&on)[U]) {
+                                               const bool (&on)[U], uint32_t nlds = 0xFFFFFFFFu, uint32_t row0 = 0) {
     size_t off = 0;
     for (uint32_t c = 0; c < in.n; ++c) {
+        if (c >= nlds) {
+            uint32_t g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                g[u] = idx[u] + row0;
+            switch (col_type(in, c)) {
+            case 0: gather_col<U>(cptr<uint8_t>(in, c), optr<uint8_t>(out, c), g, pos, on); break;
+            case 1: gather_col<U>(cptr<uint16_t>(in, c), optr<uint16_t>(out, c), g, pos, on); break;
+            case 2: gather_col<U>(cptr<uint32_t>(in, c), optr<uint32_t>(out, c), g, pos, on); break;
+            case 3: gather_col<U>(cptr<uint64_t>(in, c), optr<uint64_t>(out, c), g, pos, on); break;
+            default: gather_col<U>(cptr<ulonglong2>(in, c), optr<ulonglong2>(out, c), g, pos, on); break;
+            }
+            continue;
+        }
         const unsigned char *src = base + off;
         off += emit_rcol_bytes(lds_max_rows, type_size(col_type(in, c)));
         switch (col_type(in, c)) {
@@ -490,7 +541,11 @@ __device__ __forceinline__ void gather_set_lds(const unsigned char *base, uint32
 // issued before its stores); CHAINS == true derives the same layout and
 // writes only the multi-match rows: each thread publishes where its groups
 // land in the round's output, and every thread then takes consecutive output
-// rows, finding each row's group by binary search.
+// rows, finding each row's group by binary search. It reads the leaf's build
+// columns and the round's probe keys and leading probe columns from LDS
+// copies (when they fit), so its emission loop issues stores and no memory
+// loads: on gfx9 one counter orders a wave's loads and stores, and a load
+// there would wait for every store issued before it.
 template <int THREADS, int U, bool CHAINS>
 __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -505,6 +560,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     uint32_t *seg_end = reinterpret_cast<uint32_t *>(smem + emit_chain_lds(a.lds_max_rows)); // [THREADS * U]
     uint16_t *seg_gs = reinterpret_cast<uint16_t *>(smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 4);
     unsigned char *rlds = smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 6; // staged build columns
+    unsigned char *slds = rlds + (CHAINS && a.r_lds ? emit_rcols_bytes_dev(a.r_cols, a.lds_max_rows) : 0); // round's probe rows
     {
         const uint16_t *src = a.next16 + item_chain_base(a, it);
         for (uint32_t i = threadIdx.x; i < it.nr; i += THREADS) {
@@ -567,6 +623,12 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                 seg_end[threadIdx.x * U + u] = end;
                 seg_gs[threadIdx.x * U + u] = h[u] & kRowMask;
             }
+            // The round's probe keys (and the leading probe columns that fit)
+            // go to LDS with coalesced loads: the emission below then issues
+            // no memory loads, only stores, so no load waits behind them.
+            if (a.s_lds)
+                stage_scols<THREADS>(a.s_keys, a.s_cols, a.s_lds - 1, base, min(static_cast<uint32_t>(THREADS * U), it.s1 - base),
+                                     THREADS * U, slds);
             __syncthreads();
             const uint64_t pos0 = round_base + L.chain_start;
             for (uint32_t k0 = 0; k0 < L.chain_total; k0 += THREADS * kEmitVec) {
@@ -583,14 +645,20 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                     for (uint32_t step = THREADS * U / 2; step > 0; step >>= 1)
                         sg += seg_end[sg + step - 1] <= oc ? step : 0u;
                     const uint32_t e = oc - (sg ? seg_end[sg - 1] : 0u);
-                    sidx[v] = base + (sg % U) * THREADS + sg / U;
-                    ridx[v] = perm[seg_gs[sg] + e]; // leaf-local
+                    sidx[v] = (sg % U) * THREADS + sg / U; // round-relative
+                    ridx[v] = perm[seg_gs[sg] + e];         // leaf-local
                     opos[v] = pos0 + o;
                 }
                 uint64_t kv[kEmitVec];
+                if (a.s_lds) {
 #pragma unroll
-                for (int v = 0; v < kEmitVec; ++v)
-                    kv[v] = a.s_keys[sidx[v]];
+                    for (int v = 0; v < kEmitVec; ++v)
+                        kv[v] = reinterpret_cast<const uint64_t *>(slds)[sidx[v]];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < kEmitVec; ++v)
+                        kv[v] = a.s_keys[base + sidx[v]];
+                }
 #pragma unroll
                 for (int v = 0; v < kEmitVec; ++v)
                     if (on[v])
@@ -603,7 +671,8 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                         ridx[v] += it.r0;
                     gather_set<kEmitVec>(a.r_cols, a.out.left, ridx, opos, on);
                 }
-                gather_set<kEmitVec>(a.s_cols, a.out.right, sidx, opos, on);
+                gather_set_lds<kEmitVec>(slds + emit_rcol_bytes(THREADS * U, 8), THREADS * U, a.s_cols, a.out.right, sidx,
+                                         opos, on, a.s_lds ? a.s_lds - 1 : 0u, base);
             }
             __syncthreads(); // seg_end and seg_gs are rewritten by the next round
         }
@@ -1644,23 +1713,41 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
     const size_t lds_chains = emit_chain_lds(a.lds_max_rows);
     {
         TimedScope ts(tm, "leaf_emit", s);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<256, 8, false>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<kEmitThreads, kEmitSlots, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_chains));
-        k_leaf_emit<256, 8, false><<<checked_grid(grid8(max_work), 256, "leaf_emit"), 256, lds_chains, s>>>(a);
+        k_leaf_emit<kEmitThreads, kEmitSlots, false>
+            <<<checked_grid(grid8(max_work), kEmitThreads, "leaf_emit"), kEmitThreads, lds_chains, s>>>(a);
     }
     if (chains) {
         TimedScope ts(tm, "leaf_emit_chains", s);
         // Build columns go to LDS when they fit beside the groups: the chain
         // rows then gather them from LDS instead of from random HBM lines.
-        size_t rbytes = 0;
-        for (uint32_t c = 0; c < a.r_cols.n; ++c)
-            rbytes += emit_rcol_bytes(a.lds_max_rows, type_size(a.r_cols.types[c]));
+        const size_t rbytes = emit_rcols_bytes(a.r_cols, a.lds_max_rows);
         LeafJoinArgs b = a;
         b.r_lds = a.r_cols.n > 0 && rbytes <= kEmitRcolLds ? 1u : 0u;
-        const size_t lds = lds_chains + 8 * 256 * (4 + 2) + (b.r_lds ? rbytes : 0);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<256, 8, true>),
+        // Each round's probe keys and the leading probe columns that fit
+        // kEmitScolLds go to LDS.
+        constexpr uint32_t kRoundRows = kEmitThreads * kEmitSlots;
+        size_t sbytes = emit_rcol_bytes(kRoundRows, 8);
+        b.s_lds = sbytes <= kEmitScolLds ? 1u : 0u;
+        if (!b.s_lds)
+            sbytes = 0;
+        while (b.s_lds && b.s_lds - 1 < a.s_cols.n &&
+               sbytes + emit_rcol_bytes(kRoundRows, type_size(a.s_cols.types[b.s_lds - 1])) <= kEmitScolLds)
+            sbytes += emit_rcol_bytes(kRoundRows, type_size(a.s_cols.types[b.s_lds++ - 1]));
+        constexpr size_t kLdsMax = 160 * 1024;
+        const size_t fixed = lds_chains + kRoundRows * (4 + 2);
+        if (fixed + (b.r_lds ? rbytes : 0) + sbytes > kLdsMax) {
+            b.s_lds = 0; // probe rows gathered from memory
+            sbytes = 0;
+        }
+        if (fixed + (b.r_lds ? rbytes : 0) > kLdsMax)
+            b.r_lds = 0;
+        const size_t lds = fixed + (b.r_lds ? rbytes : 0) + sbytes;
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_leaf_emit<kEmitThreads, kEmitSlots, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        k_leaf_emit<256, 8, true><<<checked_grid(grid8(max_work), 256, "leaf_emit"), 256, lds, s>>>(b);
+        k_leaf_emit<kEmitThreads, kEmitSlots, true>
+            <<<checked_grid(grid8(max_work), kEmitThreads, "leaf_emit"), kEmitThreads, lds, s>>>(b);
     }
 }
 

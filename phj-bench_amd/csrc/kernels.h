@@ -146,6 +146,8 @@ struct LeafJoinArgs {
     OutSink out;                // emit pass: rows land at out_base + item_off[item] + ...
     uint64_t out_base;
     uint32_t r_lds;             // multi-match emit: nonzero when the leaf's build columns are staged in LDS
+    uint32_t s_lds;             // multi-match emit: nonzero when each round's probe keys are staged in LDS,
+                                // with the first s_lds - 1 probe columns
     // Batched joins (fused path): leaf -> join, and per join an output cursor,
     // the end of its output region (capacity) and a repeated-key flag. When
     // leaf_join is null the fused join uses out.cursor, out.cap, chain_flag.
