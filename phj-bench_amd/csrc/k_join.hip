@@ -1354,9 +1354,15 @@ __device__ __forceinline__ uint64_t ref_pack(const HtArgs &a, uint64_t row, uint
     return static_cast<uint64_t>(bno) | (static_cast<uint64_t>(rno) << 32);
 }
 
+// Bit of an HBM table cell's row_no set once a second build row of the cell's
+// key is inserted: probes of keys without it take the head as the only match
+// and skip the chain walk (one random read of next[] less per matched row).
+// Row numbers stay below 2^31 (block rows, leaf rows).
+constexpr uint32_t kRefMore = 0x80000000u;
+
 __device__ __forceinline__ uint32_t ref_row(const HtArgs &a, uint64_t ref) {
     const uint32_t bno = static_cast<uint32_t>(ref);
-    const uint32_t rno = static_cast<uint32_t>(ref >> 32);
+    const uint32_t rno = static_cast<uint32_t>(ref >> 32) & ~kRefMore;
     if (bno == kInvalidRow)
         return kInvalidRow;
     if (a.mode == kHtLeaves)
@@ -1418,6 +1424,10 @@ __global__ __launch_bounds__(256) void k_ht_build(HtArgs a) {
                                                      static_cast<unsigned long long>(k));
             if (old == kEmptyKey || old == k) {
                 prev = atomicExch(reinterpret_cast<unsigned long long *>(&c->block_no), static_cast<unsigned long long>(myref));
+                // every exchange that displaces a row is followed by its own
+                // OR, so the flag ends set exactly when the key has 2+ rows
+                if (static_cast<uint32_t>(prev) != kInvalidRow)
+                    atomicOr(&c->row_no, kRefMore);
                 break;
             }
             pos = (pos + 1) & mask;
@@ -1459,7 +1469,7 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
     for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * THREADS * U; base < n; base += per_round) {
         uint64_t key[U], prow[U], cb[U], cm[U], cp[U];
         uint32_t head[U], len[U];
-        uint32_t pend = 0;
+        uint32_t pend = 0, many = 0; // many: rows whose chain must be walked
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint64_t g = base + static_cast<uint64_t>(u) * THREADS + threadIdx.x;
@@ -1471,10 +1481,12 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
             ht_home(a, int_hash64(key[u]), table, cb[u], cm[u], cp[u]);
             head[u] = kInvalidRow;
             if (g < n) {
-                if (key[u] == kEmptyKey)
+                if (key[u] == kEmptyKey) {
                     head[u] = a.ht.side_head[table];
-                else
+                    many |= 1u << u;
+                } else {
                     pend |= 1u << u;
+                }
             }
         }
         while (pend) {
@@ -1488,6 +1500,8 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
                     continue;
                 if (c[u].key == key[u]) {
                     head[u] = ref_row(a, static_cast<uint64_t>(c[u].block_no) | (static_cast<uint64_t>(c[u].row_no) << 32));
+                    if (c[u].row_no & kRefMore)
+                        many |= 1u << u;
                     pend &= ~(1u << u);
                 } else if (c[u].key == kEmptyKey) {
                     pend &= ~(1u << u);
@@ -1496,15 +1510,20 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
                 }
             }
         }
-        // chain lengths, all chains of the thread one step per iteration
+        // chain lengths, all chains of the thread one step per iteration;
+        // a key with one build row has length 1 without a walk
         uint32_t x[U];
         uint32_t walking = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             x[u] = head[u];
             len[u] = 0;
-            if (x[u] != kInvalidRow)
-                walking |= 1u << u;
+            if (x[u] != kInvalidRow) {
+                if ((many >> u) & 1u)
+                    walking |= 1u << u;
+                else
+                    len[u] = 1;
+            }
         }
         while (walking) {
             uint32_t nx[U];
