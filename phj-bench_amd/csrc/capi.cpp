@@ -1140,7 +1140,13 @@ void run_chj(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, phj_joi
     uint64_t *d_off = scr.get<uint64_t>(kTables + 1);
     HIPCHK(hipMemcpyAsync(d_off, cell_off.data(), (kTables + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
     GlobalHT ht{};
-    ht.cells = scr.get<Cell16>(cell_off[kTables]);
+    // Cells of 32 or 64 bytes carry a one-row key's build columns inline, so
+    // its probes read them from the cell they just fetched instead of from
+    // random build rows; wider payloads keep 16-byte cells.
+    ht.cell_shift = ht_wide_shift(R.cols);
+    if (ht.cell_shift)
+        ht.pay = R.cols;
+    ht.cells = scr.get<Cell16>(cell_off[kTables] << (ht.cell_shift ? ht.cell_shift - 4 : 0));
     ht.cell_off = d_off;
     ht.next = scr.get<uint32_t>(R.rows);
     ht.side_head = scr.get<uint32_t>(kTables);

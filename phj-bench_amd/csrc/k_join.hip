@@ -1308,6 +1308,43 @@ __global__ __launch_bounds__(THREADS) void k_prebuilt_join(LeafJoinArgs a, uint6
 
 // ---------------------------------------------------------------- HBM tables
 
+// Build columns of one-row keys from their wide cells: column c of cell ci[u]
+// (byte (ci << shift) + its inline offset) to output row pos[u].
+template <int U, class T>
+__device__ __forceinline__ void gather_cell_col(const unsigned char *src, uint32_t shift, T *__restrict__ dst,
+                                                const uint64_t (&ci)[U], const uint64_t (&pos)[U], const bool (&on)[U]) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        v[u] = *reinterpret_cast<const T *>(src + ((on[u] ? ci[u] : 0u) << shift));
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (on[u])
+            dst[pos[u]] = v[u];
+}
+
+template <int U>
+__device__ __forceinline__ void gather_cells(const GlobalHT &ht, const ColSet &in, const OutColSet &out,
+                                             const uint64_t (&ci)[U], const uint64_t (&pos)[U], const bool (&on)[U]) {
+    const unsigned char *cells = reinterpret_cast<const unsigned char *>(ht.cells);
+    const uint32_t shift = ht.cell_shift;
+    uint32_t off = 16;
+    for (uint32_t c = 0; c < in.n; ++c) {
+        const uint32_t t = col_type(in, c);
+        const uint32_t w = type_size(t);
+        off = ht_pay_at(off, w);
+        const unsigned char *src = cells + off;
+        switch (t) {
+        case 0: gather_cell_col<U>(src, shift, optr<uint8_t>(out, c), ci, pos, on); break;
+        case 1: gather_cell_col<U>(src, shift, optr<uint16_t>(out, c), ci, pos, on); break;
+        case 2: gather_cell_col<U>(src, shift, optr<uint32_t>(out, c), ci, pos, on); break;
+        case 3: gather_cell_col<U>(src, shift, optr<uint64_t>(out, c), ci, pos, on); break;
+        default: gather_cell_col<U>(src, shift, optr<ulonglong2>(out, c), ci, pos, on); break;
+        }
+        off += w;
+    }
+}
+
 struct HtArgs {
     GlobalHT ht;
     HtMode mode;
@@ -1370,18 +1407,46 @@ __device__ __forceinline__ uint32_t ref_row(const HtArgs &a, uint64_t ref) {
     return bno * a.ht.block_rows + rno;
 }
 
-__global__ void k_ht_init(Cell16 *__restrict__ cells, uint64_t n, uint32_t *__restrict__ side_head, uint32_t n_tables) {
+__device__ __forceinline__ uint32_t cell_shift_of(const GlobalHT &ht) { return ht.cell_shift ? ht.cell_shift : 4u; }
+__device__ __forceinline__ Cell16 *cell_at(const GlobalHT &ht, uint64_t i) {
+    return reinterpret_cast<Cell16 *>(reinterpret_cast<unsigned char *>(ht.cells) + (i << cell_shift_of(ht)));
+}
+
+__global__ void k_ht_init(GlobalHT ht, uint64_t n) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         Cell16 c;
         c.key = kEmptyKey;
         c.block_no = kInvalidRow;
         c.row_no = 0;
-        cells[i] = c;
+        *cell_at(ht, i) = c;
     }
     const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g < n_tables)
-        side_head[g] = kInvalidRow;
+    if (g < ht.n_tables)
+        ht.side_head[g] = kInvalidRow;
+}
+
+// Copies build row `row` of the payload columns into a wide cell.
+__device__ __forceinline__ void put_payload(const GlobalHT &ht, Cell16 *cell, uint64_t row) {
+    unsigned char *base = reinterpret_cast<unsigned char *>(cell);
+    uint32_t off = 16;
+#pragma unroll 1
+    for (uint32_t c = 0; c < ht.pay.n; ++c) {
+        const uint32_t t = col_type(ht.pay, c);
+        const uint32_t w = type_size(t);
+        off = ht_pay_at(off, w);
+        const void *src = col_ptr(ht.pay, c);
+        switch (t) {
+        case 0: *reinterpret_cast<uint8_t *>(base + off) = static_cast<const uint8_t *>(src)[row]; break;
+        case 1: *reinterpret_cast<uint16_t *>(base + off) = static_cast<const uint16_t *>(src)[row]; break;
+        case 2: *reinterpret_cast<uint32_t *>(base + off) = static_cast<const uint32_t *>(src)[row]; break;
+        case 3: *reinterpret_cast<uint64_t *>(base + off) = static_cast<const uint64_t *>(src)[row]; break;
+        default:
+            *reinterpret_cast<ulonglong2 *>(base + off) = static_cast<const ulonglong2 *>(src)[row];
+            break;
+        }
+        off += w;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_chj_counts(const uint64_t *__restrict__ keys, uint64_t n, uint32_t n_tables,
@@ -1418,7 +1483,7 @@ __global__ __launch_bounds__(256) void k_ht_build(HtArgs a) {
             continue;
         }
         while (true) {
-            Cell16 *c = &a.ht.cells[base + pos];
+            Cell16 *c = cell_at(a.ht, base + pos);
             const unsigned long long old = atomicCAS(reinterpret_cast<unsigned long long *>(&c->key),
                                                      static_cast<unsigned long long>(kEmptyKey),
                                                      static_cast<unsigned long long>(k));
@@ -1428,6 +1493,8 @@ __global__ __launch_bounds__(256) void k_ht_build(HtArgs a) {
                 // OR, so the flag ends set exactly when the key has 2+ rows
                 if (static_cast<uint32_t>(prev) != kInvalidRow)
                     atomicOr(&c->row_no, kRefMore);
+                else if (a.ht.pay.n)
+                    put_payload(a.ht, c, row); // the key's first row: read only while it stays the only one
                 break;
             }
             pos = (pos + 1) & mask;
@@ -1443,7 +1510,7 @@ __device__ __forceinline__ uint32_t ht_lookup(const HtArgs &a, uint64_t k, uint3
     if (k == kEmptyKey)
         return a.ht.side_head[table];
     while (true) {
-        const Cell16 c = a.ht.cells[base + pos];
+        const Cell16 c = *cell_at(a.ht, base + pos);
         if (c.key == k)
             return ref_row(a, static_cast<uint64_t>(c.block_no) | (static_cast<uint64_t>(c.row_no) << 32));
         if (c.key == kEmptyKey)
@@ -1493,7 +1560,7 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
             Cell16 c[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                c[u] = a.ht.cells[cb[u] + cp[u]];
+                c[u] = *cell_at(a.ht, cb[u] + cp[u]);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (!((pend >> u) & 1u))
@@ -1502,7 +1569,7 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
                     head[u] = ref_row(a, static_cast<uint64_t>(c[u].block_no) | (static_cast<uint64_t>(c[u].row_no) << 32));
                     if (c[u].row_no & kRefMore)
                         many |= 1u << u;
-                    pend &= ~(1u << u);
+                    pend &= ~(1u << u); // cb + cp now names the matched cell
                 } else if (c[u].key == kEmptyKey) {
                     pend &= ~(1u << u);
                 } else {
@@ -1567,7 +1634,23 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
         for (int u = 0; u < U; ++u)
             if (one[u])
                 out.keys[opos[u]] = key[u];
-        gather_set<U>(rc, out.left, ridx, opos, one);
+        if (a.ht.pay.n) {
+            // one-row keys: build columns from the matched cell (just read)
+            uint64_t ci[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                ci[u] = one[u] && !((many >> u) & 1u) ? cb[u] + cp[u] : 0u;
+            bool fromcell[U], fromcols[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                fromcell[u] = one[u] && !((many >> u) & 1u);
+                fromcols[u] = one[u] && ((many >> u) & 1u); // a one-row side chain (key kEmptyKey)
+            }
+            gather_cells<U>(a.ht, rc, out.left, ci, opos, fromcell);
+            gather_set<U>(rc, out.left, ridx, opos, fromcols);
+        } else {
+            gather_set<U>(rc, out.left, ridx, opos, one);
+        }
         gather_set<U>(sc, out.right, sidx, opos, one);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1783,7 +1866,7 @@ void ht_init(hipStream_t s, const GlobalHT &ht, uint64_t total_cells, uint64_t, 
     TimedScope ts(tm, "ht_init", s);
     const uint64_t n = std::max<uint64_t>(total_cells, ht.n_tables);
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 256 * 64));
-    k_ht_init<<<grid, 256, 0, s>>>(ht.cells, total_cells, ht.side_head, ht.n_tables);
+    k_ht_init<<<grid, 256, 0, s>>>(ht, total_cells);
 }
 
 static HtArgs make_args(const GlobalHT &ht, HtMode mode, const HtRows &r) {

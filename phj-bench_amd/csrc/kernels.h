@@ -249,7 +249,7 @@ void group_join(hipStream_t s, const GroupImage &g, const uint64_t *keys, const 
                 const OutSink &out, KernelTimer *tm);
 
 struct GlobalHT {
-    Cell16 *cells;
+    Cell16 *cells;            // cell i at byte (i << cell_shift): a Cell16 header, then payload
     const uint64_t *cell_off; // n_tables + 1
     uint32_t *next;           // chain successor per build row (row-index space)
     uint32_t *side_head;      // per table: head row of key == kEmptyKey
@@ -257,7 +257,28 @@ struct GlobalHT {
     uint32_t block_rows;      // RowRefCell{row / block_rows, row % block_rows}
     // kHtLeaves: RowRefCell{leaf, row - build_seg_off[leaf]}
     const uint32_t *build_seg_off;
+    // Cells wider than 16 bytes (cell_shift 5 or 6; 0 means 16-byte cells)
+    // also hold the build columns `pay` of a key's first row after the header,
+    // at ht_pay_next offsets; they are read only for keys with one build row.
+    uint32_t cell_shift;
+    ColSet pay;
 };
+// Offset of the next inline payload column of width w after byte `off` of a
+// cell (16-byte columns 16-aligned, others aligned to their width).
+__host__ __device__ inline uint32_t ht_pay_at(uint32_t off, uint32_t w) {
+    const uint32_t a = w >= 16 ? 16u : w;
+    return (off + a - 1) & ~(a - 1);
+}
+// log2 of the cell size that holds a 16-byte header and payload columns of
+// these widths inline, or 0 when they need more than 64 bytes.
+inline uint32_t ht_wide_shift(const ColSet &cs) {
+    if (cs.n == 0)
+        return 0;
+    uint32_t off = 16;
+    for (uint32_t c = 0; c < cs.n; ++c)
+        off = ht_pay_at(off, type_size(cs.types[c])) + type_size(cs.types[c]);
+    return off <= 32 ? 5u : off <= 64 ? 6u : 0u;
+}
 
 // Largest segment (rows) of an offsets array, atomically maxed into *out.
 void leaf_max_rows(hipStream_t s, const uint32_t *off, uint32_t n, uint32_t *out);
