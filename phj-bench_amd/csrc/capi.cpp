@@ -1152,11 +1152,31 @@ void run_chj(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, phj_joi
     ht.side_head = scr.get<uint32_t>(kTables);
     ht.n_tables = kTables;
     ht.block_rows = PHJ_PIPELINE_BLOCK_ROWS;
+    ht.dups = scr.get<uint32_t>(1);
+    launch_fill_u32(ctx->stream, ht.dups, 1, 0);
     ht_init(ctx->stream, ht, cell_off[kTables], R.rows, ctx->tm());
     HtRows hr{};
     hr.keys = R.keys;
     hr.rows = R.rows;
     ht_build(ctx->stream, ht, kHtChj, hr, R.rows, ctx->tm(), "chj_build");
+    if (read_scalar(ctx, ht.dups)) {
+        // Repeated keys: every chain is laid out once as a contiguous group of
+        // build rows, so no probe walks next[] (src/JoinOps.h:260-271 emits
+        // the same rows per match).
+        const uint64_t cells = cell_off[kTables];
+        uint32_t *goff = scr.get<uint32_t>(cells + 1);
+        ht_group_count(ctx->stream, ht, kHtChj, cells, goff, ctx->tm());
+        uint32_t *gtmp = scr.get<uint32_t>(radix_scan_blocks(cells + 1) + 1);
+        scan_u32(ctx->stream, goff, cells + 1, nullptr, 1, gtmp, ctx->tm(), "ht_group");
+        const uint32_t grows = read_scalar(ctx, goff + cells);
+        ht.grp.n = R.cols.n;
+        for (uint32_t c = 0; c < R.cols.n; ++c) {
+            ht.grp.types[c] = R.cols.types[c];
+            ht.grp.ptr[c] = scr.get<unsigned char>(std::max<uint64_t>(grows, 1) * type_size(R.cols.types[c]));
+        }
+        ht.grouped = 1;
+        ht_group_fill(ctx->stream, ht, kHtChj, cells, goff, R.cols, ctx->tm());
+    }
     HIPCHK(hipEventRecord(eb.b, ctx->stream));
     HIPCHK(hipEventRecord(ep.a, ctx->stream));
     HtRows hp{};

@@ -1491,15 +1491,61 @@ __global__ __launch_bounds__(256) void k_ht_build(HtArgs a) {
                 prev = atomicExch(reinterpret_cast<unsigned long long *>(&c->block_no), static_cast<unsigned long long>(myref));
                 // every exchange that displaces a row is followed by its own
                 // OR, so the flag ends set exactly when the key has 2+ rows
-                if (static_cast<uint32_t>(prev) != kInvalidRow)
+                if (static_cast<uint32_t>(prev) != kInvalidRow) {
                     atomicOr(&c->row_no, kRefMore);
-                else if (a.ht.pay.n)
+                    if (a.ht.dups)
+                        *a.ht.dups = 1u;
+                } else if (a.ht.pay.n)
                     put_payload(a.ht, c, row); // the key's first row: read only while it stays the only one
                 break;
             }
             pos = (pos + 1) & mask;
         }
         a.ht.next[row] = ref_row(a, prev);
+    }
+}
+
+// One thread per cell: the row count of a flagged cell's chain.
+__global__ __launch_bounds__(256) void k_ht_group_count(HtArgs a, uint64_t total_cells, uint32_t *__restrict__ cnt) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i <= total_cells; i += stride) {
+        uint32_t len = 0;
+        if (i < total_cells) {
+            const Cell16 c = *cell_at(a.ht, i);
+            if (c.key != kEmptyKey && (c.row_no & kRefMore))
+                for (uint32_t r = ref_row(a, static_cast<uint64_t>(c.block_no) | (static_cast<uint64_t>(c.row_no) << 32));
+                     r != kInvalidRow; r = a.ht.next[r])
+                    ++len;
+        }
+        cnt[i] = len;
+    }
+}
+
+// One thread per cell: a flagged cell's chain rows (newest first, the order
+// a chain walk emits them) copied to their group, then the cell rewritten as
+// {group start, row count | kRefMore}.
+__global__ __launch_bounds__(256) void k_ht_group_fill(HtArgs a, uint64_t total_cells, const uint32_t *__restrict__ goff,
+                                                       ColSet cols) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total_cells; i += stride) {
+        Cell16 *cp = cell_at(a.ht, i);
+        const Cell16 c = *cp;
+        if (c.key == kEmptyKey || !(c.row_no & kRefMore))
+            continue;
+        const uint32_t g0 = goff[i];
+        uint32_t k = g0;
+        for (uint32_t r = ref_row(a, static_cast<uint64_t>(c.block_no) | (static_cast<uint64_t>(c.row_no) << 32));
+             r != kInvalidRow; r = a.ht.next[r], ++k) {
+#pragma unroll 1
+            for (uint32_t col = 0; col < cols.n; ++col) {
+                uint64_t lo, hi;
+                const uint32_t t = col_type(cols, col);
+                load_elem(col_ptr(cols, col), t, r, lo, hi);
+                store_elem(const_cast<void *>(col_ptr(a.ht.grp, col)), t, k, lo, hi);
+            }
+        }
+        cp->block_no = g0;
+        cp->row_no = (k - g0) | kRefMore;
     }
 }
 
@@ -1527,16 +1573,26 @@ __device__ __forceinline__ uint32_t ht_lookup(const HtArgs &a, uint64_t k, uint3
 // before its stores; rows with several matches (repeated build keys) walk
 // their chain row by row. Output row = probe key | build columns | probe
 // columns (src/JoinOps.h:77-125).
+//
+// With grouped chains (CHJ) a repeated key's cell names its group, so its rows
+// need no walk: the workgroup writes the round's group rows together, each
+// thread taking consecutive output rows and finding their slot by binary
+// search over the slots' group-row ends (as the two-phase leaf emit does).
 template <int THREADS, int U>
 __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, ColSet sc, OutSink out) {
+    constexpr int kSlots = THREADS * U;
+    constexpr int kV = 4; // group rows per thread per step
     __shared__ uint32_t tmp[THREADS / 64 + 1];
     __shared__ unsigned long long out_base;
+    __shared__ uint32_t g_end[kSlots], g_pos[kSlots], g_st[kSlots], g_row[kSlots];
+    __shared__ uint64_t g_key[kSlots];
     const uint64_t n = ht_total_rows(a);
     const uint64_t per_round = static_cast<uint64_t>(gridDim.x) * THREADS * U;
     for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * THREADS * U; base < n; base += per_round) {
         uint64_t key[U], prow[U], cb[U], cm[U], cp[U];
-        uint32_t head[U], len[U];
+        uint32_t head[U], len[U], gst[U];
         uint32_t pend = 0, many = 0; // many: rows whose chain must be walked
+        uint32_t grpm = 0;           // rows whose key's rows are one group (grouped chains)
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint64_t g = base + static_cast<uint64_t>(u) * THREADS + threadIdx.x;
@@ -1566,9 +1622,15 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
                 if (!((pend >> u) & 1u))
                     continue;
                 if (c[u].key == key[u]) {
-                    head[u] = ref_row(a, static_cast<uint64_t>(c[u].block_no) | (static_cast<uint64_t>(c[u].row_no) << 32));
-                    if (c[u].row_no & kRefMore)
-                        many |= 1u << u;
+                    if ((c[u].row_no & kRefMore) && a.ht.grouped) {
+                        grpm |= 1u << u;
+                        gst[u] = c[u].block_no;              // group start
+                        len[u] = c[u].row_no & ~kRefMore;    // group rows
+                    } else {
+                        head[u] = ref_row(a, static_cast<uint64_t>(c[u].block_no) | (static_cast<uint64_t>(c[u].row_no) << 32));
+                        if (c[u].row_no & kRefMore)
+                            many |= 1u << u;
+                    }
                     pend &= ~(1u << u); // cb + cp now names the matched cell
                 } else if (c[u].key == kEmptyKey) {
                     pend &= ~(1u << u);
@@ -1584,6 +1646,8 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             x[u] = head[u];
+            if ((grpm >> u) & 1u)
+                continue; // len is the group's row count
             len[u] = 0;
             if (x[u] != kInvalidRow) {
                 if ((many >> u) & 1u)
@@ -1652,9 +1716,58 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
             gather_set<U>(rc, out.left, ridx, opos, one);
         }
         gather_set<U>(sc, out.right, sidx, opos, one);
+        if (a.ht.grouped) {
+            uint32_t gcnt = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                gcnt += (grpm >> u) & 1u ? len[u] : 0u;
+            uint32_t g_total;
+            const uint32_t g_off = block_excl_scan<THREADS>(gcnt, tmp, &g_total);
+            if (g_total > 0) {
+                uint32_t e = g_off;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t sl = threadIdx.x * U + u;
+                    const bool gm = (grpm >> u) & 1u;
+                    e += gm ? len[u] : 0u;
+                    g_end[sl] = e;
+                    g_pos[sl] = static_cast<uint32_t>(opos[u] - out_base);
+                    g_st[sl] = gm ? gst[u] : 0u;
+                    g_row[sl] = static_cast<uint32_t>(prow[u]);
+                    g_key[sl] = key[u];
+                }
+                __syncthreads();
+                for (uint32_t q0 = 0; q0 < g_total; q0 += THREADS * kV) {
+                    uint32_t bidx[kV], gsidx[kV];
+                    uint64_t gpos[kV], kv[kV];
+                    bool gon[kV];
+#pragma unroll
+                    for (int v = 0; v < kV; ++v) {
+                        const uint32_t q = q0 + v * THREADS + threadIdx.x;
+                        const uint32_t qc = q < g_total ? q : g_total - 1;
+                        uint32_t sg = 0; // first slot whose group rows end after qc
+#pragma unroll
+                        for (uint32_t step = kSlots / 2; step > 0; step >>= 1)
+                            sg += g_end[sg + step - 1] <= qc ? step : 0u;
+                        const uint32_t ei = qc - (sg ? g_end[sg - 1] : 0u);
+                        bidx[v] = g_st[sg] + ei;
+                        gsidx[v] = g_row[sg];
+                        gpos[v] = out_base + g_pos[sg] + ei;
+                        kv[v] = g_key[sg];
+                        gon[v] = q < g_total && gpos[v] < out.cap;
+                    }
+#pragma unroll
+                    for (int v = 0; v < kV; ++v)
+                        if (gon[v])
+                            out.keys[gpos[v]] = kv[v];
+                    gather_set<kV>(a.ht.grp, out.left, bidx, gpos, gon);
+                    gather_set<kV>(sc, out.right, gsidx, gpos, gon);
+                }
+            }
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (len[u] < 2)
+            if (len[u] < 2 || ((grpm >> u) & 1u))
                 continue;
             uint64_t p = opos[u];
             for (uint32_t r = head[u]; r != kInvalidRow; r = a.ht.next[r])
@@ -1895,6 +2008,23 @@ void ht_probe_emit(hipStream_t s, const GlobalHT &ht, HtMode mode, const HtRows 
     constexpr int T = 256, U = 2; // measured on 2^30 CHJ probes: faster than U = 4 or 8
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((max_rows + T * U - 1) / (T * U), 256 * 64));
     k_ht_probe_emit<T, U><<<grid, T, 0, s>>>(make_args(ht, mode, probe), r_cols, s_cols, out);
+}
+
+void ht_group_count(hipStream_t s, const GlobalHT &ht, HtMode mode, uint64_t total_cells, uint32_t *cnt, KernelTimer *tm) {
+    TimedScope ts(tm, "ht_group", s);
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((total_cells + 256) / 256, 256 * 64));
+    HtRows r{};
+    k_ht_group_count<<<grid, 256, 0, s>>>(make_args(ht, mode, r), total_cells, cnt);
+}
+
+void ht_group_fill(hipStream_t s, const GlobalHT &ht, HtMode mode, uint64_t total_cells, const uint32_t *goff,
+                   const ColSet &cols, KernelTimer *tm) {
+    if (total_cells == 0)
+        return;
+    TimedScope ts(tm, "ht_group", s);
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((total_cells + 255) / 256, 256 * 64));
+    HtRows r{};
+    k_ht_group_fill<<<grid, 256, 0, s>>>(make_args(ht, mode, r), total_cells, goff, cols);
 }
 
 void ht_find(hipStream_t s, const GlobalHT &ht, const uint64_t *keys, uint64_t n, Cell16 *, uint32_t *heads_rows,

@@ -262,7 +262,17 @@ struct GlobalHT {
     // at ht_pay_next offsets; they are read only for keys with one build row.
     uint32_t cell_shift;
     ColSet pay;
+    // Grouped chains (CHJ): when `grouped`, the rows of every key with several
+    // build rows are laid out once, after the build, as one contiguous group
+    // of `grp` (the build columns, group by group); such a key's cell then
+    // holds block_no = its group's first row and row_no = its row count with
+    // the several-rows flag. dups is set (to 1) by any insert that displaces
+    // a previous head.
+    uint32_t grouped;
+    ColSet grp;
+    uint32_t *dups;
 };
+
 // Offset of the next inline payload column of width w after byte `off` of a
 // cell (16-byte columns 16-aligned, others aligned to their width).
 __host__ __device__ inline uint32_t ht_pay_at(uint32_t off, uint32_t w) {
@@ -308,6 +318,12 @@ void ht_build(hipStream_t s, const GlobalHT &ht, HtMode mode, const HtRows &r, u
 // Probe and emit (keys + build cols + probe cols) into `out`.
 void ht_probe_emit(hipStream_t s, const GlobalHT &ht, HtMode mode, const HtRows &probe, uint64_t max_rows,
                    const ColSet &r_cols, const ColSet &s_cols, const OutSink &out, KernelTimer *tm, const char *name);
+// Rows of every flagged cell's chain (0 for other cells), cnt[total_cells] = 0.
+void ht_group_count(hipStream_t s, const GlobalHT &ht, HtMode mode, uint64_t total_cells, uint32_t *cnt, KernelTimer *tm);
+// Writes each flagged cell's chain rows (build columns `cols`) at goff[cell]
+// of ht.grp and turns the cell into {group start, row count | flag}.
+void ht_group_fill(hipStream_t s, const GlobalHT &ht, HtMode mode, uint64_t total_cells, const uint32_t *goff,
+                   const ColSet &cols, KernelTimer *tm);
 // Probe only: heads per probe key (RowRefCell), plus next chain export.
 void ht_find(hipStream_t s, const GlobalHT &ht, const uint64_t *keys, uint64_t n, phjd::Cell16 *unused, uint32_t *heads_rows,
              KernelTimer *tm);
