@@ -693,6 +693,27 @@ struct LeafTableImages {
 // output rows at construction (LDS leaves by a count pass, oversize leaves
 // through HBM tables probed in count-only mode); emit() then writes them from
 // a given base row of an output buffer.
+// After an HBM table's build (with ht.dups set up before it): when a key
+// repeated, every flagged cell's chain is laid out once as a contiguous group
+// of the build columns `cols`, so no probe walks next[] (src/JoinOps.h:260-271
+// emits the same rows per match).
+void group_chains(phj_ctx *ctx, Scratch &scr, GlobalHT &ht, HtMode mode, uint64_t cells, const ColSet &cols) {
+    if (!read_scalar(ctx, ht.dups))
+        return;
+    uint32_t *goff = scr.get<uint32_t>(cells + 1);
+    ht_group_count(ctx->stream, ht, mode, cells, goff, ctx->tm());
+    uint32_t *gtmp = scr.get<uint32_t>(radix_scan_blocks(cells + 1) + 1);
+    scan_u32(ctx->stream, goff, cells + 1, nullptr, 1, gtmp, ctx->tm(), "ht_group");
+    const uint32_t grows = read_scalar(ctx, goff + cells);
+    ht.grp.n = cols.n;
+    for (uint32_t c = 0; c < cols.n; ++c) {
+        ht.grp.types[c] = cols.types[c];
+        ht.grp.ptr[c] = scr.get<unsigned char>(std::max<uint64_t>(grows, 1) * type_size(cols.types[c]));
+    }
+    ht.grouped = 1;
+    ht_group_fill(ctx->stream, ht, mode, cells, goff, cols, ctx->tm());
+}
+
 class LeafJoin {
 public:
     // `img`: prebuilt fused tables of R's leaves (then R has no repeated key
@@ -890,6 +911,8 @@ private:
         ht_.n_tables = nb_;
         ht_.block_rows = 1;
         ht_.build_seg_off = R_.off;
+        ht_.dups = scr_.get<uint32_t>(1);
+        launch_fill_u32(ctx_->stream, ht_.dups, 1, 0);
         EventPair ev;
         HIPCHK(hipEventRecord(ev.a, ctx_->stream));
         ht_init(ctx_->stream, ht_, cell_off[nb_], R_.rows, ctx_->tm());
@@ -900,6 +923,7 @@ private:
         hr.leaf_of = d_leaf;
         hr.first = d_rfirst;
         ht_build(ctx_->stream, ht_, kHtLeaves, hr, rfirst[nb_], ctx_->tm(), "leaf_ht_build");
+        group_chains(ctx_, scr_, ht_, kHtLeaves, cell_off[nb_], as_in(R_.cols));
         HIPCHK(hipEventRecord(ev.b, ctx_->stream));
         hp_.keys = S_.keys;
         hp_.rows = S_.rows;
@@ -1159,24 +1183,7 @@ void run_chj(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, phj_joi
     hr.keys = R.keys;
     hr.rows = R.rows;
     ht_build(ctx->stream, ht, kHtChj, hr, R.rows, ctx->tm(), "chj_build");
-    if (read_scalar(ctx, ht.dups)) {
-        // Repeated keys: every chain is laid out once as a contiguous group of
-        // build rows, so no probe walks next[] (src/JoinOps.h:260-271 emits
-        // the same rows per match).
-        const uint64_t cells = cell_off[kTables];
-        uint32_t *goff = scr.get<uint32_t>(cells + 1);
-        ht_group_count(ctx->stream, ht, kHtChj, cells, goff, ctx->tm());
-        uint32_t *gtmp = scr.get<uint32_t>(radix_scan_blocks(cells + 1) + 1);
-        scan_u32(ctx->stream, goff, cells + 1, nullptr, 1, gtmp, ctx->tm(), "ht_group");
-        const uint32_t grows = read_scalar(ctx, goff + cells);
-        ht.grp.n = R.cols.n;
-        for (uint32_t c = 0; c < R.cols.n; ++c) {
-            ht.grp.types[c] = R.cols.types[c];
-            ht.grp.ptr[c] = scr.get<unsigned char>(std::max<uint64_t>(grows, 1) * type_size(R.cols.types[c]));
-        }
-        ht.grouped = 1;
-        ht_group_fill(ctx->stream, ht, kHtChj, cells, goff, R.cols, ctx->tm());
-    }
+    group_chains(ctx, scr, ht, kHtChj, cell_off[kTables], R.cols);
     HIPCHK(hipEventRecord(eb.b, ctx->stream));
     HIPCHK(hipEventRecord(ep.a, ctx->stream));
     HtRows hp{};

@@ -415,6 +415,34 @@ def test_bep_schedule_counters(ctx, pass_bits):
     assert seen == sorted(seen)
 
 
+@pytest.mark.parametrize("scheme,cfg", [("chj", None), ("phj_pure", [8, 8]), ("phj_pure", [4]), ("phj", [8, 8])])
+def test_one_key_with_many_build_rows(ctx, scheme, cfg):
+    """A key with 20,000 build rows (its leaf cannot fit LDS however many
+    partition bits are used: the HBM-table path with the key's rows laid out as
+    one group) beside ordinary keys, a second repeated key and the empty-key
+    sentinel, against the oracle row for row."""
+    rng = np.random.default_rng(77)
+    hot, warm = np.uint64(0x1234_5678_9ABC_DEF0), np.uint64(2**64 - 1)
+    bkeys = rng.integers(0, 2**64, 30_000, dtype=np.uint64)
+    bkeys[:20_000] = hot
+    bkeys[20_000:20_300] = warm
+    rng.shuffle(bkeys)
+    plain = bkeys[(bkeys != hot) & (bkeys != warm)]
+    pkeys = np.concatenate([np.full(60, hot), np.full(40, warm), rng.choice(plain, 3000),
+                            rng.integers(0, 2**64, 2000, dtype=np.uint64)])
+    rng.shuffle(pkeys)
+    ob = O.Relation(bkeys, [rng.integers(0, 2**32, bkeys.size, dtype=np.uint32),
+                            rng.integers(0, 2**64, (bkeys.size, 2), dtype=np.uint64)], [G.U32, G.U128])
+    op = O.Relation(pkeys, [rng.integers(0, 2**16, pkeys.size, dtype=np.uint16)], [G.U16])
+    want = O.serialize_sorted(O.reference_join(ob, op))
+    res = run_scheme(ctx, scheme, cfg, host_rel(ob), host_rel(op))
+    out = res.to_host()
+    assert out.rows == want.shape[0] and want.shape[0] > 1_200_000
+    assert np.array_equal(sorted_rows_of(out), want)
+    assert res.checksum() == O.checksum_rows(want)
+    res.release()
+
+
 @pytest.mark.parametrize("dsname", ["zipf_multi", "sentinel_dups", "no_payload", "wide_payload"])
 @pytest.mark.parametrize("pass_bits", [[4, 4], [6], [8, 8]])
 def test_bep_group_images_many_rounds(ctx, dsname, pass_bits):
