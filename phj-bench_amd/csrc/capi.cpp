@@ -705,13 +705,17 @@ void group_chains(phj_ctx *ctx, Scratch &scr, GlobalHT &ht, HtMode mode, uint64_
     uint32_t *gtmp = scr.get<uint32_t>(radix_scan_blocks(cells + 1) + 1);
     scan_u32(ctx->stream, goff, cells + 1, nullptr, 1, gtmp, ctx->tm(), "ht_group");
     const uint32_t grows = read_scalar(ctx, goff + cells);
-    ht.grp.n = cols.n;
+    OutColSet grp{};
+    grp.n = cols.n;
     for (uint32_t c = 0; c < cols.n; ++c) {
-        ht.grp.types[c] = cols.types[c];
-        ht.grp.ptr[c] = scr.get<unsigned char>(std::max<uint64_t>(grows, 1) * type_size(cols.types[c]));
+        grp.types[c] = cols.types[c];
+        grp.ptr[c] = scr.get<unsigned char>(std::max<uint64_t>(grows, 1) * type_size(cols.types[c]));
     }
+    ht.grp = as_in(grp);
     ht.grouped = 1;
-    ht_group_fill(ctx->stream, ht, mode, cells, goff, cols, ctx->tm());
+    uint32_t *rows = scr.get<uint32_t>(grows);
+    ht_group_fill(ctx->stream, ht, mode, cells, goff, rows, ctx->tm());
+    gather_perm(ctx->stream, cols, grp, rows, grows, ctx->tm());
 }
 
 class LeafJoin {
@@ -1189,8 +1193,20 @@ void run_chj(phj_ctx *ctx, const phj_relation &b, const phj_relation &p, phj_joi
     HtRows hp{};
     hp.keys = S.keys;
     hp.rows = S.rows;
+    // Without repeated keys |S| bounds the output; with them a count-only
+    // probe (no chain walks: the groups give each key's rows) sizes it.
+    uint64_t cap = std::max<uint64_t>(S.rows, 1);
+    if (ht.grouped) {
+        unsigned long long *cnt = scr.get<unsigned long long>(1);
+        launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(cnt), 1, 0);
+        OutSink counter{};
+        counter.cursor = cnt;
+        counter.cap = 0;
+        ht_probe_emit(ctx->stream, ht, kHtChj, hp, S.rows, R.cols, S.cols, counter, ctx->tm(), "chj_count");
+        cap = std::max<uint64_t>(read_scalar(ctx, reinterpret_cast<const uint64_t *>(cnt)), 1);
+    }
     OutBuffers out;
-    const uint64_t n = join_with_capacity(ctx, R.cols, S.cols, std::max<uint64_t>(S.rows, 1), out,
+    const uint64_t n = join_with_capacity(ctx, R.cols, S.cols, cap, out,
                                           [&](const OutBuffers &o, unsigned long long *cur) {
                                               ht_probe_emit(ctx->stream, ht, kHtChj, hp, S.rows, R.cols, S.cols,
                                                             sink_of(o, cur), ctx->tm(), "chj_probe");

@@ -1522,10 +1522,11 @@ __global__ __launch_bounds__(256) void k_ht_group_count(HtArgs a, uint64_t total
 }
 
 // One thread per cell: a flagged cell's chain rows (newest first, the order
-// a chain walk emits them) copied to their group, then the cell rewritten as
-// {group start, row count | kRefMore}.
+// a chain walk emits them) listed at their group's place in rows[], then the
+// cell rewritten as {group start, row count | kRefMore}. The build columns
+// follow in one gather over rows[].
 __global__ __launch_bounds__(256) void k_ht_group_fill(HtArgs a, uint64_t total_cells, const uint32_t *__restrict__ goff,
-                                                       ColSet cols) {
+                                                       uint32_t *__restrict__ rows) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total_cells; i += stride) {
         Cell16 *cp = cell_at(a.ht, i);
@@ -1535,15 +1536,8 @@ __global__ __launch_bounds__(256) void k_ht_group_fill(HtArgs a, uint64_t total_
         const uint32_t g0 = goff[i];
         uint32_t k = g0;
         for (uint32_t r = ref_row(a, static_cast<uint64_t>(c.block_no) | (static_cast<uint64_t>(c.row_no) << 32));
-             r != kInvalidRow; r = a.ht.next[r], ++k) {
-#pragma unroll 1
-            for (uint32_t col = 0; col < cols.n; ++col) {
-                uint64_t lo, hi;
-                const uint32_t t = col_type(cols, col);
-                load_elem(col_ptr(cols, col), t, r, lo, hi);
-                store_elem(const_cast<void *>(col_ptr(a.ht.grp, col)), t, k, lo, hi);
-            }
-        }
+             r != kInvalidRow; r = a.ht.next[r], ++k)
+            rows[k] = r;
         cp->block_no = g0;
         cp->row_no = (k - g0) | kRefMore;
     }
@@ -1682,6 +1676,11 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
         if (threadIdx.x == 0)
             out_base = atomicAdd(out.cursor, static_cast<unsigned long long>(round_total));
         __syncthreads();
+        // Past the sink's capacity (a count-only sink has none) the round's
+        // rows are only counted. Uniform: every thread read the same out_base,
+        // and thread 0 rewrites it only after the next round's scan barriers.
+        if (out_base >= out.cap)
+            continue;
         uint64_t opos[U];
         uint32_t ridx[U], sidx[U];
         bool one[U];
@@ -2018,13 +2017,13 @@ void ht_group_count(hipStream_t s, const GlobalHT &ht, HtMode mode, uint64_t tot
 }
 
 void ht_group_fill(hipStream_t s, const GlobalHT &ht, HtMode mode, uint64_t total_cells, const uint32_t *goff,
-                   const ColSet &cols, KernelTimer *tm) {
+                   uint32_t *rows, KernelTimer *tm) {
     if (total_cells == 0)
         return;
     TimedScope ts(tm, "ht_group", s);
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((total_cells + 255) / 256, 256 * 64));
     HtRows r{};
-    k_ht_group_fill<<<grid, 256, 0, s>>>(make_args(ht, mode, r), total_cells, goff, cols);
+    k_ht_group_fill<<<grid, 256, 0, s>>>(make_args(ht, mode, r), total_cells, goff, rows);
 }
 
 void ht_find(hipStream_t s, const GlobalHT &ht, const uint64_t *keys, uint64_t n, Cell16 *, uint32_t *heads_rows,

@@ -320,10 +320,11 @@ void ht_probe_emit(hipStream_t s, const GlobalHT &ht, HtMode mode, const HtRows 
                    const ColSet &r_cols, const ColSet &s_cols, const OutSink &out, KernelTimer *tm, const char *name);
 // Rows of every flagged cell's chain (0 for other cells), cnt[total_cells] = 0.
 void ht_group_count(hipStream_t s, const GlobalHT &ht, HtMode mode, uint64_t total_cells, uint32_t *cnt, KernelTimer *tm);
-// Writes each flagged cell's chain rows (build columns `cols`) at goff[cell]
-// of ht.grp and turns the cell into {group start, row count | flag}.
+// Lists each flagged cell's chain rows (build row indices) at goff[cell] of
+// rows[] and turns the cell into {group start, row count | flag}; the build
+// columns are then gathered into ht.grp through rows[] (gather_perm).
 void ht_group_fill(hipStream_t s, const GlobalHT &ht, HtMode mode, uint64_t total_cells, const uint32_t *goff,
-                   const ColSet &cols, KernelTimer *tm);
+                   uint32_t *rows, KernelTimer *tm);
 // Probe only: heads per probe key (RowRefCell), plus next chain export.
 void ht_find(hipStream_t s, const GlobalHT &ht, const uint64_t *keys, uint64_t n, phjd::Cell16 *unused, uint32_t *heads_rows,
              KernelTimer *tm);
