@@ -417,6 +417,27 @@ __device__ __forceinline__ void round_layout(const uint16_t (&h)[U], const uint3
 // workgroups give a SIMD four waves, where 256 x 8 gave it two.
 constexpr int kEmitThreads = 512;
 constexpr int kEmitSlots = 4;
+// Number of entries <= x among a[lo, lo + SPAN) (ascending). Each level reads
+// FAN - 1 block ends together and keeps the block that holds the answer; the
+// levels below FAN halve. Binary (FAN 2) by default: on the skewed workload's
+// multi-match emission FAN 4, 8 and 16 (fewer dependent LDS rounds, more
+// reads) measured within 1% of it.
+template <uint32_t SPAN, uint32_t FAN = 2>
+__device__ __forceinline__ uint32_t count_le(const uint32_t *a, uint32_t lo, uint32_t x) {
+    if constexpr (SPAN <= 1) {
+        return lo;
+    } else if constexpr (SPAN < FAN) {
+        return count_le<SPAN / 2, FAN>(a, lo + (a[lo + SPAN / 2 - 1] <= x ? SPAN / 2 : 0u), x);
+    } else {
+        constexpr uint32_t STEP = SPAN / FAN;
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t j = 1; j < FAN; ++j)
+            c += a[lo + j * STEP - 1] <= x ? 1u : 0u;
+        return count_le<STEP, FAN>(a, lo + c * STEP, x);
+    }
+}
+
 // Multi-match rows are written by the whole workgroup, kEmitVec rows per
 // thread per step, at consecutive output positions.
 constexpr int kEmitVec = 4;
@@ -640,10 +661,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                     const uint32_t o = k0 + v * THREADS + threadIdx.x;
                     on[v] = o < L.chain_total;
                     const uint32_t oc = on[v] ? o : L.chain_total - 1;
-                    uint32_t sg = 0;
-#pragma unroll
-                    for (uint32_t step = THREADS * U / 2; step > 0; step >>= 1)
-                        sg += seg_end[sg + step - 1] <= oc ? step : 0u;
+                    const uint32_t sg = count_le<THREADS * U>(seg_end, 0, oc); // first segment ending after oc
                     const uint32_t e = oc - (sg ? seg_end[sg - 1] : 0u);
                     sidx[v] = (sg % U) * THREADS + sg / U; // round-relative
                     ridx[v] = perm[seg_gs[sg] + e];         // leaf-local
@@ -1744,10 +1762,7 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
                     for (int v = 0; v < kV; ++v) {
                         const uint32_t q = q0 + v * THREADS + threadIdx.x;
                         const uint32_t qc = q < g_total ? q : g_total - 1;
-                        uint32_t sg = 0; // first slot whose group rows end after qc
-#pragma unroll
-                        for (uint32_t step = kSlots / 2; step > 0; step >>= 1)
-                            sg += g_end[sg + step - 1] <= qc ? step : 0u;
+                        const uint32_t sg = count_le<kSlots>(g_end, 0, qc); // first slot whose group rows end after qc
                         const uint32_t ei = qc - (sg ? g_end[sg - 1] : 0u);
                         bidx[v] = g_st[sg] + ei;
                         gsidx[v] = g_row[sg];
