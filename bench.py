@@ -403,6 +403,49 @@ def headline(args, tuning):
                 roofline=roofline, n=n)
 
 
+def layout_probes(scale_log2=0):
+    """The measurements behind the partitioning layout (phj_layout_probe):
+    random 16-byte lookups by table size (where a table must live for a pass
+    to be saved), a buffer written and read back at once by scratch size
+    (whether the caches absorb a fused pass's intermediate), and one radix
+    pass's memory pattern over 2^30 rows of 32 bytes into column or row
+    outputs and into 256..4,096 partitions (the cost of a wider single pass)."""
+    ctx = G.Context(0)
+    reads = 1 << (28 - scale_log2)
+    rr = {}
+    for mib in (1, 2, 4, 8, 16, 32, 64, 128, 256, 1024, 8192):
+        ms = ctx.layout_probe(G.Context.PROBE_RANDOM_READ, mib << 20, reads, reps=3)
+        rr[f"{mib} MiB"] = round(reads / (ms * 1e-3) / 1e9, 1)
+    log("random 16-byte reads, G/s by table size: " + ", ".join(f"{k} {v}" for k, v in rr.items()))
+    nb = 16 << (30 - scale_log2)
+    sc = {}
+    for mib in (16, 32, 64, 128, 256, 512, nb >> 20):
+        sc[f"{mib} MiB"] = round(ctx.layout_probe(G.Context.PROBE_SCRATCH_COPY, nb, mib << 20, reps=3), 3)
+    base = sc[f"{nb >> 20} MiB"]
+    log("copy of %d GiB through a scratch, ms by scratch size: " % (nb >> 30) +
+        ", ".join(f"{k} {v}" for k, v in sc.items()) + f" (the last holds the whole buffer: no reuse)")
+    pb = 32 << (30 - scale_log2)
+    passes = {}
+    for parts in (256, 512, 1024, 2048, 4096):
+        passes[f"columns {parts}"] = round(ctx.layout_probe(G.Context.PROBE_PASS_COLUMNS, pb, parts, reps=3), 3)
+    passes["rows 256"] = round(ctx.layout_probe(G.Context.PROBE_PASS_ROWS, pb, 256, reps=3), 3)
+    log("one pass over %d rows of 32 bytes, ms: " % (pb // 32) + ", ".join(f"{k} {v}" for k, v in passes.items()))
+    ctx.close()
+    # one pass into fresh output columns: four sets of separately allocated
+    # columns, then four sets each allocated as one block (columns back to back)
+    pctx = G.Context(0)
+    placement = {"separate": [round(v, 3) for v in pctx.placement_probe(pb // 32, 4, reps=3)],
+                 "one_block": [round(v, 3) for v in pctx.placement_probe(pb // 32, 4, reps=3, gap=0)]}
+    pctx.close()
+    log("the same pass (columns, 256) into 4 freshly allocated sets, ms: separate columns " +
+        ", ".join(map(str, placement["separate"])) + "; one block per set " + ", ".join(map(str, placement["one_block"])))
+    return {"random_read_G_per_s": rr, "reads_per_size": reads,
+            "scratch_copy_ms": sc, "scratch_copy_bytes": nb, "scratch_copy_no_reuse_ms": base,
+            "pass_ms": passes, "pass_rows": pb // 32, "placement_sets_ms": placement,
+            "workload": "phj_layout_probe: random 16-byte reads by table size; a copy through a reused scratch; "
+                        "one radix pass's memory pattern (32-byte rows, 4,096-row tiles) into columns or rows"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,8 +465,9 @@ def main():
     ap.add_argument("--serve-rates", default="250,500,1000,2000,5000,10000,20000")
     ap.add_argument("--no-skew", action="store_true", help="skip the skewed-join measurement")
     ap.add_argument("--workloads", default="main,batch,skew,serve",
-                    help="comma-separated subset of main (the headline join), batch, skew, serve; a run without "
-                         "main prints value null (for profiling one workload)")
+                    help="comma-separated subset of main (the headline join), batch, skew, serve, probes (the "
+                         "layout probes, not in the default set); a run without main prints value null (for "
+                         "profiling one workload)")
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_baseline_only:  # the child process of cpu_baseline_child; touches no GPU
@@ -453,6 +497,8 @@ def main():
     if args.serve_requests > 0:
         serving = serving_bench(args.serve_requests, [int(r) for r in args.serve_rates.split(",") if r],
                                 scale_log2=args.scale_log2)
+
+    probes = layout_probes(args.scale_log2) if "probes" in wl else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -493,6 +539,8 @@ def main():
         "skew": skew,
         "serving": serving,
     }
+    if probes is not None:
+        line["layout_probes"] = probes
     print(json.dumps(line), flush=True)
     if ((h is not None and not h["parity"]) or (batched is not None and not batched["parity_per_join"])
             or (serving is not None and not serving["parity_alone_vs_mix"])

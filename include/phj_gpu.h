@@ -378,6 +378,37 @@ phj_status phj_copy_bench(phj_ctx *ctx, void *dst, const void *src, uint64_t byt
 #define PHJ_STREAM_WRITE 2u
 phj_status phj_stream_bench(phj_ctx *ctx, void *dst, const void *src, uint64_t bytes, uint32_t mode, int reps,
                             double *avg_ms);
+/* Harness: measurements behind the partitioning layout, each on buffers the
+ * call allocates from the context's pool and frees again; *avg_ms is the mean
+ * of `reps` timed passes after one untimed pass. No reference counterpart.
+ *   PHJ_PROBE_RANDOM_READ: `param` random 16-byte reads (0: 2^28) of a table
+ *     of `bytes` (rounded down to a power of two, at least 4 KiB): the lookup
+ *     rate of a hash table of that size;
+ *   PHJ_PROBE_SCRATCH_COPY: `bytes` copied from one buffer to another through
+ *     a scratch buffer of `param` bytes, a chunk at a time (written to the
+ *     scratch, then read back from it): whether a buffer reused at once is
+ *     served by the caches;
+ *   PHJ_PROBE_PASS_COLUMNS / PHJ_PROBE_PASS_ROWS: one radix pass's memory
+ *     pattern over bytes / 32 rows of 32 bytes (columns of 8, 8 and 16 bytes)
+ *     into `param` partitions (a power of two <= 4,096; 4,096-row tiles send
+ *     runs of 4096 / param rows), written as three columns or as one array of
+ *     32-byte rows. */
+#define PHJ_PROBE_RANDOM_READ 0u
+#define PHJ_PROBE_SCRATCH_COPY 1u
+#define PHJ_PROBE_PASS_COLUMNS 2u
+#define PHJ_PROBE_PASS_ROWS 3u
+phj_status phj_layout_probe(phj_ctx *ctx, uint32_t mode, uint64_t bytes, uint64_t param, int reps, double *avg_ms);
+/* Harness: the dependence of one radix pass on where its buffers land. Writes
+ * the PHJ_PROBE_PASS_COLUMNS pattern (256 partitions) over `rows` rows (rounded
+ * down to a multiple of 4,096) from one input into each of `sets` (1..8) sets
+ * of three output columns (8-, 8- and 16-byte elements). With gap ==
+ * PHJ_PLACEMENT_SEPARATE every column is its own fresh hipMalloc; otherwise a
+ * set is one fresh hipMalloc holding the columns one after another with `gap`
+ * bytes (rounded up to 256) between them. Allocated set by set outside the
+ * context's pool and freed on return; ms[k] is the mean of `reps` timed passes
+ * into set k after one untimed pass. No reference counterpart. */
+#define PHJ_PLACEMENT_SEPARATE UINT64_MAX
+phj_status phj_placement_probe(phj_ctx *ctx, uint64_t rows, uint32_t sets, uint64_t gap, int reps, double *ms);
 
 #ifdef __cplusplus
 }

@@ -11,11 +11,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -3526,6 +3528,111 @@ phj_status phj_stream_bench(phj_ctx *ctx, void *dst, const void *src, uint64_t b
         ctx->sync();
         if (avg_ms)
             *avg_ms = ev.ms() / reps;
+    });
+}
+
+phj_status phj_layout_probe(phj_ctx *ctx, uint32_t mode, uint64_t bytes, uint64_t param, int reps, double *avg_ms) {
+    return guard(ctx, [&] {
+        if (reps <= 0 || mode > PHJ_PROBE_PASS_ROWS)
+            invalid("reps must be positive and mode one of PHJ_PROBE_*");
+        Scratch scr(ctx);
+        std::function<void()> once;
+        if (mode == PHJ_PROBE_RANDOM_READ) {
+            if (bytes < 4096)
+                invalid("random-read table below 4 KiB");
+            const uint64_t words = 1ull << (63 - __builtin_clzll(bytes / 16));
+            const uint64_t reads = param ? param : (1ull << 28);
+            unsigned char *t = scr.get<unsigned char>(words * 16);
+            launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(t), words * 2, 0x0123456789ABCDEFull);
+            uint32_t *sink = scr.get<uint32_t>(4);
+            uint64_t seed = 0;
+            once = [=, &seed] { launch_probe_random_read(ctx->stream, t, words, reads, seed, sink); seed += reads; };
+        } else if (mode == PHJ_PROBE_SCRATCH_COPY) {
+            const uint64_t n = bytes & ~uint64_t{15};
+            const uint64_t chunk = param & ~uint64_t{15};
+            if (n == 0 || chunk == 0)
+                invalid("scratch copy needs bytes and a scratch size of at least 16");
+            unsigned char *src = scr.get<unsigned char>(n);
+            unsigned char *dst = scr.get<unsigned char>(n);
+            unsigned char *tmp = scr.get<unsigned char>(std::min(chunk, n));
+            launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(src), n / 8, 0x0123456789ABCDEFull);
+            once = [=] {
+                for (uint64_t off = 0; off < n; off += chunk) {
+                    const uint64_t len = std::min(chunk, n - off);
+                    launch_probe_copy(ctx->stream, tmp, src + off, len);
+                    launch_probe_copy(ctx->stream, dst + off, tmp, len);
+                }
+            };
+        } else {
+            const uint64_t n = bytes / 32 / 4096 * 4096;
+            if (n == 0 || param == 0 || param > 4096 || (param & (param - 1)) != 0)
+                invalid("pass probe needs >= 4,096 rows of 32 bytes and a power-of-two partition count <= 4,096");
+            unsigned char *in = scr.get<unsigned char>(n * 32);
+            unsigned char *out = scr.get<unsigned char>(n * 32);
+            launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t *>(in), n * 4, 0x0123456789ABCDEFull);
+            const bool rows = mode == PHJ_PROBE_PASS_ROWS;
+            const uint32_t parts = static_cast<uint32_t>(param);
+            once = [=] { launch_probe_pass(ctx->stream, rows, in, out, n, parts); };
+        }
+        once(); // untimed: first touch of the buffers
+        EventPair ev;
+        HIPCHK(hipEventRecord(ev.a, ctx->stream));
+        for (int r = 0; r < reps; ++r)
+            once();
+        HIPCHK(hipEventRecord(ev.b, ctx->stream));
+        ctx->sync();
+        if (avg_ms)
+            *avg_ms = ev.ms() / reps;
+    });
+}
+
+phj_status phj_placement_probe(phj_ctx *ctx, uint64_t rows, uint32_t sets, uint64_t gap, int reps, double *ms) {
+    return guard(ctx, [&] {
+        const uint64_t n = rows / 4096 * 4096;
+        if (n == 0 || sets == 0 || sets > 8 || reps <= 0 || ms == nullptr)
+            invalid("placement probe needs >= 4,096 rows, 1..8 sets, positive reps and an output array");
+        // Fresh device allocations, outside the context's pool, released on return.
+        std::vector<void *> raw;
+        struct Release {
+            std::vector<void *> &v;
+            ~Release() {
+                for (void *p : v)
+                    (void)hipFree(p);
+            }
+        } release{raw};
+        auto get = [&](uint64_t bytes) {
+            void *p = nullptr;
+            if (hipMalloc(&p, bytes) != hipSuccess)
+                throw std::bad_alloc();
+            raw.push_back(p);
+            return p;
+        };
+        void *in = get(n * 32);
+        launch_fill_u64(ctx->stream, static_cast<uint64_t *>(in), n * 4, 0x0123456789ABCDEFull);
+        std::vector<std::array<void *, 3>> out(sets);
+        for (uint32_t k = 0; k < sets; ++k) {
+            if (gap == PHJ_PLACEMENT_SEPARATE) {
+                out[k] = {get(n * 8), get(n * 8), get(n * 16)};
+            } else {
+                const uint64_t g = (gap + 255) & ~uint64_t{255};
+                char *b = static_cast<char *>(get(n * 32 + 2 * g));
+                out[k] = {b, b + n * 8 + g, b + n * 16 + 2 * g};
+            }
+        }
+        for (uint32_t k = 0; k < sets; ++k) {
+            auto pass = [&] {
+                launch_probe_pass_cols(ctx->stream, in, static_cast<uint64_t *>(out[k][0]), static_cast<uint64_t *>(out[k][1]),
+                                       out[k][2], n, 256);
+            };
+            pass(); // first touch
+            EventPair ev;
+            HIPCHK(hipEventRecord(ev.a, ctx->stream));
+            for (int r = 0; r < reps; ++r)
+                pass();
+            HIPCHK(hipEventRecord(ev.b, ctx->stream));
+            ctx->sync();
+            ms[k] = ev.ms() / reps;
+        }
     });
 }
 

@@ -238,6 +238,160 @@ __global__ __launch_bounds__(256) void k_stream_copy(const v4u32 *__restrict__ a
     }
 }
 
+// ---- layout probes: the measurements behind the partitioning layout ----
+
+// Random 16-byte reads of a table of `mask + 1` words (a power of two): read
+// i goes to word int_hash64(seed + i) & mask, eight reads in flight per
+// thread. The rate of lookups whose table lives in L2, the MALL or HBM.
+__global__ __launch_bounds__(256) void k_probe_random_read(const v4u32 *__restrict__ t, uint64_t mask, uint64_t n,
+                                                           uint64_t seed, uint32_t *__restrict__ sink) {
+    constexpr int U = 8;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * U;
+    uint32_t acc = 0;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x * U + threadIdx.x; i < n; i += stride) {
+        v4u32 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t j = i + static_cast<uint64_t>(u) * blockDim.x;
+            v[u] = t[int_hash64(seed + j) & mask];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    if (acc == 0x9E3779B9u) // practically never; keeps the loads
+        sink[0] = acc;
+}
+
+// Plain copy with cached loads and stores (a scratch buffer written and read
+// back at once may stay in the caches).
+__global__ __launch_bounds__(256) void k_probe_copy(const v4u32 *__restrict__ a, v4u32 *__restrict__ b, uint64_t n) {
+    constexpr int U = 4;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x * U;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x * U + threadIdx.x; i < n; i += stride) {
+        v4u32 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t j = i + static_cast<uint64_t>(u) * blockDim.x;
+            v[u] = a[j < n ? j : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t j = i + static_cast<uint64_t>(u) * blockDim.x;
+            if (j < n)
+                b[j] = v[u];
+        }
+    }
+}
+
+// The memory pattern of one radix pass over 32-byte rows (an 8-, an 8- and a
+// 16-byte column), without the ranking: each 4,096-row tile sends its rows in
+// runs of 4096 / parts consecutive rows to `parts` partitions of n / parts
+// rows each (tile t's run of partition p at row p * n / parts + t * run), as
+// the scatter's staged stores do. Input columns are read with nontemporal
+// loads. n is a multiple of 4,096 and parts a power of two <= 4,096.
+constexpr int kProbeTile = 4096;
+constexpr int kProbeThreads = 1024;
+__device__ __forceinline__ uint64_t probe_dest(uint64_t n, uint32_t parts, uint32_t tile, uint32_t k) {
+    const uint32_t run = kProbeTile / parts;
+    return static_cast<uint64_t>(k / run) * (n / parts) + static_cast<uint64_t>(tile) * run + k % run;
+}
+// Into three column outputs.
+__global__ __launch_bounds__(kProbeThreads) void k_probe_pass_cols(const uint64_t *__restrict__ ik, const uint64_t *__restrict__ i1,
+                                                                   const v4u32 *__restrict__ i2, uint64_t *__restrict__ ok,
+                                                                   uint64_t *__restrict__ o1, v4u32 *__restrict__ o2, uint64_t n,
+                                                                   uint32_t parts) {
+    constexpr int R = kProbeTile / kProbeThreads;
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = static_cast<uint64_t>(tile) * kProbeTile;
+    uint64_t a[R], b[R];
+    v4u32 c[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        a[r] = __builtin_nontemporal_load(ik + base + r * kProbeThreads + threadIdx.x);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        ok[probe_dest(n, parts, tile, r * kProbeThreads + threadIdx.x)] = a[r];
+    __syncthreads(); // a column at a time per workgroup, as the scatter's staged columns go
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        b[r] = __builtin_nontemporal_load(i1 + base + r * kProbeThreads + threadIdx.x);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        o1[probe_dest(n, parts, tile, r * kProbeThreads + threadIdx.x)] = b[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        c[r] = __builtin_nontemporal_load(i2 + base + r * kProbeThreads + threadIdx.x);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        o2[probe_dest(n, parts, tile, r * kProbeThreads + threadIdx.x)] = c[r];
+}
+// Into one row-major output: a lane pair writes a row's two 16-byte halves.
+__global__ __launch_bounds__(kProbeThreads) void k_probe_pass_rows(const uint64_t *__restrict__ ik, const uint64_t *__restrict__ i1,
+                                                                   const v4u32 *__restrict__ i2, v4u32 *__restrict__ orow,
+                                                                   uint64_t n, uint32_t parts) {
+    constexpr int R = 2 * kProbeTile / kProbeThreads;
+    const uint32_t tile = blockIdx.x;
+    const uint64_t base = static_cast<uint64_t>(tile) * kProbeTile;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t w = r * kProbeThreads + threadIdx.x;
+        const uint32_t k = w >> 1;
+        v4u32 v;
+        if (w & 1u) {
+            v = __builtin_nontemporal_load(i2 + base + k);
+        } else {
+            const uint64_t x = __builtin_nontemporal_load(ik + base + k);
+            const uint64_t y = __builtin_nontemporal_load(i1 + base + k);
+            v = v4u32{static_cast<uint32_t>(x), static_cast<uint32_t>(x >> 32), static_cast<uint32_t>(y),
+                      static_cast<uint32_t>(y >> 32)};
+        }
+        orow[probe_dest(n, parts, tile, k) * 2 + (w & 1u)] = v;
+    }
+}
+
+void launch_probe_random_read(hipStream_t s, const void *table, uint64_t words, uint64_t reads, uint64_t seed,
+                              uint32_t *sink) {
+    if (reads == 0 || words == 0)
+        return;
+    k_probe_random_read<<<256 * 16, 256, 0, s>>>(static_cast<const v4u32 *>(table), words - 1, reads, seed, sink);
+}
+
+void launch_probe_copy(hipStream_t s, void *dst, const void *src, uint64_t bytes) {
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0)
+        return;
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n16 + 1023) / 1024, 256 * 64));
+    k_probe_copy<<<grid, 256, 0, s>>>(static_cast<const v4u32 *>(src), static_cast<v4u32 *>(dst), n16);
+}
+
+void launch_probe_pass_cols(hipStream_t s, const void *in, uint64_t *ok, uint64_t *o1, void *o2, uint64_t n,
+                            uint32_t parts) {
+    if (n == 0)
+        return;
+    const uint32_t tiles = checked_grid(n / kProbeTile, kProbeThreads, "probe_pass");
+    const uint64_t *ik = static_cast<const uint64_t *>(in);
+    k_probe_pass_cols<<<tiles, kProbeThreads, 0, s>>>(ik, ik + n, reinterpret_cast<const v4u32 *>(ik + 2 * n), ok, o1,
+                                                      static_cast<v4u32 *>(o2), n, parts);
+}
+
+void launch_probe_pass(hipStream_t s, bool rows, const void *in, void *out, uint64_t n, uint32_t parts) {
+    if (n == 0)
+        return;
+    const uint32_t tiles = checked_grid(n / kProbeTile, kProbeThreads, "probe_pass");
+    const uint64_t *ik = static_cast<const uint64_t *>(in);
+    const uint64_t *i1 = ik + n;
+    const v4u32 *i2 = reinterpret_cast<const v4u32 *>(i1 + n);
+    if (rows) {
+        k_probe_pass_rows<<<tiles, kProbeThreads, 0, s>>>(ik, i1, i2, static_cast<v4u32 *>(out), n, parts);
+    } else {
+        uint64_t *ok = static_cast<uint64_t *>(out);
+        uint64_t *o1 = ok + n;
+        k_probe_pass_cols<<<tiles, kProbeThreads, 0, s>>>(ik, i1, i2, ok, o1, reinterpret_cast<v4u32 *>(o1 + n), n, parts);
+    }
+}
+
 void launch_stream(hipStream_t s, int mode, void *dst, const void *src, uint64_t bytes, KernelTimer *tm) {
     const uint64_t n16 = bytes / 16;
     if (n16 == 0)

@@ -53,6 +53,20 @@ void launch_copy(hipStream_t s, void *dst, const void *src, uint64_t bytes, Kern
 // mode 0: copy src -> dst; 1: read src (dst receives at most one word); 2: write dst.
 // Whole 16-byte words only (bytes / 16 of them).
 void launch_stream(hipStream_t s, int mode, void *dst, const void *src, uint64_t bytes, KernelTimer *tm);
+// Layout probes. Random 16-byte reads of a table of `words` 16-byte words (a
+// power of two); a copy with cached loads and stores; one radix pass's memory
+// pattern over n 32-byte rows (columns of 8, 8 and 16 bytes, one after
+// another in `in`) into `parts` partitions, written as the same three columns
+// (rows == false) or as one array of 32-byte rows (rows == true). n is a
+// multiple of 4,096 and parts a power of two <= 4,096.
+void launch_probe_random_read(hipStream_t s, const void *table, uint64_t words, uint64_t reads, uint64_t seed,
+                              uint32_t *sink);
+void launch_probe_copy(hipStream_t s, void *dst, const void *src, uint64_t bytes);
+void launch_probe_pass(hipStream_t s, bool rows, const void *in, void *out, uint64_t n, uint32_t parts);
+// The column form into three separately allocated output columns (8-, 8- and
+// 16-byte elements, n each).
+void launch_probe_pass_cols(hipStream_t s, const void *in, uint64_t *ok, uint64_t *o1, void *o2, uint64_t n,
+                            uint32_t parts);
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint64_t n, uint32_t v);
 void launch_fill_u64(hipStream_t s, uint64_t *p, uint64_t n, uint64_t v);
 

@@ -208,6 +208,9 @@ _SIGNATURES = {
     "phj_copy_bench": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "phj_stream_bench": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_double)],
                          C.c_int),
+    "phj_layout_probe": ([C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "phj_placement_probe": ([C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int, C.POINTER(C.c_double)],
+                            C.c_int),
 }
 
 _lib = None
@@ -875,6 +878,29 @@ class Context:
         finally:
             lib.phj_dev_free(self._p, a)
             lib.phj_dev_free(self._p, b)
+
+    PROBE_RANDOM_READ, PROBE_SCRATCH_COPY, PROBE_PASS_COLUMNS, PROBE_PASS_ROWS = 0, 1, 2, 3
+
+    def layout_probe(self, mode, nbytes, param=0, reps=3):
+        """Mean ms of one layout-probe pass (phj_layout_probe): PROBE_RANDOM_READ
+        (param random 16-byte reads of an nbytes table), PROBE_SCRATCH_COPY
+        (nbytes copied through a param-byte scratch), PROBE_PASS_COLUMNS /
+        PROBE_PASS_ROWS (one radix pass's pattern over nbytes / 32 rows into
+        param partitions)."""
+        ms = C.c_double()
+        _raise(self._p, load_library().phj_layout_probe(self._p, mode, nbytes, param, reps, C.byref(ms)))
+        return float(ms.value)
+
+    PLACEMENT_SEPARATE = (1 << 64) - 1
+
+    def placement_probe(self, rows, sets, reps=3, gap=PLACEMENT_SEPARATE):
+        """Mean ms of one radix pass's write pattern (256 partitions, rows of
+        8 + 8 + 16 bytes) into each of `sets` freshly allocated sets of output
+        columns (phj_placement_probe): each column its own allocation, or one
+        allocation per set with `gap` bytes between its columns."""
+        ms = (C.c_double * max(sets, 1))()
+        _raise(self._p, load_library().phj_placement_probe(self._p, rows, sets, gap, reps, ms))
+        return [float(v) for v in ms]
 
     def copy_bench(self, nbytes, reps=5):
         lib = load_library()

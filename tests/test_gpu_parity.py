@@ -589,3 +589,37 @@ def test_stream_bench_rates(ctx):
     assert set(rates) == {"copy", "read", "write"}
     for v in rates.values():
         assert 1e11 < v < 1e13
+
+
+def test_layout_probe_modes(ctx):
+    """The layout probes run each mode with plausible timings and refuse
+    arguments outside their documented ranges."""
+    # 2^20 random reads of a 1 MiB table, of a 64 MiB table
+    for mib in (1, 64):
+        ms = ctx.layout_probe(G.Context.PROBE_RANDOM_READ, mib << 20, 1 << 20, reps=2)
+        assert 0 < ms < 1000
+    # 64 MiB through a 4 MiB scratch, and through one that holds it all
+    for scratch in (4 << 20, 64 << 20):
+        ms = ctx.layout_probe(G.Context.PROBE_SCRATCH_COPY, 64 << 20, scratch, reps=2)
+        assert 0 < ms < 1000
+    # one pass over 2^20 rows of 32 bytes: 32 MiB read and written at > 100 GB/s
+    for mode in (G.Context.PROBE_PASS_COLUMNS, G.Context.PROBE_PASS_ROWS):
+        for parts in (1, 256, 4096):
+            ms = ctx.layout_probe(mode, 32 << 20, parts, reps=2)
+            assert 0 < ms < 2 * (32 << 20) / 1e11 * 1e3
+    for mode, nbytes, param in ((G.Context.PROBE_RANDOM_READ, 1024, 0), (G.Context.PROBE_SCRATCH_COPY, 1 << 20, 0),
+                                (G.Context.PROBE_PASS_COLUMNS, 32 << 20, 3), (G.Context.PROBE_PASS_ROWS, 32 << 20, 8192),
+                                (G.Context.PROBE_PASS_ROWS, 1024, 256), (7, 1 << 20, 1)):
+        with pytest.raises(ValueError):
+            ctx.layout_probe(mode, nbytes, param, reps=1)
+
+
+def test_placement_probe(ctx):
+    """The placement probe times the pass pattern into each freshly allocated
+    set and refuses out-of-range arguments."""
+    for gap in (ctx.PLACEMENT_SEPARATE, 0, 4096):
+        ms = ctx.placement_probe(1 << 20, 2, reps=2, gap=gap)
+        assert len(ms) == 2 and all(0 < v < 2 * (32 << 20) / 1e11 * 1e3 for v in ms)
+    for rows, sets in ((1000, 1), (1 << 20, 0), (1 << 20, 9)):
+        with pytest.raises(ValueError):
+            ctx.placement_probe(rows, sets, reps=1)
