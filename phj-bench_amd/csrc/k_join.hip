@@ -1363,6 +1363,113 @@ __device__ __forceinline__ void gather_cells(const GlobalHT &ht, const ColSet &i
     }
 }
 
+// Element of type t at byte address p (two 64-bit halves for 16 bytes).
+__device__ __forceinline__ void load_at(const unsigned char *p, uint32_t t, uint64_t &lo, uint64_t &hi) {
+    hi = 0;
+    switch (t) {
+    case 0: lo = *p; break;
+    case 1: lo = *reinterpret_cast<const uint16_t *>(p); break;
+    case 2: lo = *reinterpret_cast<const uint32_t *>(p); break;
+    case 3: lo = *reinterpret_cast<const uint64_t *>(p); break;
+    default: {
+        const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(p);
+        lo = v.x;
+        hi = v.y;
+    }
+    }
+}
+
+// Loads column j of a round's one-match rows (see emit_one_rows); pay_off is
+// the inline offset of build column j in a wide cell, advanced past it.
+template <int U>
+__device__ __forceinline__ void load_one_col(uint32_t j, const unsigned char *cells, uint32_t shift, const ColSet &rc,
+                                             const ColSet &sc, const uint64_t (&ci)[U], const bool (&fromcell)[U],
+                                             const ulonglong2 (&pay)[U], const uint32_t (&ridx)[U],
+                                             const uint32_t (&sidx)[U], const bool (&on)[U], uint32_t &pay_off,
+                                             uint64_t (&lo)[U], uint64_t (&hi)[U]) {
+    if (j < rc.n) {
+        const uint32_t t = col_type(rc, j);
+        const uint32_t w = type_size(t);
+        pay_off = ht_pay_at(pay_off, w);
+        const unsigned char *col = static_cast<const unsigned char *>(col_ptr(rc, j));
+        if (shift == 5) {
+            // 32-byte cells: the column from the payload already in registers
+            // (bytes [pay_off - 16, + w) of the 16 inline bytes, aligned to w)
+            const uint32_t o = pay_off - 16;
+            const uint64_t m = w >= 8 ? ~0ull : (1ull << (8 * w)) - 1;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (fromcell[u]) {
+                    const uint64_t half = o >= 8 ? pay[u].y : pay[u].x;
+                    lo[u] = w == 16 ? pay[u].x : (half >> (8 * (o & 7))) & m;
+                    hi[u] = w == 16 ? pay[u].y : 0;
+                } else {
+                    load_at(col + static_cast<uint64_t>(on[u] ? ridx[u] : 0u) * w, t, lo[u], hi[u]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned char *p = fromcell[u] ? cells + (ci[u] << shift) + pay_off
+                                                     : col + static_cast<uint64_t>(on[u] ? ridx[u] : 0u) * w;
+                load_at(p, t, lo[u], hi[u]);
+            }
+        }
+        pay_off += w;
+    } else {
+        const uint32_t c = j - rc.n;
+        const uint32_t t = col_type(sc, c);
+        const void *col = col_ptr(sc, c);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            load_elem(col, t, on[u] ? sidx[u] : 0u, lo[u], hi[u]);
+    }
+}
+
+// The build and probe columns of a round's one-match rows, column after
+// column, each column's loads issued before the previous column's stores (a
+// load issued after a store waits for it: one counter orders a wave's memory
+// operations). Build column c of row u comes from wide cell ci[u] when
+// fromcell[u] (from its payload pay[u], already loaded, for 32-byte cells),
+// else from the build columns at ridx[u]; probe columns from sidx.
+template <int U>
+__device__ __forceinline__ void emit_one_rows(const GlobalHT &ht, const ColSet &rc, const ColSet &sc, const OutSink &out,
+                                              const uint64_t (&ci)[U], const bool (&fromcell)[U],
+                                              const ulonglong2 (&pay)[U], const uint32_t (&ridx)[U],
+                                              const uint32_t (&sidx)[U], const uint64_t (&opos)[U], const bool (&on)[U]) {
+    const uint32_t nc = rc.n + sc.n;
+    if (nc == 0)
+        return;
+    const unsigned char *cells = reinterpret_cast<const unsigned char *>(ht.cells);
+    const uint32_t shift = ht.cell_shift;
+    uint32_t pay_off = 16;
+    uint64_t lo[U], hi[U];
+    load_one_col<U>(0, cells, shift, rc, sc, ci, fromcell, pay, ridx, sidx, on, pay_off, lo, hi);
+    for (uint32_t j = 0; j < nc; ++j) {
+        uint64_t nlo[U], nhi[U];
+        if (j + 1 < nc)
+            load_one_col<U>(j + 1, cells, shift, rc, sc, ci, fromcell, pay, ridx, sidx, on, pay_off, nlo, nhi);
+        uint32_t t;
+        void *dst;
+        if (j < rc.n) { // no reference chosen between the two column sets: it would copy both to scratch
+            t = col_type(rc, j);
+            dst = col_ptr(out.left, j);
+        } else {
+            t = col_type(sc, j - rc.n);
+            dst = col_ptr(out.right, j - rc.n);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u])
+                store_elem(dst, t, opos[u], lo[u], hi[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            lo[u] = nlo[u];
+            hi[u] = nhi[u];
+        }
+    }
+}
+
 struct HtArgs {
     GlobalHT ht;
     HtMode mode;
@@ -1430,14 +1537,20 @@ __device__ __forceinline__ Cell16 *cell_at(const GlobalHT &ht, uint64_t i) {
     return reinterpret_cast<Cell16 *>(reinterpret_cast<unsigned char *>(ht.cells) + (i << cell_shift_of(ht)));
 }
 
+// Every 16-byte word of the table is written (a wide cell's inline columns
+// as zeros): whole cache lines leave L2, not lines with holes in them.
 __global__ void k_ht_init(GlobalHT ht, uint64_t n) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t wsh = cell_shift_of(ht) - 4; // log2 of 16-byte words per cell
+    const uint64_t words = n << wsh;
+    Cell16 *w16 = ht.cells;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) {
         Cell16 c;
-        c.key = kEmptyKey;
-        c.block_no = kInvalidRow;
+        const bool head = (i & ((1u << wsh) - 1)) == 0;
+        c.key = head ? kEmptyKey : 0;
+        c.block_no = head ? kInvalidRow : 0;
         c.row_no = 0;
-        *cell_at(ht, i) = c;
+        w16[i] = c;
     }
     const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g < ht.n_tables)
@@ -1591,7 +1704,7 @@ __device__ __forceinline__ uint32_t ht_lookup(const HtArgs &a, uint64_t k, uint3
 // thread taking consecutive output rows and finding their slot by binary
 // search over the slots' group-row ends (as the two-phase leaf emit does).
 template <int THREADS, int U>
-__global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, ColSet sc, OutSink out) {
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_ht_probe_emit(HtArgs a, ColSet rc, ColSet sc, OutSink out) {
     constexpr int kSlots = THREADS * U;
     constexpr int kV = 4; // group rows per thread per step
     __shared__ uint32_t tmp[THREADS / 64 + 1];
@@ -1624,15 +1737,31 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
                 }
             }
         }
+        // Two cells per step when the next cell shares the probe position's
+        // 128-byte line (loaded together, one line fetched): a wave waits for
+        // its longest probe sequence, and at load factor 0.5 most end within
+        // the first line.
+        const uint32_t sh = cell_shift_of(a.ht);
         while (pend) {
-            Cell16 c[U];
+            Cell16 c[U], c2[U];
+            bool two[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                c[u] = *cell_at(a.ht, cb[u] + cp[u]);
+            for (int u = 0; u < U; ++u) {
+                const uint64_t i0 = cb[u] + cp[u];
+                const uint64_t i1 = cb[u] + ((cp[u] + 1) & cm[u]);
+                two[u] = ((i0 << sh) >> 7) == ((i1 << sh) >> 7);
+                c[u] = *cell_at(a.ht, i0);
+                c2[u] = *cell_at(a.ht, two[u] ? i1 : i0); // no second line fetched
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (!((pend >> u) & 1u))
                     continue;
+                if (two[u] && c[u].key != key[u] && c[u].key != kEmptyKey) {
+                    // the first cell neither holds the key nor ends the sequence
+                    c[u] = c2[u];
+                    cp[u] = (cp[u] + 1) & cm[u];
+                }
                 if (c[u].key == key[u]) {
                     if ((c[u].row_no & kRefMore) && a.ht.grouped) {
                         grpm |= 1u << u;
@@ -1650,6 +1779,15 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
                     cp[u] = (cp[u] + 1) & cm[u];
                 }
             }
+        }
+        // 32-byte cells: the matched cell's inline build columns, read while
+        // its line is still cached (by the time the round's rows are written
+        // the line has left L2)
+        ulonglong2 pay[U];
+        if (sh == 5) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                pay[u] = *reinterpret_cast<const ulonglong2 *>(reinterpret_cast<const unsigned char *>(cell_at(a.ht, cb[u] + cp[u])) + 16);
         }
         // chain lengths, all chains of the thread one step per iteration;
         // a key with one build row has length 1 without a walk
@@ -1715,24 +1853,19 @@ __global__ __launch_bounds__(THREADS) void k_ht_probe_emit(HtArgs a, ColSet rc, 
         for (int u = 0; u < U; ++u)
             if (one[u])
                 out.keys[opos[u]] = key[u];
-        if (a.ht.pay.n) {
-            // one-row keys: build columns from the matched cell (just read)
+        {
+            // one-row keys take their build columns from the matched wide
+            // cell (just read); others (narrow cells, a one-row side chain of
+            // key kEmptyKey) from the build columns
             uint64_t ci[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                ci[u] = one[u] && !((many >> u) & 1u) ? cb[u] + cp[u] : 0u;
-            bool fromcell[U], fromcols[U];
+            bool fromcell[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                fromcell[u] = one[u] && !((many >> u) & 1u);
-                fromcols[u] = one[u] && ((many >> u) & 1u); // a one-row side chain (key kEmptyKey)
+                fromcell[u] = a.ht.pay.n && one[u] && !((many >> u) & 1u);
+                ci[u] = fromcell[u] ? cb[u] + cp[u] : 0u;
             }
-            gather_cells<U>(a.ht, rc, out.left, ci, opos, fromcell);
-            gather_set<U>(rc, out.left, ridx, opos, fromcols);
-        } else {
-            gather_set<U>(rc, out.left, ridx, opos, one);
+            emit_one_rows<U>(a.ht, rc, sc, out, ci, fromcell, pay, ridx, sidx, opos, one);
         }
-        gather_set<U>(sc, out.right, sidx, opos, one);
         if (a.ht.grouped) {
             uint32_t gcnt = 0;
 #pragma unroll
@@ -1991,7 +2124,7 @@ void ht_chj_plan(hipStream_t s, const uint64_t *keys, uint64_t rows, uint32_t n_
 
 void ht_init(hipStream_t s, const GlobalHT &ht, uint64_t total_cells, uint64_t, KernelTimer *tm) {
     TimedScope ts(tm, "ht_init", s);
-    const uint64_t n = std::max<uint64_t>(total_cells, ht.n_tables);
+    const uint64_t n = std::max<uint64_t>(total_cells << ((ht.cell_shift ? ht.cell_shift : 4u) - 4), ht.n_tables);
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n + 255) / 256, 256 * 64));
     k_ht_init<<<grid, 256, 0, s>>>(ht, total_cells);
 }
