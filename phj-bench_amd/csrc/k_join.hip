@@ -438,9 +438,6 @@ __device__ __forceinline__ uint32_t count_le(const uint32_t *a, uint32_t lo, uin
     }
 }
 
-// Multi-match rows are written by the whole workgroup, kEmitVec rows per
-// thread per step, at consecutive output positions.
-constexpr int kEmitVec = 4;
 // LDS for the staged build columns of the multi-match emit (beyond it they
 // are gathered from HBM).
 constexpr size_t kEmitRcolLds = 64 * 1024;
@@ -554,6 +551,97 @@ __device__ __forceinline__ void gather_set_lds(const unsigned char *base, uint32
     }
 }
 
+// Four consecutive output rows [p, p + 4) of one column. Packed: one store
+// of the four values (4 bytes for u8 up to 4 x 16 bytes for u128), for a
+// group whose rows are all written and whose first element is 4-aligned in a
+// column whose base allows it; otherwise row by row. A wave's narrow
+// row-per-lane stores (64 bytes per u8 instruction) cost about as much as its
+// 8-byte ones, so packing makes a column's cost follow its bytes.
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef uint64_t u64x2_t __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ void store4(T *dst, uint64_t p, const T (&v)[4], const bool (&on)[4], bool packed) {
+    if (packed) {
+        if constexpr (sizeof(T) == 1) {
+            const uint32_t x = static_cast<uint32_t>(v[0]) | static_cast<uint32_t>(v[1]) << 8 |
+                               static_cast<uint32_t>(v[2]) << 16 | static_cast<uint32_t>(v[3]) << 24;
+            *reinterpret_cast<uint32_t *>(dst + p) = x;
+        } else if constexpr (sizeof(T) == 2) {
+            const uint64_t x = static_cast<uint64_t>(v[0]) | static_cast<uint64_t>(v[1]) << 16 |
+                               static_cast<uint64_t>(v[2]) << 32 | static_cast<uint64_t>(v[3]) << 48;
+            *reinterpret_cast<uint64_t *>(dst + p) = x;
+        } else if constexpr (sizeof(T) == 4) {
+            *reinterpret_cast<u32x4_t *>(dst + p) = u32x4_t{v[0], v[1], v[2], v[3]};
+        } else if constexpr (sizeof(T) == 8) {
+            *reinterpret_cast<u64x2_t *>(dst + p) = u64x2_t{v[0], v[1]};
+            *reinterpret_cast<u64x2_t *>(dst + p + 2) = u64x2_t{v[2], v[3]};
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                dst[p + r] = v[r];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (on[r])
+                dst[p + r] = v[r];
+    }
+}
+
+// Column of four rows: values src[idx[r]] (LDS or memory) to rows [p, p + 4).
+template <class T>
+__device__ __forceinline__ void move4(const T *src, const uint32_t (&idx)[4], T *dst, uint64_t p, const bool (&on)[4],
+                                      bool packed) {
+    T v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        v[r] = src[idx[r]];
+    store4<T>(dst, p, v, on, packed);
+}
+
+// Every column of a set for four rows: the first `nlds` columns from their LDS
+// copy at `base` (lds_rows elements per column, emit_rcol_bytes apart), the
+// others from memory at idx + row0.
+__device__ __forceinline__ void move4_set(const unsigned char *base, uint32_t lds_rows, uint32_t nlds, const ColSet &in,
+                                          uint64_t row0, const OutColSet &out, const uint32_t (&idx)[4], uint64_t p,
+                                          const bool (&on)[4], bool packed) {
+    size_t off = 0;
+    for (uint32_t c = 0; c < in.n; ++c) {
+        const uint32_t t = col_type(in, c);
+        if (c < nlds) { // LDS and memory sources in separate calls: one pointer for both would be a generic one
+            const unsigned char *src = base + off;
+            off += emit_rcol_bytes(lds_rows, type_size(t));
+            switch (t) {
+            case 0: move4<uint8_t>(reinterpret_cast<const uint8_t *>(src), idx, optr<uint8_t>(out, c), p, on, packed); break;
+            case 1: move4<uint16_t>(reinterpret_cast<const uint16_t *>(src), idx, optr<uint16_t>(out, c), p, on, packed); break;
+            case 2: move4<uint32_t>(reinterpret_cast<const uint32_t *>(src), idx, optr<uint32_t>(out, c), p, on, packed); break;
+            case 3: move4<uint64_t>(reinterpret_cast<const uint64_t *>(src), idx, optr<uint64_t>(out, c), p, on, packed); break;
+            default: move4<u64x2_t>(reinterpret_cast<const u64x2_t *>(src), idx, optr<u64x2_t>(out, c), p, on, packed); break;
+            }
+        } else {
+            switch (t) {
+            case 0: move4<uint8_t>(cptr<uint8_t>(in, c) + row0, idx, optr<uint8_t>(out, c), p, on, packed); break;
+            case 1: move4<uint16_t>(cptr<uint16_t>(in, c) + row0, idx, optr<uint16_t>(out, c), p, on, packed); break;
+            case 2: move4<uint32_t>(cptr<uint32_t>(in, c) + row0, idx, optr<uint32_t>(out, c), p, on, packed); break;
+            case 3: move4<uint64_t>(cptr<uint64_t>(in, c) + row0, idx, optr<uint64_t>(out, c), p, on, packed); break;
+            default: move4<u64x2_t>(cptr<u64x2_t>(in, c) + row0, idx, optr<u64x2_t>(out, c), p, on, packed); break;
+            }
+        }
+    }
+}
+
+// Whether every column of `out` starts where 4-row groups of its elements are
+// aligned for move4's packed stores (4 bytes for u8 up to 16 bytes).
+__device__ __forceinline__ bool cols_pack_aligned(const OutColSet &out) {
+    bool ok = true;
+    for (uint32_t c = 0; c < out.n; ++c) {
+        const uint32_t w = type_size(col_type(out, c));
+        const uint32_t need = w * 4 < 16 ? w * 4 : 16;
+        ok &= (reinterpret_cast<uint64_t>(col_ptr(out, c)) & (need - 1)) == 0;
+    }
+    return ok;
+}
+
 // Emit pass: a streaming kernel over the probe rows of each work item using
 // the recorded heads; no table. Output row = probe key | build columns of the
 // matched row | probe columns (src/JoinOps.h:77-125), at the item's offset.
@@ -594,6 +682,8 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     }
     uint64_t round_base = a.out_base + a.item_off[w];
     uint32_t parity = 0;
+    const bool pack_ok = CHAINS && (reinterpret_cast<uint64_t>(a.out.keys) & 15) == 0 && cols_pack_aligned(a.out.left) &&
+                         cols_pack_aligned(a.out.right);
     for (uint32_t base = it.s0; base < it.s1; base += THREADS * U, parity ^= 1) {
         uint16_t h[U];
         uint32_t chainlen[U];
@@ -651,46 +741,42 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                 stage_scols<THREADS>(a.s_keys, a.s_cols, a.s_lds - 1, base, min(static_cast<uint32_t>(THREADS * U), it.s1 - base),
                                      THREADS * U, slds);
             __syncthreads();
+            // Output rows in aligned groups of four consecutive rows per
+            // thread: the group's rows mostly share a segment (one search,
+            // then a step to the next segment), and each column of a full
+            // group leaves as one packed store.
             const uint64_t pos0 = round_base + L.chain_start;
-            for (uint32_t k0 = 0; k0 < L.chain_total; k0 += THREADS * kEmitVec) {
-                uint32_t ridx[kEmitVec], sidx[kEmitVec];
-                uint64_t opos[kEmitVec];
-                bool on[kEmitVec];
+            const uint32_t lead = static_cast<uint32_t>(pos0 & 3u); // rows of the first group before pos0
+            const uint64_t a0 = pos0 - lead;
+            const uint32_t ngroups = (lead + L.chain_total + 3) / 4;
+            for (uint32_t g0 = 0; g0 < ngroups; g0 += THREADS) {
+                const uint32_t g = g0 + threadIdx.x;
+                const int64_t o0 = static_cast<int64_t>(g) * 4 - lead; // chain index of the group's first row
+                uint32_t oc = static_cast<uint32_t>(min<int64_t>(max<int64_t>(o0, 0), L.chain_total - 1));
+                uint32_t sg = count_le<THREADS * U>(seg_end, 0, oc); // first segment ending after oc
+                uint32_t ridx[4], sidx[4];
+                bool on[4];
 #pragma unroll
-                for (int v = 0; v < kEmitVec; ++v) {
-                    const uint32_t o = k0 + v * THREADS + threadIdx.x;
-                    on[v] = o < L.chain_total;
-                    const uint32_t oc = on[v] ? o : L.chain_total - 1;
-                    const uint32_t sg = count_le<THREADS * U>(seg_end, 0, oc); // first segment ending after oc
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t o = o0 + r;
+                    on[r] = g < ngroups && o >= 0 && o < static_cast<int64_t>(L.chain_total);
+                    if (on[r])
+                        oc = static_cast<uint32_t>(o); // rows without output repeat the last row with one
+                    if (oc >= seg_end[sg]) // the row starts a later segment
+                        sg = count_le<THREADS * U>(seg_end, 0, oc);
                     const uint32_t e = oc - (sg ? seg_end[sg - 1] : 0u);
-                    sidx[v] = (sg % U) * THREADS + sg / U; // round-relative
-                    ridx[v] = perm[seg_gs[sg] + e];         // leaf-local
-                    opos[v] = pos0 + o;
+                    sidx[r] = (sg % U) * THREADS + sg / U; // round-relative
+                    ridx[r] = perm[seg_gs[sg] + e];         // leaf-local
                 }
-                uint64_t kv[kEmitVec];
-                if (a.s_lds) {
-#pragma unroll
-                    for (int v = 0; v < kEmitVec; ++v)
-                        kv[v] = reinterpret_cast<const uint64_t *>(slds)[sidx[v]];
-                } else {
-#pragma unroll
-                    for (int v = 0; v < kEmitVec; ++v)
-                        kv[v] = a.s_keys[base + sidx[v]];
-                }
-#pragma unroll
-                for (int v = 0; v < kEmitVec; ++v)
-                    if (on[v])
-                        a.out.keys[opos[v]] = kv[v];
-                if (a.r_lds) {
-                    gather_set_lds<kEmitVec>(rlds, a.lds_max_rows, a.r_cols, a.out.left, ridx, opos, on);
-                } else {
-#pragma unroll
-                    for (int v = 0; v < kEmitVec; ++v)
-                        ridx[v] += it.r0;
-                    gather_set<kEmitVec>(a.r_cols, a.out.left, ridx, opos, on);
-                }
-                gather_set_lds<kEmitVec>(slds + emit_rcol_bytes(THREADS * U, 8), THREADS * U, a.s_cols, a.out.right, sidx,
-                                         opos, on, a.s_lds ? a.s_lds - 1 : 0u, base);
+                const uint64_t p = a0 + static_cast<uint64_t>(g) * 4;
+                const bool packed = pack_ok && on[0] && on[1] && on[2] && on[3];
+                if (a.s_lds)
+                    move4<uint64_t>(reinterpret_cast<const uint64_t *>(slds), sidx, a.out.keys, p, on, packed);
+                else
+                    move4<uint64_t>(a.s_keys + base, sidx, a.out.keys, p, on, packed);
+                move4_set(rlds, a.lds_max_rows, a.r_lds ? a.r_cols.n : 0u, a.r_cols, it.r0, a.out.left, ridx, p, on, packed);
+                move4_set(slds + emit_rcol_bytes(THREADS * U, 8), THREADS * U, a.s_lds ? a.s_lds - 1 : 0u, a.s_cols, base,
+                          a.out.right, sidx, p, on, packed);
             }
             __syncthreads(); // seg_end and seg_gs are rewritten by the next round
         }
