@@ -668,7 +668,8 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
     uint16_t *glen = perm + round_up16(a.lds_max_rows);
     uint32_t *seg_end = reinterpret_cast<uint32_t *>(smem + emit_chain_lds(a.lds_max_rows)); // [THREADS * U]
     uint16_t *seg_gs = reinterpret_cast<uint16_t *>(smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 4);
-    unsigned char *rlds = smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 6; // staged build columns
+    uint16_t *seg_sx = reinterpret_cast<uint16_t *>(smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 6);
+    unsigned char *rlds = smem + emit_chain_lds(a.lds_max_rows) + THREADS * U * 8; // staged build columns
     unsigned char *slds = rlds + (CHAINS && a.r_lds ? emit_rcols_bytes_dev(a.r_cols, a.lds_max_rows) : 0); // round's probe rows
     {
         const uint16_t *src = a.next16 + item_chain_base(a, it);
@@ -724,15 +725,29 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
             gather_set<U>(a.r_cols, a.out.left, ridx, opos, on);
             gather_set<U>(a.s_cols, a.out.right, sidx, opos, on);
         } else if (L.chain_total > 0) {
-            // Segment s = thread * U + u holds the rows of slot u's group, in
-            // output order; seg_end[s] is its exclusive end (round-relative).
-            // Output row o belongs to the first segment ending after o.
+            // The round's multi-match slots as a compact list of segments in
+            // output order (thread by thread, slot by slot): segment s holds
+            // the rows of one slot's group, seg_end[s] is its exclusive end
+            // (round-relative), seg_gs its group start, seg_sx its probe row.
+            // Output row o belongs to the first segment ending after o, and
+            // since no listed segment is empty, row o + 1 lies in that segment
+            // or the next one.
+            uint32_t ne = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                ne += chainlen[u] > 0 ? 1u : 0u;
+            uint32_t n_seg;
+            uint32_t k = block_excl_scan<THREADS>(ne, tmp, &n_seg);
             uint32_t end = L.chain_pos - L.chain_start;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                end += chainlen[u];
-                seg_end[threadIdx.x * U + u] = end;
-                seg_gs[threadIdx.x * U + u] = h[u] & kRowMask;
+                if (chainlen[u] > 0) {
+                    end += chainlen[u];
+                    seg_end[k] = end;
+                    seg_gs[k] = h[u] & kRowMask;
+                    seg_sx[k] = static_cast<uint16_t>(u * THREADS + threadIdx.x);
+                    ++k;
+                }
             }
             // The round's probe keys (and the leading probe columns that fit)
             // go to LDS with coalesced loads: the emission below then issues
@@ -753,7 +768,17 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                 const uint32_t g = g0 + threadIdx.x;
                 const int64_t o0 = static_cast<int64_t>(g) * 4 - lead; // chain index of the group's first row
                 uint32_t oc = static_cast<uint32_t>(min<int64_t>(max<int64_t>(o0, 0), L.chain_total - 1));
-                uint32_t sg = count_le<THREADS * U>(seg_end, 0, oc); // first segment ending after oc
+                // first segment ending after oc (the same number of steps in every lane)
+                uint32_t sg = 0;
+                for (uint32_t n = n_seg; n > 0;) {
+                    const uint32_t half = n >> 1;
+                    if (seg_end[sg + half] <= oc) {
+                        sg += half + 1;
+                        n -= half + 1;
+                    } else {
+                        n = half;
+                    }
+                }
                 uint32_t ridx[4], sidx[4];
                 bool on[4];
 #pragma unroll
@@ -762,11 +787,10 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                     on[r] = g < ngroups && o >= 0 && o < static_cast<int64_t>(L.chain_total);
                     if (on[r])
                         oc = static_cast<uint32_t>(o); // rows without output repeat the last row with one
-                    if (oc >= seg_end[sg]) // the row starts a later segment
-                        sg = count_le<THREADS * U>(seg_end, 0, oc);
+                    sg += oc >= seg_end[sg] ? 1u : 0u; // the row starts the next segment
                     const uint32_t e = oc - (sg ? seg_end[sg - 1] : 0u);
-                    sidx[r] = (sg % U) * THREADS + sg / U; // round-relative
-                    ridx[r] = perm[seg_gs[sg] + e];         // leaf-local
+                    sidx[r] = seg_sx[sg];           // round-relative
+                    ridx[r] = perm[seg_gs[sg] + e]; // leaf-local
                 }
                 const uint64_t p = a0 + static_cast<uint64_t>(g) * 4;
                 const bool packed = pack_ok && on[0] && on[1] && on[2] && on[3];
@@ -778,7 +802,7 @@ __global__ __launch_bounds__(THREADS) void k_leaf_emit(LeafJoinArgs a) {
                 move4_set(slds + emit_rcol_bytes(THREADS * U, 8), THREADS * U, a.s_lds ? a.s_lds - 1 : 0u, a.s_cols, base,
                           a.out.right, sidx, p, on, packed);
             }
-            __syncthreads(); // seg_end and seg_gs are rewritten by the next round
+            __syncthreads(); // the segment list is rewritten by the next round
         }
         round_base += L.total;
     }
@@ -2184,7 +2208,7 @@ void leaf_emit(hipStream_t s, const LeafJoinArgs &a, uint64_t max_work, bool cha
                sbytes + emit_rcol_bytes(kRoundRows, type_size(a.s_cols.types[b.s_lds - 1])) <= kEmitScolLds)
             sbytes += emit_rcol_bytes(kRoundRows, type_size(a.s_cols.types[b.s_lds++ - 1]));
         constexpr size_t kLdsMax = 160 * 1024;
-        const size_t fixed = lds_chains + kRoundRows * (4 + 2);
+        const size_t fixed = lds_chains + kRoundRows * (4 + 2 + 2);
         if (fixed + (b.r_lds ? rbytes : 0) + sbytes > kLdsMax) {
             b.s_lds = 0; // probe rows gathered from memory
             sbytes = 0;
