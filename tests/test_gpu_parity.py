@@ -623,3 +623,24 @@ def test_placement_probe(ctx):
     for rows, sets in ((1000, 1), (1 << 20, 0), (1 << 20, 9)):
         with pytest.raises(ValueError):
             ctx.placement_probe(rows, sets, reps=1)
+
+
+@pytest.mark.parametrize("build_types", [[G.U128], [G.U16, G.U32], [G.U8, G.U64], [G.U32, G.U16, G.U8],
+                                         [G.U64, G.U64], [G.U8, G.U128], [G.U64, G.U128, G.U64]])
+def test_chj_inline_cell_layouts(ctx, build_types):
+    """CHJ with the build columns carried inline in 32- and 64-byte cells (and
+    16-byte cells when they do not fit): every layout of the inline bytes, one
+    row per key and a few repeated keys, against the oracle."""
+    rng = np.random.default_rng(len(build_types) * 7 + build_types[0])
+    keys = np.unique(rng.integers(0, 2**64, 6000, dtype=np.uint64))
+    keys = np.concatenate([keys, keys[:50]])  # 50 keys with two build rows
+    rng.shuffle(keys)
+    b = random_relation(rng, keys.size, build_types)
+    b = O.Relation(keys, b.cols, b.types)
+    pk = np.where(rng.random(20000) < 0.7, rng.choice(keys, 20000), rng.integers(0, 2**64, 20000, dtype=np.uint64))
+    p = random_relation(rng, pk.size, [G.U64])
+    p = O.Relation(pk, p.cols, p.types)
+    want = O.serialize_sorted(O.reference_join(b, p))
+    res = ctx.runCHJ(host_rel(b), host_rel(p))
+    assert np.array_equal(sorted_rows_of(res.to_host()), want)
+    res.release()
