@@ -2483,28 +2483,17 @@ __global__ __launch_bounds__(THREADS) void k_group_count(GroupImage g, const uin
 // Rows o of a work item's output, G per lane at a time: entry e (probe row
 // sb + e) and build row b of each; every column moves G rows with the loads
 // issued before the stores.
-template <int G>
-__device__ __forceinline__ void group_move_col(const void *src, uint32_t t, void *dst, uint64_t dst_row0,
-                                               const uint64_t (&srow)[G], const uint32_t (&opos)[G],
-                                               const bool (&on)[G]) {
-    uint64_t lo[G], hi[G];
-#pragma unroll
-    for (int u = 0; u < G; ++u)
-        load_elem(src, t, on[u] ? srow[u] : 0u, lo[u], hi[u]);
-    char *d = static_cast<char *>(dst) + dst_row0 * type_size(t);
-#pragma unroll
-    for (int u = 0; u < G; ++u)
-        if (on[u])
-            store_elem(d, t, opos[u], lo[u], hi[u]);
-}
+// LDS for a group-join work item's probe keys and leading probe columns.
+constexpr size_t kGroupProbeLds = 16 * 1024;
 
 // Join of a range of probe rows against the group images. A work item of
 // THREADS * K probe rows looks every row up (LB in lock step), scans the
 // rows' match counts in probe-row order in LDS, reserves its output rows with
 // one atomic add, then writes them in order: output row o comes from the
 // entry e whose range [e_off[e], e_off[e] + count) holds o (binary search),
-// build row e_start[e] + (o - e_off[e]) of the grouped columns.
-template <int THREADS, int K, int G>
+// build row e_start[e] + (o - e_off[e]) of the grouped columns. Each thread
+// writes four consecutive output rows at a time.
+template <int THREADS, int K>
 __global__ __launch_bounds__(THREADS) void k_group_join(GroupImage g, const uint64_t *__restrict__ keys, ColSet cols,
                                                         uint64_t rows, OutSink out) {
     constexpr int LB = K < 4 ? K : 4;
@@ -2514,6 +2503,7 @@ __global__ __launch_bounds__(THREADS) void k_group_join(GroupImage g, const uint
     __shared__ uint32_t e_start[N];
     __shared__ uint32_t scan_tmp[THREADS / 64 + 1];
     __shared__ unsigned long long base_s;
+    __shared__ __align__(16) unsigned char sprobe[kGroupProbeLds];
     const uint64_t sb = static_cast<uint64_t>(blockIdx.x) * N;
     if (sb >= rows)
         return; // uniform
@@ -2556,36 +2546,51 @@ __global__ __launch_bounds__(THREADS) void k_group_join(GroupImage g, const uint
         return; // uniform: the block total
     if (threadIdx.x == 0)
         base_s = atomicAdd(out.cursor, static_cast<unsigned long long>(total));
+    // The item's probe keys and the leading probe columns that fit go to
+    // LDS, so the emission below reads its probe side without memory loads
+    // (a load issued after the previous rows' stores waits for them).
+    uint32_t nlds = 0; // probe columns staged
+    {
+        size_t used = emit_rcol_bytes(N, 8);
+        while (nlds < cols.n && used + emit_rcol_bytes(N, type_size(col_type(cols, nlds))) <= kGroupProbeLds)
+            used += emit_rcol_bytes(N, type_size(col_type(cols, nlds++)));
+        stage_scols<THREADS>(keys, cols, nlds, static_cast<uint32_t>(sb), n_here, N, sprobe);
+    }
     __syncthreads();
     const uint64_t base = base_s;
-    for (uint32_t o0 = 0; o0 < total; o0 += THREADS * G) {
-        uint64_t erow[G], brow[G];
-        uint32_t opos[G];
-        bool on[G];
+    const bool pack_ok = (reinterpret_cast<uint64_t>(out.keys) & 15) == 0 && cols_pack_aligned(out.left) &&
+                         cols_pack_aligned(out.right);
+    // Output rows in aligned groups of four consecutive rows per thread: the
+    // rows of a group mostly come from one probe row's consecutive build rows
+    // (one search per group), and each column of a full group leaves as one
+    // packed store.
+    const uint32_t lead = static_cast<uint32_t>(base & 3u);
+    const uint64_t a0 = base - lead;
+    const uint32_t ngroups = (lead + total + 3) / 4;
+    for (uint32_t g0 = 0; g0 < ngroups; g0 += THREADS) {
+        const uint32_t gi = g0 + threadIdx.x;
+        const int64_t o0 = static_cast<int64_t>(gi) * 4 - lead; // item-relative output row of the group's first row
+        uint32_t oc = static_cast<uint32_t>(min<int64_t>(max<int64_t>(o0, 0), total - 1));
+        uint32_t e = upper_bound_idx(e_off, N, oc); // the probe row whose rows hold oc (empty rows share its start)
+        uint32_t sidx[4], bidx[4];
+        bool on[4];
 #pragma unroll
-        for (int u = 0; u < G; ++u) {
-            const uint32_t o = o0 + static_cast<uint32_t>(u * THREADS) + threadIdx.x;
-            on[u] = o < total && base + o < out.cap;
-            const uint32_t x = on[u] ? o : 0u;
-            const uint32_t e = upper_bound_idx(e_off, N, x);
-            erow[u] = sb + e;
-            brow[u] = static_cast<uint64_t>(e_start[e]) + (x - e_off[e]);
-            opos[u] = o;
+        for (int r = 0; r < 4; ++r) {
+            const int64_t o = o0 + r;
+            on[r] = gi < ngroups && o >= 0 && o < static_cast<int64_t>(total) && base + o < out.cap;
+            if (on[r])
+                oc = static_cast<uint32_t>(o); // rows without output repeat the last row with one
+            const uint32_t next = e + 1 < N ? e_off[e + 1] : total;
+            if (oc >= next) // the row belongs to a later probe row
+                e = upper_bound_idx(e_off, N, oc);
+            sidx[r] = e;
+            bidx[r] = e_start[e] + (oc - e_off[e]);
         }
-        {
-            uint64_t kv[G];
-#pragma unroll
-            for (int u = 0; u < G; ++u)
-                kv[u] = keys[on[u] ? erow[u] : sb];
-#pragma unroll
-            for (int u = 0; u < G; ++u)
-                if (on[u])
-                    out.keys[base + opos[u]] = kv[u];
-        }
-        for (uint32_t c = 0; c < g.cols.n; ++c)
-            group_move_col<G>(col_ptr(g.cols, c), col_type(g.cols, c), col_ptr(out.left, c), base, brow, opos, on);
-        for (uint32_t c = 0; c < cols.n; ++c)
-            group_move_col<G>(col_ptr(cols, c), col_type(cols, c), col_ptr(out.right, c), base, erow, opos, on);
+        const uint64_t p = a0 + static_cast<uint64_t>(gi) * 4;
+        const bool packed = pack_ok && on[0] && on[1] && on[2] && on[3];
+        move4<uint64_t>(reinterpret_cast<const uint64_t *>(sprobe), sidx, out.keys, p, on, packed);
+        move4_set(nullptr, 0, 0u, g.cols, 0, out.left, bidx, p, on, packed);
+        move4_set(sprobe + emit_rcol_bytes(N, 8), N, nlds, cols, sb, out.right, sidx, p, on, packed);
     }
 }
 
@@ -2622,11 +2627,11 @@ void group_count(hipStream_t s, const GroupImage &g, const uint64_t *keys, uint6
 
 void group_join(hipStream_t s, const GroupImage &g, const uint64_t *keys, const ColSet &cols, uint64_t rows,
                 const OutSink &out, KernelTimer *tm) {
-    constexpr int T = 256, K = 2, G = 8; // short items: each probe row may expand to many output rows
+    constexpr int T = 256, K = 2; // short items: each probe row may expand to many output rows
     if (rows == 0)
         return;
     TimedScope ts(tm, "leaf_join", s);
-    k_group_join<T, K, G><<<checked_grid((rows + T * K - 1) / (T * K), T, "group_join"), T, 0, s>>>(g, keys, cols, rows,
+    k_group_join<T, K><<<checked_grid((rows + T * K - 1) / (T * K), T, "group_join"), T, 0, s>>>(g, keys, cols, rows,
                                                                                                       out);
 }
 
