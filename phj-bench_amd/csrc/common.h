@@ -95,6 +95,41 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds, u
     return off;
 }
 
+// block_excl_scan of two values at once (the same barriers): `lds` needs
+// 2 * (THREADS/64 + 1) words.
+template <int THREADS>
+__device__ __forceinline__ void block_excl_scan2(uint32_t v0, uint32_t v1, uint32_t *lds, uint32_t &o0, uint32_t &o1,
+                                                 uint32_t &t0, uint32_t &t1) {
+    constexpr int W = THREADS / 64;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64;
+    const uint32_t i0 = wave_incl_scan(v0);
+    const uint32_t i1 = wave_incl_scan(v1);
+    if (lane == 63) {
+        lds[wave] = i0;
+        lds[W + 1 + wave] = i1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a0 = 0, a1 = 0;
+        for (int w = 0; w < W; ++w) {
+            const uint32_t x0 = lds[w], x1 = lds[W + 1 + w];
+            lds[w] = a0;
+            lds[W + 1 + w] = a1;
+            a0 += x0;
+            a1 += x1;
+        }
+        lds[W] = a0;
+        lds[2 * W + 1] = a1;
+    }
+    __syncthreads();
+    o0 = lds[wave] + i0 - v0;
+    o1 = lds[W + 1 + wave] + i1 - v1;
+    t0 = lds[W];
+    t1 = lds[2 * W + 1];
+    __syncthreads();
+}
+
 // Largest i in [0, n) with a[i] <= x, for a non-decreasing array with a[0] <= x.
 template <class T>
 __device__ __forceinline__ uint32_t upper_bound_idx(const T *a, uint32_t n, T x) {

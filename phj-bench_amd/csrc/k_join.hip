@@ -1800,6 +1800,9 @@ __device__ __forceinline__ uint32_t ht_lookup(const HtArgs &a, uint64_t k, uint3
     }
 }
 
+// LDS for the probe columns of a round's grouped slots.
+constexpr size_t kHtGroupProbeLds = 8 * 1024;
+
 // Probe + emit against HBM tables (CHJ, oversize leaves). Per round of
 // THREADS*U probe rows: all U lookups of a thread advance in lock step (one
 // probe step each per iteration, their cell loads in flight together), chain
@@ -1816,11 +1819,12 @@ __device__ __forceinline__ uint32_t ht_lookup(const HtArgs &a, uint64_t k, uint3
 template <int THREADS, int U>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) void k_ht_probe_emit(HtArgs a, ColSet rc, ColSet sc, OutSink out) {
     constexpr int kSlots = THREADS * U;
-    constexpr int kV = 4; // group rows per thread per step
-    __shared__ uint32_t tmp[THREADS / 64 + 1];
+    __shared__ uint32_t tmp[2 * (THREADS / 64 + 1)];
     __shared__ unsigned long long out_base;
-    __shared__ uint32_t g_end[kSlots], g_pos[kSlots], g_st[kSlots], g_row[kSlots];
+    // the round's grouped slots, listed compactly (see the group rows below)
+    __shared__ uint32_t g_end[kSlots], g_st[kSlots], g_row[kSlots];
     __shared__ uint64_t g_key[kSlots];
+    __shared__ __align__(16) unsigned char g_probe[kHtGroupProbeLds];
     const uint64_t n = ht_total_rows(a);
     const uint64_t per_round = static_cast<uint64_t>(gridDim.x) * THREADS * U;
     for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * THREADS * U; base < n; base += per_round) {
@@ -1931,12 +1935,21 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
                     walking &= ~(1u << u);
             }
         }
-        uint32_t cnt = 0;
+        // The round's output: its one-match and walked rows thread by thread,
+        // then the rows of its grouped keys as one contiguous range.
+        uint32_t cnt = 0, gcnt = 0, ne = 0;
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            cnt += len[u];
-        uint32_t round_total;
-        const uint32_t off = block_excl_scan<THREADS>(cnt, tmp, &round_total);
+        for (int u = 0; u < U; ++u) {
+            if ((grpm >> u) & 1u) {
+                gcnt += len[u];
+                ne += len[u] > 0 ? 1u : 0u;
+            } else {
+                cnt += len[u];
+            }
+        }
+        uint32_t off, g_off, t_ow, g_total;
+        block_excl_scan2<THREADS>(cnt, gcnt, tmp, off, g_off, t_ow, g_total);
+        const uint32_t round_total = t_ow + g_total;
         if (round_total == 0)
             continue;
         if (threadIdx.x == 0)
@@ -1954,8 +1967,8 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             opos[u] = pos;
-            pos += len[u];
-            one[u] = len[u] == 1 && opos[u] < out.cap;
+            pos += (grpm >> u) & 1u ? 0u : len[u];
+            one[u] = len[u] == 1 && !((grpm >> u) & 1u) && opos[u] < out.cap;
             ridx[u] = one[u] ? head[u] : 0u;
             sidx[u] = static_cast<uint32_t>(prow[u]);
         }
@@ -1976,50 +1989,83 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6))) vo
             }
             emit_one_rows<U>(a.ht, rc, sc, out, ci, fromcell, pay, ridx, sidx, opos, one);
         }
-        if (a.ht.grouped) {
-            uint32_t gcnt = 0;
+        if (g_total > 0) { // uniform
+            // Grouped slots with rows, in output order: slot k's rows end at
+            // g_end[k] (relative to the group range), its key's build rows
+            // start at g_st[k] of the grouped columns; its probe row and key,
+            // and its probe columns when they fit, are kept in LDS.
+            uint32_t n_seg;
+            uint32_t k = block_excl_scan<THREADS>(ne, tmp, &n_seg);
+            uint32_t nlds = 0; // probe columns staged: all or none
+            {
+                size_t used = 0;
+                for (uint32_t c = 0; c < sc.n; ++c)
+                    used += emit_rcol_bytes(kSlots, type_size(col_type(sc, c)));
+                nlds = used <= kHtGroupProbeLds ? sc.n : 0u;
+            }
+            uint32_t e = g_off;
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                gcnt += (grpm >> u) & 1u ? len[u] : 0u;
-            uint32_t g_total;
-            const uint32_t g_off = block_excl_scan<THREADS>(gcnt, tmp, &g_total);
-            if (g_total > 0) {
-                uint32_t e = g_off;
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t sl = threadIdx.x * U + u;
-                    const bool gm = (grpm >> u) & 1u;
-                    e += gm ? len[u] : 0u;
-                    g_end[sl] = e;
-                    g_pos[sl] = static_cast<uint32_t>(opos[u] - out_base);
-                    g_st[sl] = gm ? gst[u] : 0u;
-                    g_row[sl] = static_cast<uint32_t>(prow[u]);
-                    g_key[sl] = key[u];
-                }
-                __syncthreads();
-                for (uint32_t q0 = 0; q0 < g_total; q0 += THREADS * kV) {
-                    uint32_t bidx[kV], gsidx[kV];
-                    uint64_t gpos[kV], kv[kV];
-                    bool gon[kV];
-#pragma unroll
-                    for (int v = 0; v < kV; ++v) {
-                        const uint32_t q = q0 + v * THREADS + threadIdx.x;
-                        const uint32_t qc = q < g_total ? q : g_total - 1;
-                        const uint32_t sg = count_le<kSlots>(g_end, 0, qc); // first slot whose group rows end after qc
-                        const uint32_t ei = qc - (sg ? g_end[sg - 1] : 0u);
-                        bidx[v] = g_st[sg] + ei;
-                        gsidx[v] = g_row[sg];
-                        gpos[v] = out_base + g_pos[sg] + ei;
-                        kv[v] = g_key[sg];
-                        gon[v] = q < g_total && gpos[v] < out.cap;
+            for (int u = 0; u < U; ++u) {
+                if (((grpm >> u) & 1u) && len[u] > 0) {
+                    e += len[u];
+                    g_end[k] = e;
+                    g_st[k] = gst[u];
+                    g_row[k] = static_cast<uint32_t>(prow[u]);
+                    g_key[k] = key[u];
+                    size_t co = 0;
+                    for (uint32_t c = 0; c < nlds; ++c) {
+                        const uint32_t t = col_type(sc, c);
+                        uint64_t lo, hi;
+                        load_elem(col_ptr(sc, c), t, prow[u], lo, hi);
+                        store_elem(g_probe + co, t, k, lo, hi);
+                        co += emit_rcol_bytes(kSlots, type_size(t));
                     }
-#pragma unroll
-                    for (int v = 0; v < kV; ++v)
-                        if (gon[v])
-                            out.keys[gpos[v]] = kv[v];
-                    gather_set<kV>(a.ht.grp, out.left, bidx, gpos, gon);
-                    gather_set<kV>(sc, out.right, gsidx, gpos, gon);
+                    ++k;
                 }
+            }
+            __syncthreads();
+            // Output rows in aligned groups of four consecutive rows per
+            // thread: a group's rows mostly belong to one slot's consecutive
+            // build rows (one search per group, then a step to the next slot),
+            // and each column of a full group leaves as one packed store.
+            const uint64_t gbase = out_base + t_ow;
+            const bool pack_ok = (reinterpret_cast<uint64_t>(out.keys) & 15) == 0 && cols_pack_aligned(out.left) &&
+                                 cols_pack_aligned(out.right);
+            const uint32_t lead = static_cast<uint32_t>(gbase & 3u);
+            const uint64_t a0 = gbase - lead;
+            const uint32_t ngroups = (lead + g_total + 3) / 4;
+            for (uint32_t q0 = 0; q0 < ngroups; q0 += THREADS) {
+                const uint32_t gi = q0 + threadIdx.x;
+                const int64_t o0 = static_cast<int64_t>(gi) * 4 - lead; // group-range row of the first row
+                uint32_t oc = static_cast<uint32_t>(min<int64_t>(max<int64_t>(o0, 0), g_total - 1));
+                uint32_t sg = 0; // first slot ending after oc (the same number of steps in every lane)
+                for (uint32_t m = n_seg; m > 0;) {
+                    const uint32_t half = m >> 1;
+                    if (g_end[sg + half] <= oc) {
+                        sg += half + 1;
+                        m -= half + 1;
+                    } else {
+                        m = half;
+                    }
+                }
+                uint32_t bidx[4], slot[4], sidx[4];
+                bool on[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t o = o0 + r;
+                    on[r] = gi < ngroups && o >= 0 && o < static_cast<int64_t>(g_total) && gbase + o < out.cap;
+                    if (on[r])
+                        oc = static_cast<uint32_t>(o); // rows without output repeat the last row with one
+                    sg += oc >= g_end[sg] ? 1u : 0u; // the row starts the next slot's rows
+                    bidx[r] = g_st[sg] + (oc - (sg ? g_end[sg - 1] : 0u));
+                    slot[r] = sg;
+                    sidx[r] = nlds ? sg : g_row[sg]; // staged probe columns by slot, else by probe row
+                }
+                const uint64_t p = a0 + static_cast<uint64_t>(gi) * 4;
+                const bool packed = pack_ok && on[0] && on[1] && on[2] && on[3];
+                move4<uint64_t>(g_key, slot, out.keys, p, on, packed);
+                move4_set(nullptr, 0, 0u, a.ht.grp, 0, out.left, bidx, p, on, packed);
+                move4_set(g_probe, kSlots, nlds, sc, 0, out.right, sidx, p, on, packed);
             }
         }
 #pragma unroll
