@@ -906,7 +906,8 @@ __device__ __forceinline__ void sweep_store(void *dst, const uint32_t (&meta)[K]
 template <int THREADS, int K, int MODE, class T>
 __device__ __forceinline__ void sweep_col(const void *src, void *dst, const uint32_t (&meta)[K],
                                           const uint32_t (&rows)[K], uint32_t tid) {
-    constexpr int G = sizeof(T) >= 8 ? static_cast<int>(32 / sizeof(T)) : 8; // <= 8 VGPRs of values per group
+    constexpr int G0 = sizeof(T) >= 8 ? static_cast<int>(64 / sizeof(T)) : 16; // <= 16 VGPRs of values per group
+    constexpr int G = G0 < K ? G0 : K;
     static_assert(K % G == 0, "slots in groups");
     // group h + 1's loads are issued before group h's stores
     T cur[G];
