@@ -340,6 +340,8 @@ def headline(args, tuning):
     ctx.set_kernel_timing(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step (wall), e2e events {['%.2f' % x for x in e2e]}")
+    marked = ctx.placement_stats()
+    log(f"scatter placement watch: {marked[0]} output blocks marked slow, {marked[1] >> 20} MiB marked now")
 
     if args.calib_copy_mib:
         ms = ctx.copy_bench(args.calib_copy_mib << 20, reps=1)
@@ -400,6 +402,7 @@ def headline(args, tuning):
     ctx.close()  # each workload below runs in a context of its own, on a device with nothing cached
     return dict(R=R, S=S, rb=rb, rp=rp, pass_bits=pass_bits, ms_per_step=ms_per_step, tuples_per_s=tuples_per_s,
                 best_e2e=best_e2e, b_alg=b_alg, out_rows=out_rows, parity=parity, phases=phases, ktimes=ktimes,
+                marked=marked,
                 roofline=roofline, n=n)
 
 
@@ -535,6 +538,7 @@ def main():
         "phase_ms_last": None if h is None else dict(zip(("build_shuffle", "probe_shuffle", "build", "probe"),
                                                           h["phases"][-1])),
         "kernel_ms_per_step": None if h is None else {k: round(v[0] / h["n"], 4) for k, v in sorted(h["ktimes"].items())},
+        "placement_marked": None if h is None else {"blocks": h["marked"][0], "bytes": h["marked"][1]},
         "batched": batched,
         "skew": skew,
         "serving": serving,

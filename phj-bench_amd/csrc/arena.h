@@ -10,6 +10,11 @@
 // than 60% of the device's memory sits free in this pool, and only after the
 // stream's queued work (which may still use them) has finished.
 //
+// Ranges marked slow (mark_slow) are avoided by alloc_avoiding(), which places
+// a block outside them (in a free block's unmarked part, else in a new
+// segment); alloc() uses them as usual. A range is forgotten when its segment
+// goes back to the backend.
+//
 // Backend interface (static members): Stream; void *alloc(uint64_t) (null on
 // failure); void free(void *); void sync(Stream); bool mem_info(uint64_t
 // &free, uint64_t &total). The header needs no HIP, so the CPU tests drive the
@@ -23,6 +28,8 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <utility>
+#include <vector>
 
 namespace phjd {
 
@@ -46,8 +53,10 @@ public:
         stream_ = s;
         bound_ = true;
         uint64_t free_mem = 0, total = 0;
-        if (Dev::mem_info(free_mem, total))
+        if (Dev::mem_info(free_mem, total)) {
             cache_limit_ = total / 10 * 6;
+            slow_limit_ = total / 5;
+        }
     }
 
     void *alloc(uint64_t bytes) {
@@ -64,12 +73,39 @@ public:
         }
         if (b == nullptr)
             throw ArenaError(ArenaError::kOutOfMemory, "device allocation of " + std::to_string(sz) + " bytes failed");
-        live_[b->ptr] = b;
-        free_bytes_ -= b->size;
-        current_ += b->size;
-        peak_ = std::max(peak_, current_);
-        return b->ptr;
+        return commit(b);
     }
+
+    // alloc() outside the slow ranges; falls back to alloc() when neither a
+    // free block nor a new segment gives such a place.
+    void *alloc_avoiding(uint64_t bytes) {
+        if (slow_.empty())
+            return alloc(bytes);
+        const uint64_t sz = (std::max<uint64_t>(bytes, 1) + kAlign - 1) & ~(kAlign - 1);
+        if (limit_ != 0 && current_ + sz > limit_)
+            return alloc(bytes); // throws
+        Block *b = take_avoiding(sz);
+        if (b == nullptr && grow_segment(sz))
+            b = take_avoiding(sz);
+        return b != nullptr ? commit(b) : alloc(bytes);
+    }
+
+    // Marks [p, p + bytes) slow. Returns false (and marks nothing) when the
+    // range is already marked or the marked bytes would pass a fifth of the
+    // device's memory.
+    bool mark_slow(const void *p, uint64_t bytes) {
+        char *a = static_cast<char *>(const_cast<void *>(p));
+        for (auto &r : slow_)
+            if (r.first <= a && a + bytes <= r.second)
+                return false;
+        if (slow_bytes_ + bytes > slow_limit_)
+            return false;
+        slow_.emplace_back(a, a + bytes);
+        slow_bytes_ += bytes;
+        return true;
+    }
+    uint64_t slow_bytes() const { return slow_bytes_; }
+    size_t slow_ranges() const { return slow_.size(); }
 
     void free(void *p) {
         if (p == nullptr)
@@ -107,6 +143,7 @@ public:
             Block *b = it->second;
             auto seg = segments_.find(b->ptr);
             if (seg != segments_.end() && b->prev == nullptr && b->next == nullptr && b->size == seg->second) {
+                forget_slow(b->ptr, b->ptr + b->size);
                 Dev::free(b->ptr);
                 free_bytes_ -= b->size;
                 segments_.erase(seg);
@@ -172,6 +209,12 @@ private:
             return nullptr;
         Block *b = it->second;
         free_.erase(it);
+        return split_take(b, sz);
+    }
+
+    // Marks b (already out of free_) used for sz bytes; the rest stays free
+    // when it is worth keeping.
+    Block *split_take(Block *b, uint64_t sz) {
         if (b->size - sz >= kMinSplit) {
             Block *r = new Block{b->ptr + sz, b->size - sz, true, b, b->next};
             if (b->next)
@@ -184,15 +227,76 @@ private:
         return b;
     }
 
-    Block *grow(uint64_t sz) {
+    Block *grow(uint64_t sz) { return grow_segment(sz) ? take(sz) : nullptr; }
+
+    bool grow_segment(uint64_t sz) {
         const uint64_t seg = std::max(sz, kSegmentBytes);
         char *p = static_cast<char *>(Dev::alloc(seg));
         if (p == nullptr)
-            return nullptr;
+            return false;
+        forget_slow(p, p + seg); // an address range the backend handed out again
         segments_[p] = seg;
         free_bytes_ += seg;
         free_.emplace(seg, new Block{p, seg, true, nullptr, nullptr});
-        return take(sz);
+        return true;
+    }
+
+    void *commit(Block *b) {
+        live_[b->ptr] = b;
+        free_bytes_ -= b->size;
+        current_ += b->size;
+        peak_ = std::max(peak_, current_);
+        return b->ptr;
+    }
+
+    // Lowest aligned start in [lo, hi) of sz bytes clear of every slow range.
+    char *clear_start(char *lo, char *hi, uint64_t sz) const {
+        char *st = lo;
+        for (bool moved = true; moved && st + sz <= hi;) {
+            moved = false;
+            for (auto &r : slow_)
+                if (r.first < st + sz && st < r.second) {
+                    const uint64_t off = static_cast<uint64_t>(r.second - lo);
+                    st = lo + ((off + kAlign - 1) & ~(kAlign - 1));
+                    moved = true;
+                }
+        }
+        return st + sz <= hi ? st : nullptr;
+    }
+
+    // Smallest free block with sz bytes clear of the slow ranges; the part
+    // in front of the placement stays free as a block of its own.
+    Block *take_avoiding(uint64_t sz) {
+        for (auto it = free_.lower_bound(sz); it != free_.end(); ++it) {
+            Block *b = it->second;
+            char *st = clear_start(b->ptr, b->ptr + b->size, sz);
+            if (st == nullptr)
+                continue;
+            free_.erase(it);
+            if (st == b->ptr)
+                return split_take(b, sz);
+            const uint64_t front = static_cast<uint64_t>(st - b->ptr);
+            Block *f = new Block{b->ptr, front, true, b->prev, b};
+            if (b->prev)
+                b->prev->next = f;
+            b->prev = f;
+            b->ptr = st;
+            b->size -= front;
+            free_.emplace(f->size, f);
+            return split_take(b, sz);
+        }
+        return nullptr;
+    }
+
+    void forget_slow(char *lo, char *hi) {
+        for (size_t i = 0; i < slow_.size();)
+            if (slow_[i].first >= lo && slow_[i].first < hi) {
+                slow_bytes_ -= static_cast<uint64_t>(slow_[i].second - slow_[i].first);
+                slow_[i] = slow_.back();
+                slow_.pop_back();
+            } else {
+                ++i;
+            }
     }
 
     Stream stream_{};
@@ -203,6 +307,8 @@ private:
     uint64_t current_ = 0, peak_ = 0, free_bytes_ = 0;
     uint64_t cache_limit_ = 64ull << 30;
     uint64_t limit_ = 0;
+    std::vector<std::pair<char *, char *>> slow_; // [begin, end) address ranges
+    uint64_t slow_bytes_ = 0, slow_limit_ = 16ull << 30;
 };
 
 } // namespace phjd

@@ -269,6 +269,84 @@ struct phj_ctx {
     uint32_t batch_leaf_rows[kBatchClasses];
     phj_ctx() { std::fill(std::begin(batch_leaf_rows), std::end(batch_leaf_rows), kBatchDefaultLeafRows); }
 
+    // Placement watch of the radix scatter. The same scatter runs up to 30%
+    // slower into some placements of its output than into others, reproducibly
+    // for a given block within a process (measured on MI355X at 2^30 rows; the
+    // cause is not in the kernel). Passes over at least kWatchRows rows are
+    // timed; one slower than the fastest pass of its shape by more than 15%
+    // marks its largest output block slow in the pool, and later pass outputs
+    // are placed clear of the marked blocks.
+    struct PassWatch {
+        uint64_t shape = 0;
+        hipEvent_t a = nullptr, b = nullptr;
+        void *block = nullptr; // the pass's largest output block
+        uint64_t bytes = 0;
+        float ms = -1;
+    };
+    static constexpr uint64_t kWatchRows = 1ull << 26;
+    static constexpr size_t kWatchKeep = 64;
+    std::vector<PassWatch> watch_open, watch_done;
+    std::vector<hipEvent_t> watch_events; // unused events, for reuse
+    std::unordered_map<uint64_t, float> watch_best;
+    uint64_t watch_marked = 0;
+
+    hipEvent_t watch_event() {
+        if (!watch_events.empty()) {
+            hipEvent_t e = watch_events.back();
+            watch_events.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return e;
+    }
+    // Reads the finished timed passes (never waits) and marks slow blocks.
+    void watch_review() {
+        for (size_t i = 0; i < watch_open.size();) {
+            PassWatch &w = watch_open[i];
+            if (hipEventQuery(w.b) != hipSuccess) {
+                (void)hipGetLastError();
+                ++i;
+                continue;
+            }
+            if (hipEventElapsedTime(&w.ms, w.a, w.b) != hipSuccess) {
+                (void)hipGetLastError();
+                w.ms = -1;
+            }
+            watch_events.push_back(w.a);
+            watch_events.push_back(w.b);
+            if (w.ms > 0) {
+                auto it = watch_best.find(w.shape);
+                if (it == watch_best.end() || w.ms < it->second)
+                    watch_best[w.shape] = w.ms;
+                if (watch_done.size() == kWatchKeep)
+                    watch_done.erase(watch_done.begin());
+                watch_done.push_back(w);
+            }
+            watch_open[i] = watch_open.back();
+            watch_open.pop_back();
+        }
+        for (auto &w : watch_done)
+            if (w.block != nullptr && w.ms > 1.15f * watch_best[w.shape]) {
+                if (arena.mark_slow(w.block, w.bytes))
+                    ++watch_marked;
+                w.block = nullptr;
+            }
+    }
+    void watch_release() {
+        for (auto &w : watch_open) {
+            (void)hipEventDestroy(w.a);
+            (void)hipEventDestroy(w.b);
+        }
+        for (auto e : watch_events)
+            (void)hipEventDestroy(e);
+        watch_open.clear();
+        watch_events.clear();
+    }
+
     KernelTimer *tm() { return timing ? &timer : nullptr; }
     void sync() {
         HIPCHK(hipGetLastError()); // launch failures (bad configuration, LDS size)
@@ -425,12 +503,14 @@ void max_part_wait(phj_ctx *ctx, Partitioned &P) {
     P.max_part = *ctx->pinned;
 }
 
-OutColSet alloc_cols(phj_ctx *ctx, const ColSet &like, uint64_t rows, std::vector<void *> &track) {
+OutColSet alloc_cols(phj_ctx *ctx, const ColSet &like, uint64_t rows, std::vector<void *> &track,
+                     bool avoid_slow = false) {
     OutColSet o{};
     o.n = like.n;
     for (uint32_t c = 0; c < like.n; ++c) {
         o.types[c] = like.types[c] == kTypeRowIndex ? 2u : like.types[c]; // synthesised row indices land as u32
-        o.ptr[c] = ctx->arena.alloc(std::max<uint64_t>(rows * type_size(o.types[c]), 1));
+        const uint64_t bytes = std::max<uint64_t>(rows * type_size(o.types[c]), 1);
+        o.ptr[c] = avoid_slow ? ctx->arena.alloc_avoiding(bytes) : ctx->arena.alloc(bytes);
         track.push_back(o.ptr[c]);
     }
     return o;
@@ -485,10 +565,14 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
             dig.next_shift = 64 - consumed - passes[p + 1];
             dig.next_bits = passes[p + 1];
         }
+        const bool watch = rows >= phj_ctx::kWatchRows && ctx->watch_open.size() < phj_ctx::kWatchKeep;
+        if (watch)
+            ctx->watch_review();
         Held bufs(ctx);
-        uint64_t *nkeys = ctx->alloc<uint64_t>(std::max<uint64_t>(rows, 1));
+        uint64_t *nkeys = static_cast<uint64_t *>(watch ? ctx->arena.alloc_avoiding(rows * sizeof(uint64_t))
+                                                        : ctx->arena.alloc(std::max<uint64_t>(rows, 1) * sizeof(uint64_t)));
         bufs.push_back(nkeys);
-        OutColSet ncols = alloc_cols(ctx, cur_cols, rows, bufs);
+        OutColSet ncols = alloc_cols(ctx, cur_cols, rows, bufs, watch);
         uint32_t *nseg = ctx->alloc<uint32_t>(out_nseg + 1);
         bufs.push_back(nseg);
         const uint64_t max_tiles = radix_max_tiles(rows, cur_nseg);
@@ -496,8 +580,43 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
         rs.seg_tile_first = scr.get<uint32_t>(2 * (static_cast<uint64_t>(cur_nseg) + 1));
         rs.hist = scr.get<uint32_t>(max_tiles * B);
         rs.block_sums = scr.get<uint32_t>(radix_max_chunks(rows, cur_nseg) * B);
+        phj_ctx::PassWatch w;
+        if (watch) {
+            uint64_t row_bytes = sizeof(uint64_t);
+            w.block = nkeys;
+            w.bytes = rows * sizeof(uint64_t);
+            for (uint32_t c = 0; c < ncols.n; ++c) {
+                const uint64_t cb = rows * type_size(ncols.types[c]);
+                row_bytes += type_size(ncols.types[c]);
+                if (cb > w.bytes) {
+                    w.block = ncols.ptr[c];
+                    w.bytes = cb;
+                }
+            }
+            w.shape = rows * 0x9E3779B97F4A7C15ull ^ row_bytes << 40 ^ static_cast<uint64_t>(bits) << 50 ^
+                      static_cast<uint64_t>(p) << 56 ^ static_cast<uint64_t>(cur_nseg > 1) << 62;
+            w.a = ctx->watch_event();
+            w.b = ctx->watch_event();
+            if (w.a == nullptr || w.b == nullptr || hipEventRecord(w.a, ctx->stream) != hipSuccess) {
+                (void)hipGetLastError();
+                if (w.a)
+                    ctx->watch_events.push_back(w.a);
+                if (w.b)
+                    ctx->watch_events.push_back(w.b);
+                w.a = w.b = nullptr;
+            }
+        }
         radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, uniform_bits(shift, bits), nseg, rs,
                    flags, ctx->tm(), dig);
+        if (w.a != nullptr) {
+            if (hipEventRecord(w.b, ctx->stream) == hipSuccess) {
+                ctx->watch_open.push_back(w);
+            } else {
+                (void)hipGetLastError();
+                ctx->watch_events.push_back(w.a);
+                ctx->watch_events.push_back(w.b);
+            }
+        }
         // previous pass output is dead once this pass has consumed it
         prev_bufs.free_all();
         prev_bufs.swap(bufs);
@@ -2840,6 +2959,7 @@ phj_status phj_ctx_destroy(phj_ctx *ctx) {
         for (void *p : kv.second)
             ctx->arena.free(p);
     ctx->owned.clear();
+    ctx->watch_release();
     ctx->arena.release_cached();
     (void)hipEventDestroy(ctx->note_event);
     (void)hipHostFree(ctx->pinned);
@@ -2859,6 +2979,15 @@ phj_status phj_dev_alloc(phj_ctx *ctx, uint64_t bytes, void **out) {
         if (out == nullptr)
             invalid("null output pointer");
         *out = ctx->arena.alloc(bytes);
+    });
+}
+
+phj_status phj_placement_stats(phj_ctx *ctx, uint64_t *marked_blocks, uint64_t *marked_bytes) {
+    return guard(ctx, [&] {
+        if (marked_blocks)
+            *marked_blocks = ctx->watch_marked;
+        if (marked_bytes)
+            *marked_bytes = ctx->arena.slow_bytes();
     });
 }
 

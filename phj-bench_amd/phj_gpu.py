@@ -165,6 +165,7 @@ _SIGNATURES = {
     "phj_memcpy_h2d": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
     "phj_memcpy_d2h": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
     "phj_mem_stats": ([C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "phj_placement_stats": ([C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "phj_reset_peak": ([C.c_void_p], C.c_int),
     "phj_set_mem_limit": ([C.c_void_p, C.c_uint64], C.c_int),
     "phj_int_hash64_batch": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int], C.c_int),
@@ -663,6 +664,12 @@ class Context:
         cur, peak = C.c_uint64(), C.c_uint64()
         _raise(self._p, load_library().phj_mem_stats(self._p, C.byref(cur), C.byref(peak)))
         return int(cur.value), int(peak.value)
+
+    def placement_stats(self):
+        """(blocks marked slow by the scatter placement watch, bytes marked now)."""
+        n, b = C.c_uint64(), C.c_uint64()
+        _raise(self._p, load_library().phj_placement_stats(self._p, C.byref(n), C.byref(b)))
+        return int(n.value), int(b.value)
 
     def synchronize(self):
         _raise(self._p, load_library().phj_synchronize(self._p))

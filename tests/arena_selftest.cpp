@@ -13,7 +13,10 @@
 //      free bytes pass the cache limit, and only after the pool's stream
 //      has been synchronised;
 //   5. a memory limit refuses an allocation without changing what is held;
-//   6. freeing a pointer the pool does not own is refused.
+//   6. freeing a pointer the pool does not own is refused;
+//   7. alloc_avoiding() places a block clear of the ranges marked slow (in
+//      the unmarked part of a free block, else in a new segment), alloc()
+//      still uses them, and a range is forgotten with its segment.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -145,6 +148,34 @@ int main() {
         const size_t frees = count("free");
         a.free(hold);
         expect(count("free") > frees, "free bytes past the cache limit go back to the backend");
+    }
+    {
+        // rule 7
+        Pool c;
+        c.bind(9);
+        char *seg = static_cast<char *>(c.alloc(1024 * MiB)); // one 1 GiB segment
+        c.free(seg);
+        expect(c.mark_slow(seg + 100 * MiB, 200 * MiB), "a range is marked slow");
+        expect(!c.mark_slow(seg + 150 * MiB, 10 * MiB), "a range inside a marked one is not marked again");
+        const size_t allocs = count("alloc ");
+        char *p = static_cast<char *>(c.alloc_avoiding(50 * MiB));
+        expect(p == seg, "a block that fits in front of the slow range starts the segment");
+        char *q = static_cast<char *>(c.alloc_avoiding(150 * MiB));
+        expect(q == seg + 300 * MiB, "a block that does not fit in front goes past the slow range");
+        expect(count("alloc ") == allocs, "both fit the cached segment without a backend allocation");
+        char *r = static_cast<char *>(c.alloc_avoiding(700 * MiB));
+        expect(count("alloc ") == allocs + 1 && (r < seg || r >= seg + 1024 * MiB),
+               "no clear place left: a new segment");
+        char *u = static_cast<char *>(c.alloc(40 * MiB));
+        expect(u >= seg + 50 * MiB && u < seg + 300 * MiB, "alloc() uses the slow range as usual");
+        for (char *x : {p, q, r, u})
+            c.free(x);
+        expect(c.slow_ranges() == 1, "the range stays marked while its segment is held");
+        c.release_cached();
+        expect(c.slow_ranges() == 0 && c.slow_bytes() == 0, "a range is forgotten with its segment");
+        char *v = static_cast<char *>(c.alloc_avoiding(64 * MiB));
+        expect(v != nullptr, "with no range marked, alloc_avoiding() is alloc()");
+        c.free(v);
     }
     expect(count("free") == count("alloc "), "every segment is returned when the pools are destroyed");
     if (failures == 0)
