@@ -606,8 +606,16 @@ Partitioned radix_partition(phj_ctx *ctx, const DevRel &in, const uint32_t *in_o
                 w.a = w.b = nullptr;
             }
         }
-        radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, uniform_bits(shift, bits), nseg, rs,
-                   flags, ctx->tm(), dig);
+        try {
+            radix_pass(ctx->stream, cur_keys, cur_cols, nkeys, ncols, cur_seg, cur_nseg, rows, uniform_bits(shift, bits), nseg,
+                       rs, flags, ctx->tm(), dig);
+        } catch (...) {
+            if (w.a != nullptr) {
+                ctx->watch_events.push_back(w.a);
+                ctx->watch_events.push_back(w.b);
+            }
+            throw;
+        }
         if (w.a != nullptr) {
             if (hipEventRecord(w.b, ctx->stream) == hipSuccess) {
                 ctx->watch_open.push_back(w);

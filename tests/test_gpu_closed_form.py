@@ -67,6 +67,30 @@ def test_config2_full_size():
     assert abs(want[0] - (1 << 29)) < (1 << 17)
 
 
+def test_placement_watch_keeps_results():
+    """Radix passes over 2^26 rows or more are timed, and a pass much slower than
+    the best of its shape steers later pass outputs off its output block. Joins
+    repeated in one context stay equal to the closed form whatever the watch
+    marks, and what it reports is bounded by the blocks it could have marked."""
+    ctx = G.Context(0)
+    assert ctx.placement_stats() == (0, 0)
+    spec = G.gen_spec(seed=5, build_rows=1 << 24, probe_rows=1 << 26, match_rate=0.5, build_types=(G.U32,),
+                      probe_types=(G.U64, G.U128))
+    b, p = ctx.generate(spec)
+    want = ctx.expected_checksum(spec)
+    got = []
+    for _ in range(5):
+        r = ctx.runPHJ(b, p, G.RadixConfig([8, 8]))
+        got.append((r.out_rows, r.checksum()))
+        r.release()
+    blocks, marked = ctx.placement_stats()
+    b.release()
+    p.release()
+    ctx.close()
+    assert got == [(want[0], want)] * 5
+    assert 0 <= marked <= blocks * (16 << 26)
+
+
 def test_config3_full_size():
     """2^22 distinct keys x Zipf(1.2) duplicates on [1,64] (u8+u16+u64) against
     2^27 probes, 90% Zipf(0.8)-ranked (u64): all three schemes, PHJ with a
