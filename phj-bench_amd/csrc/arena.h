@@ -55,7 +55,7 @@ public:
         uint64_t free_mem = 0, total = 0;
         if (Dev::mem_info(free_mem, total)) {
             cache_limit_ = total / 10 * 6;
-            slow_limit_ = total / 5;
+            slow_limit_ = total / 8;
         }
     }
 
@@ -91,7 +91,7 @@ public:
     }
 
     // Marks [p, p + bytes) slow. Returns false (and marks nothing) when the
-    // range is already marked or the marked bytes would pass a fifth of the
+    // range is already marked or the marked bytes would pass an eighth of the
     // device's memory.
     bool mark_slow(const void *p, uint64_t bytes) {
         char *a = static_cast<char *>(const_cast<void *>(p));
