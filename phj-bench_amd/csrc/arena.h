@@ -11,8 +11,8 @@
 // stream's queued work (which may still use them) has finished.
 //
 // Ranges marked slow (mark_slow) are avoided by alloc_avoiding(), which places
-// a block outside them (in a free block's unmarked part, else in a new
-// segment); alloc() uses them as usual. A range is forgotten when its segment
+// a block outside them in a free block's unmarked part when one has room (else
+// it is alloc()); alloc() uses them as usual. A range is forgotten when its segment
 // goes back to the backend.
 //
 // Backend interface (static members): Stream; void *alloc(uint64_t) (null on
@@ -76,8 +76,10 @@ public:
         return commit(b);
     }
 
-    // alloc() outside the slow ranges; falls back to alloc() when neither a
-    // free block nor a new segment gives such a place.
+    // alloc() outside the slow ranges, in memory the pool already holds;
+    // falls back to alloc() when no free block has such a place. It never
+    // takes a new segment for the sake of avoidance: a device allocation of
+    // many GB inside a caller's loop costs far more than a slow placement.
     void *alloc_avoiding(uint64_t bytes) {
         if (slow_.empty())
             return alloc(bytes);
@@ -85,8 +87,6 @@ public:
         if (limit_ != 0 && current_ + sz > limit_)
             return alloc(bytes); // throws
         Block *b = take_avoiding(sz);
-        if (b == nullptr && grow_segment(sz))
-            b = take_avoiding(sz);
         return b != nullptr ? commit(b) : alloc(bytes);
     }
 
@@ -227,18 +227,16 @@ private:
         return b;
     }
 
-    Block *grow(uint64_t sz) { return grow_segment(sz) ? take(sz) : nullptr; }
-
-    bool grow_segment(uint64_t sz) {
+    Block *grow(uint64_t sz) {
         const uint64_t seg = std::max(sz, kSegmentBytes);
         char *p = static_cast<char *>(Dev::alloc(seg));
         if (p == nullptr)
-            return false;
+            return nullptr;
         forget_slow(p, p + seg); // an address range the backend handed out again
         segments_[p] = seg;
         free_bytes_ += seg;
         free_.emplace(seg, new Block{p, seg, true, nullptr, nullptr});
-        return true;
+        return take(sz);
     }
 
     void *commit(Block *b) {

@@ -14,9 +14,10 @@
 //      has been synchronised;
 //   5. a memory limit refuses an allocation without changing what is held;
 //   6. freeing a pointer the pool does not own is refused;
-//   7. alloc_avoiding() places a block clear of the ranges marked slow (in
-//      the unmarked part of a free block, else in a new segment), alloc()
-//      still uses them, and a range is forgotten with its segment.
+//   7. alloc_avoiding() places a block clear of the ranges marked slow in
+//      the unmarked part of a free block, takes no new segment for that (it
+//      is alloc() when no free block has room), alloc() still uses the
+//      ranges, and a range is forgotten with its segment.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -163,9 +164,14 @@ int main() {
         char *q = static_cast<char *>(c.alloc_avoiding(150 * MiB));
         expect(q == seg + 300 * MiB, "a block that does not fit in front goes past the slow range");
         expect(count("alloc ") == allocs, "both fit the cached segment without a backend allocation");
-        char *r = static_cast<char *>(c.alloc_avoiding(700 * MiB));
-        expect(count("alloc ") == allocs + 1 && (r < seg || r >= seg + 1024 * MiB),
-               "no clear place left: a new segment");
+        char *r = static_cast<char *>(c.alloc_avoiding(100 * MiB));
+        expect(r == seg + 450 * MiB, "the rest of the segment past the slow range is used");
+        char *rest = static_cast<char *>(c.alloc(474 * MiB)); // the segment's tail
+        char *t = static_cast<char *>(c.alloc_avoiding(200 * MiB));
+        expect(rest == seg + 550 * MiB && t == seg + 50 * MiB && count("alloc ") == allocs,
+               "no clear place left: an ordinary allocation over the slow range, without a new segment");
+        c.free(t);
+        c.free(rest);
         char *u = static_cast<char *>(c.alloc(40 * MiB));
         expect(u >= seg + 50 * MiB && u < seg + 300 * MiB, "alloc() uses the slow range as usual");
         for (char *x : {p, q, r, u})
