@@ -218,10 +218,11 @@ phj_status phj_reset_peak(phj_ctx *ctx);
 /* Scatter placement watch (no reference counterpart): radix passes over at
  * least 2^26 rows are timed, and a pass more than 15% slower than the fastest
  * of its shape marks its largest output block slow in the context's pool;
- * later pass outputs are placed clear of marked blocks (other allocations use
- * them as usual). Reports how many blocks were marked and the bytes currently
- * marked (a marked range is forgotten when its memory goes back to the
- * device). Either pointer may be null. */
+ * later pass outputs are placed clear of marked blocks where the pool already
+ * holds room for them (no device allocation is made for that; other
+ * allocations use marked blocks as usual). Reports how many blocks were marked
+ * and the bytes currently marked (a marked range is forgotten when its memory
+ * goes back to the device). Either pointer may be null. */
 phj_status phj_placement_stats(phj_ctx *ctx, uint64_t *marked_blocks, uint64_t *marked_bytes);
 /* Upper bound on the device bytes the context's live allocations may hold
  * (0, the default: none). An allocation beyond it fails with
